@@ -1,0 +1,151 @@
+/*
+ * ttm_amd.h -- C ABI of the MI355X (gfx950) mode-q tensor-times-matrix product
+ *
+ *     C(i1..i(q-1), j, i(q+1)..ip) = sum_iq  A(i1..iq..ip) * B(j, iq)
+ *
+ * Every entry point takes plain pointers and sizes.  Extents, strides and
+ * layouts are one-based-layout size_t arrays of length p exactly as in the
+ * reference's low-level call; B is m x n_q (nb = {m, n_q}) and is
+ * column-major when pib[0] == 1, row-major otherwise.  C has A's layout.
+ *
+ * The arrays A, B, C must be dense in their layout (the reference requires
+ * contiguous tensors: README.md:40); the stride vectors are validated exactly
+ * as the reference validates them.
+ *
+ * Two residencies:
+ *   ttm_?ttm*       device pointers; the call is asynchronous and ordered on
+ *                   `stream` (a hipStream_t, 0 = the null stream).
+ *   ttm_?ttm_host*  host pointers; the call stages through device memory and
+ *                   returns when C is complete (the reference's semantics).
+ *
+ * Errors: every entry returns a ttm_status_t.  Statuses 1..18 correspond
+ * one-to-one to the reference's validation errors (include/tlib/ttm.h:65-90),
+ * checked in the same order, and ttm_last_error() returns the reference's
+ * message text for them (thread-local, valid until the next call on the
+ * same thread).
+ */
+#ifndef TTM_AMD_H
+#define TTM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hipStream_t, passed opaquely so that the header needs no HIP include. */
+typedef void *ttm_stream_t;
+
+typedef enum ttm_status_t {
+    TTM_SUCCESS = 0,
+    /* include/tlib/ttm.h:65-90, same order and meaning */
+    TTM_ERR_ORDER_ZERO = 1,     /* p == 0 */
+    TTM_ERR_MODE = 2,           /* q == 0 || q > p */
+    TTM_ERR_NULL_A = 3,
+    TTM_ERR_NULL_B = 4,
+    TTM_ERR_NULL_C = 5,
+    TTM_ERR_NULL_NA = 6,
+    TTM_ERR_NULL_NB = 7,
+    TTM_ERR_NULL_NC = 8,
+    TTM_ERR_NULL_WA = 9,
+    TTM_ERR_NULL_WC = 10,
+    TTM_ERR_NULL_PIA = 11,
+    TTM_ERR_CONTRACTION_DIM = 12, /* na[q-1] != nb[1] */
+    TTM_ERR_FREE_DIM = 13,        /* nc[q-1] != nb[0] */
+    TTM_ERR_SHAPE_A = 14,
+    TTM_ERR_SHAPE_C = 15,
+    TTM_ERR_LAYOUT_A = 16,
+    TTM_ERR_STRIDES_A = 17,
+    TTM_ERR_STRIDES_C = 18,
+    /* conditions the reference does not check */
+    TTM_ERR_NULL_PIB = 19,        /* the reference dereferences pib unchecked */
+    TTM_ERR_NOT_DENSE = 20,       /* valid but non-dense strides */
+    TTM_ERR_SHAPE_C_MISMATCH = 21,/* nc differs from na outside mode q */
+    TTM_ERR_PRECISION = 22,       /* precision not available for this element type */
+    TTM_ERR_HIP = 23,             /* a HIP runtime call failed */
+    TTM_ERR_ALLOC = 24,           /* device allocation failed */
+    TTM_ERR_ARGUMENT = 25         /* other invalid argument (batched descriptors) */
+} ttm_status_t;
+
+/* Arithmetic used for a float32 product.  TTM_PREC_DEFAULT is the plain
+ * FP32/FP64 drop-in.  The others take float32 inputs and return float32
+ * results, accumulating in FP32 on the matrix cores. */
+typedef enum ttm_precision_t {
+    TTM_PREC_DEFAULT = 0,   /* FP32 in, FP32 MFMA/FMA arithmetic (FP64 for double) */
+    TTM_PREC_BF16X3 = 1,    /* three-term bfloat16 split, 6 products */
+    TTM_PREC_BF16 = 2,      /* inputs rounded to bfloat16 */
+    TTM_PREC_FP16 = 3,      /* inputs rounded to binary16 */
+    TTM_PREC_FP8_E4M3 = 4   /* OCP e4m3fn with per-slice power-of-two scaling */
+} ttm_precision_t;
+
+/* --- device pointers, stream ordered ---------------------------------- */
+/* Replaces tlib::ttm::ttm(ep, sp, fp, q, p, a, na, wa, pia, b, nb, pib, c, nc, wc)
+ * of include/tlib/ttm.h:54-93 for value_t = float / double (the policy tags
+ * select CPU threading strategies and have no GPU counterpart). */
+ttm_status_t ttm_sttm(unsigned q, unsigned p,
+                      const float *a, const size_t *na, const size_t *wa, const size_t *pia,
+                      const float *b, const size_t *nb, const size_t *pib,
+                      float *c, const size_t *nc, const size_t *wc,
+                      ttm_stream_t stream);
+
+ttm_status_t ttm_dttm(unsigned q, unsigned p,
+                      const double *a, const size_t *na, const size_t *wa, const size_t *pia,
+                      const double *b, const size_t *nb, const size_t *pib,
+                      double *c, const size_t *nc, const size_t *wc,
+                      ttm_stream_t stream);
+
+/* The float32 product with a selectable arithmetic (no reference
+ * counterpart: the reference is float/double only, README.md:31,41). */
+ttm_status_t ttm_sttm_ex(unsigned q, unsigned p,
+                         const float *a, const size_t *na, const size_t *wa, const size_t *pia,
+                         const float *b, const size_t *nb, const size_t *pib,
+                         float *c, const size_t *nc, const size_t *wc,
+                         ttm_precision_t precision, ttm_stream_t stream);
+
+/* --- host pointers, synchronous ---------------------------------------- */
+/* Same contract as include/tlib/ttm.h:54-93 with caller-owned host buffers:
+ * C is complete when the call returns. */
+ttm_status_t ttm_sttm_host(unsigned q, unsigned p,
+                           const float *a, const size_t *na, const size_t *wa, const size_t *pia,
+                           const float *b, const size_t *nb, const size_t *pib,
+                           float *c, const size_t *nc, const size_t *wc);
+
+ttm_status_t ttm_dttm_host(unsigned q, unsigned p,
+                           const double *a, const size_t *na, const size_t *wa, const size_t *pia,
+                           const double *b, const size_t *nb, const size_t *pib,
+                           double *c, const size_t *nc, const size_t *wc);
+
+ttm_status_t ttm_sttm_host_ex(unsigned q, unsigned p,
+                              const float *a, const size_t *na, const size_t *wa,
+                              const size_t *pia, const float *b, const size_t *nb,
+                              const size_t *pib, float *c, const size_t *nc, const size_t *wc,
+                              ttm_precision_t precision);
+
+/* --- diagnostics -------------------------------------------------------- */
+/* Message of the last failing call on this thread ("" after a success). */
+const char *ttm_last_error(void);
+/* Fixed message for a status (the reference's text for 1..18). */
+const char *ttm_status_message(ttm_status_t status);
+/* 1 if `ptr` is device (or managed) memory visible to the current device. */
+int ttm_is_device_pointer(const void *ptr);
+/* Name of the kernel the planner picks for a problem, without running it
+ * (for tests and the bench); returns a static string or "" on invalid input. */
+const char *ttm_plan_kernel(unsigned q, unsigned p, int is_double, const size_t *na,
+                            const size_t *pia, const size_t *nb, const size_t *pib,
+                            ttm_precision_t precision);
+/* Library version string. */
+const char *ttm_version(void);
+
+/* --- testing hook --------------------------------------------------------- */
+/* Force a kernel family by name for every later call of this process
+ * ("generic", "nn_stream", "tn_stream", ...; "" or NULL restores the
+ * planner's choice).  A forced family that cannot run a problem is ignored
+ * for it.  Returns 0 for a known name, -1 otherwise.  Not thread-safe. */
+int ttm_force_kernel(const char *name);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* TTM_AMD_H */

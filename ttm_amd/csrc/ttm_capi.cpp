@@ -1,0 +1,221 @@
+// extern "C" entry points declared in include/ttm_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <type_traits>
+
+#include "ttm_amd.h"
+#include "ttm_kernels.hpp"
+#include "ttm_plan.hpp"
+
+using namespace ttm_amd;
+
+namespace
+{
+
+Kernel g_force = Kernel::none; // test hook, see ttm_force_kernel
+
+bool is_device_ptr(const void *p)
+{
+    if (!p) return false;
+    hipPointerAttribute_t attr;
+    std::memset(&attr, 0, sizeof attr);
+    hipError_t const e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError(); // unregistered host memory reports an error; clear it
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+ttm_status_t hip_fail(hipError_t e)
+{
+    set_error(TTM_ERR_HIP, hipGetErrorString(e));
+    return TTM_ERR_HIP;
+}
+
+// Device pointer path shared by the typed entries.
+template <class T>
+ttm_status_t run_device(unsigned q, unsigned p, const T *a, const size_t *na, const size_t *wa,
+                        const size_t *pia, const T *b, const size_t *nb, const size_t *pib, T *c,
+                        const size_t *nc, const size_t *wc, ttm_precision_t prec, hipStream_t s)
+{
+    Fold f;
+    ttm_status_t st = plan(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, f);
+    if (st != TTM_SUCCESS) return st;
+    Kernel used = Kernel::none;
+    if constexpr (std::is_same_v<T, float>) {
+        st = launch_f32(f, a, b, c, prec, s, g_force, &used);
+    } else {
+        if (prec != TTM_PREC_DEFAULT) {
+            set_error(TTM_ERR_PRECISION);
+            return TTM_ERR_PRECISION;
+        }
+        st = launch_f64(f, a, b, c, s, g_force, &used);
+    }
+    if (st == TTM_SUCCESS) clear_error();
+    return st;
+}
+
+// Host (or mixed) pointer path: validates first, stages every buffer that is
+// not device memory, runs on the null stream and waits for C.
+template <class T>
+ttm_status_t run_host(unsigned q, unsigned p, const T *a, const size_t *na, const size_t *wa,
+                      const size_t *pia, const T *b, const size_t *nb, const size_t *pib, T *c,
+                      const size_t *nc, const size_t *wc, ttm_precision_t prec)
+{
+    Fold f;
+    ttm_status_t st = plan(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, f);
+    if (st != TTM_SUCCESS) return st;
+    if constexpr (std::is_same_v<T, double>) {
+        if (prec != TTM_PREC_DEFAULT) {
+            set_error(TTM_ERR_PRECISION);
+            return TTM_ERR_PRECISION;
+        }
+    }
+
+    size_t const bytes_a = f.numel_a() * sizeof(T);
+    size_t const bytes_b = f.numel_b() * sizeof(T);
+    size_t const bytes_c = f.numel_c() * sizeof(T);
+    bool const a_dev = is_device_ptr(a), b_dev = is_device_ptr(b), c_dev = is_device_ptr(c);
+
+    T *da = nullptr, *db = nullptr, *dc = nullptr;
+    auto release = [&]() {
+        if (!a_dev && da) (void)hipFree(da);
+        if (!b_dev && db) (void)hipFree(db);
+        if (!c_dev && dc) (void)hipFree(dc);
+    };
+    auto alloc = [&](T **dst, size_t bytes) -> bool {
+        if (hipMalloc(reinterpret_cast<void **>(dst), bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            *dst = nullptr;
+            return false;
+        }
+        return true;
+    };
+    if ((!a_dev && !alloc(&da, bytes_a)) || (!b_dev && !alloc(&db, bytes_b)) ||
+        (!c_dev && !alloc(&dc, bytes_c))) {
+        release();
+        set_error(TTM_ERR_ALLOC);
+        return TTM_ERR_ALLOC;
+    }
+    if (a_dev) da = const_cast<T *>(a);
+    if (b_dev) db = const_cast<T *>(b);
+    if (c_dev) dc = c;
+
+    hipError_t e = hipSuccess;
+    if (!a_dev) e = hipMemcpy(da, a, bytes_a, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !b_dev) e = hipMemcpy(db, b, bytes_b, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        release();
+        return hip_fail(e);
+    }
+    Kernel used = Kernel::none;
+    if constexpr (std::is_same_v<T, float>)
+        st = launch_f32(f, da, db, dc, prec, nullptr, g_force, &used);
+    else
+        st = launch_f64(f, da, db, dc, nullptr, g_force, &used);
+    if (st == TTM_SUCCESS) {
+        e = c_dev ? hipDeviceSynchronize() : hipMemcpy(c, dc, bytes_c, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = hip_fail(e);
+    }
+    release();
+    if (st == TTM_SUCCESS) clear_error();
+    return st;
+}
+
+} // namespace
+
+extern "C" {
+
+ttm_status_t ttm_sttm(unsigned q, unsigned p, const float *a, const size_t *na, const size_t *wa,
+                      const size_t *pia, const float *b, const size_t *nb, const size_t *pib, float *c,
+                      const size_t *nc, const size_t *wc, ttm_stream_t stream)
+{
+    return run_device<float>(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, TTM_PREC_DEFAULT,
+                             static_cast<hipStream_t>(stream));
+}
+
+ttm_status_t ttm_dttm(unsigned q, unsigned p, const double *a, const size_t *na, const size_t *wa,
+                      const size_t *pia, const double *b, const size_t *nb, const size_t *pib, double *c,
+                      const size_t *nc, const size_t *wc, ttm_stream_t stream)
+{
+    return run_device<double>(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, TTM_PREC_DEFAULT,
+                              static_cast<hipStream_t>(stream));
+}
+
+ttm_status_t ttm_sttm_ex(unsigned q, unsigned p, const float *a, const size_t *na, const size_t *wa,
+                         const size_t *pia, const float *b, const size_t *nb, const size_t *pib, float *c,
+                         const size_t *nc, const size_t *wc, ttm_precision_t precision, ttm_stream_t stream)
+{
+    return run_device<float>(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, precision,
+                             static_cast<hipStream_t>(stream));
+}
+
+ttm_status_t ttm_sttm_host(unsigned q, unsigned p, const float *a, const size_t *na, const size_t *wa,
+                           const size_t *pia, const float *b, const size_t *nb, const size_t *pib, float *c,
+                           const size_t *nc, const size_t *wc)
+{
+    return run_host<float>(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, TTM_PREC_DEFAULT);
+}
+
+ttm_status_t ttm_dttm_host(unsigned q, unsigned p, const double *a, const size_t *na, const size_t *wa,
+                           const size_t *pia, const double *b, const size_t *nb, const size_t *pib,
+                           double *c, const size_t *nc, const size_t *wc)
+{
+    return run_host<double>(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, TTM_PREC_DEFAULT);
+}
+
+ttm_status_t ttm_sttm_host_ex(unsigned q, unsigned p, const float *a, const size_t *na, const size_t *wa,
+                              const size_t *pia, const float *b, const size_t *nb, const size_t *pib,
+                              float *c, const size_t *nc, const size_t *wc, ttm_precision_t precision)
+{
+    return run_host<float>(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, precision);
+}
+
+const char *ttm_last_error(void) { return last_error(); }
+
+const char *ttm_status_message(ttm_status_t status) { return status_text(status); }
+
+int ttm_is_device_pointer(const void *ptr) { return is_device_ptr(ptr) ? 1 : 0; }
+
+const char *ttm_plan_kernel(unsigned q, unsigned p, int is_double, const size_t *na, const size_t *pia,
+                            const size_t *nb, const size_t *pib, ttm_precision_t precision)
+{
+    if (!na || !pia || !nb || !pib || p == 0 || p > 64 || q == 0 || q > p) return "";
+    size_t nc[64], wa[64], wc[64];
+    for (unsigned r = 0; r < p; ++r) nc[r] = na[r];
+    nc[q - 1] = nb[0];
+    // dense strides of the layout
+    size_t sa = 1, sc = 1;
+    for (unsigned r = 0; r < p; ++r) {
+        if (pia[r] == 0 || pia[r] > p) return "";
+        wa[pia[r] - 1] = sa;
+        wc[pia[r] - 1] = sc;
+        sa *= na[pia[r] - 1];
+        sc *= nc[pia[r] - 1];
+    }
+    static const char dummy = 0;
+    Fold f;
+    if (plan(q, p, &dummy, na, wa, pia, &dummy, nb, pib, &dummy, nc, wc, f) != TTM_SUCCESS) return "";
+    return kernel_name(choose_kernel(f, is_double != 0, precision, true, g_force));
+}
+
+const char *ttm_version(void) { return "ttm_amd 0.1.0 (gfx950)"; }
+
+// Testing hook (not part of the public header's contract): force a kernel
+// family by name ("" restores the planner's choice).  Returns 0 when the
+// name is known.
+int ttm_force_kernel(const char *name)
+{
+    if (!name || !*name) {
+        g_force = Kernel::none;
+        return 0;
+    }
+    Kernel const k = kernel_from_name(name);
+    if (k == Kernel::none) return -1;
+    g_force = k;
+    return 0;
+}
+
+} // extern "C"

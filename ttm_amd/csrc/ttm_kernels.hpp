@@ -1,0 +1,36 @@
+// Kernel families and the launcher interface shared by the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ttm_amd.h"
+#include "ttm_plan.hpp"
+
+namespace ttm_amd
+{
+
+enum class Kernel : int
+{
+    none = 0,
+    generic,    // one thread per C element, any shape / alignment
+    nn_stream,  // MFMA 16x16x4, w % vec == 0: streams A slabs along the contiguous block
+    tn_stream,  // MFMA 16x16x4, w == 1: rows of A contiguous along the contraction mode
+    count
+};
+
+const char *kernel_name(Kernel k);
+Kernel kernel_from_name(const char *name);
+
+// Kernel the planner picks for a fold (aligned: all base pointers 16-byte
+// aligned).  `force` (may be Kernel::none) overrides when applicable.
+Kernel choose_kernel(const Fold &f, bool is_double, ttm_precision_t prec, bool aligned, Kernel force);
+
+// Launch the product on `stream`; returns TTM_SUCCESS or TTM_ERR_HIP.
+ttm_status_t launch_f32(const Fold &f, const float *a, const float *b, float *c, ttm_precision_t prec,
+                        hipStream_t stream, Kernel force, Kernel *used);
+ttm_status_t launch_f64(const Fold &f, const double *a, const double *b, double *c, hipStream_t stream,
+                        Kernel force, Kernel *used);
+
+} // namespace ttm_amd
