@@ -1,0 +1,279 @@
+#!/usr/bin/env python3
+"""Benchmark of the mode-q TTM on one MI355X.
+
+Workload (one "step"): the plain FP32/FP64 drop-in at scale -- an order-3
+tensor A of extents 1024 x 1024 x 512 in float32 and in float64, first-order
+(1,2,3) and last-order (3,2,1) layouts, contraction mode q in {1,2,3}, B of
+shape m x n_q with m in {16, 1024}, entries uniform[-1,1]: 24 products, all
+inputs resident in HBM before timing starts.
+
+Printed JSON line:
+  value      aggregate TFLOP/s over the step (algorithmic 2*m*N_A flops)
+  roofline   the dominant kernel (largest share of step time): algorithmic
+             work / its launch time against the MI355X peak for its bound
+  cpu_baseline  the oracle's restatement of the reference algorithm (OpenMP
+             loop over BLAS GEMMs, `combined` dispatch) timed on the host on a
+             bounded sample of the same workload
+  cases      per-product time, TFLOP/s, GB/s and roofline fraction
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM = 8.0e12                      # B/s
+PEAK_FLOPS = {"f32": 157.3e12, "f64": 78.6e12, "bf16": 2516.6e12}
+EXTENTS = [1024, 1024, 512]
+LAYOUTS = {"first": [1, 2, 3], "last": [3, 2, 1]}
+MS = [16, 1024]
+METRIC = "TTM TFLOP/s + HBM GB/s vs roofline; BF16x3 vs FP32 at equal FP64-oracle error"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--dtypes", default="f32,f64")
+    ap.add_argument("--only", default="", help="comma list of case labels to run (profiling)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    return ap.parse_args()
+
+
+def case_list(dtypes):
+    out = []
+    for dt in dtypes:
+        for lay, pia in LAYOUTS.items():
+            for q in (1, 2, 3):
+                for m in MS:
+                    out.append(dict(dtype=dt, layout=lay, pia=pia, q=q, m=m,
+                                    label=f"{dt}-{lay}-q{q}-m{m}"))
+    return out
+
+
+def work(na, q, m, itemsize):
+    n_a = int(np.prod(na))
+    n_c = n_a // na[q - 1] * m
+    flops = 2.0 * m * n_a
+    bytes_ = itemsize * (n_a + m * na[q - 1] + n_c)
+    return flops, bytes_
+
+
+def gpu_bench(args, cases):
+    import torch
+    import ttm_amd
+
+    dev = torch.device("cuda:0")
+    tdt = {"f32": torch.float32, "f64": torch.float64}
+    n_a = int(np.prod(EXTENTS))
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0x77140002)
+    A = {}
+    Cbuf = {}
+    for dt in {c["dtype"] for c in cases}:
+        A[dt] = torch.empty(n_a, dtype=tdt[dt], device=dev).uniform_(-1, 1, generator=gen)
+        max_c = max(n_a // EXTENTS[c["q"] - 1] * c["m"] for c in cases if c["dtype"] == dt)
+        Cbuf[dt] = torch.empty(max_c, dtype=tdt[dt], device=dev)
+    for c in cases:
+        nq = EXTENTS[c["q"] - 1]
+        c["B"] = torch.empty((c["m"], nq), dtype=tdt[c["dtype"]], device=dev).uniform_(-1, 1, generator=gen)
+        c["na"] = list(EXTENTS)
+        c["wa"] = ttm_amd.dense_strides(EXTENTS, c["pia"])
+        c["nc"] = list(EXTENTS)
+        c["nc"][c["q"] - 1] = c["m"]
+        c["wc"] = ttm_amd.dense_strides(c["nc"], c["pia"])
+        c["kernel"] = ttm_amd.plan_kernel(c["q"], EXTENTS, c["pia"], [c["m"], nq], [2, 1],
+                                          is_double=c["dtype"] == "f64")
+        c["flops"], c["bytes"] = work(EXTENTS, c["q"], c["m"], 4 if c["dtype"] == "f32" else 8)
+    torch.cuda.synchronize()
+
+    def run(c):
+        ttm_amd.ttm_raw(c["q"], 3, A[c["dtype"]], c["na"], c["wa"], c["pia"], c["B"], [c["m"], EXTENTS[c["q"] - 1]],
+                        [2, 1], Cbuf[c["dtype"]], c["nc"], c["wc"])
+
+    for _ in range(args.warmup):
+        for c in cases:
+            run(c)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    events = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in cases]
+              for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        for c, (e0, e1) in zip(cases, events[s]):
+            e0.record(stream)
+            run(c)
+            e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    for i, c in enumerate(cases):
+        ms = [events[s][i][0].elapsed_time(events[s][i][1]) for s in range(args.steps)]
+        c["ms"] = float(np.mean(ms))
+        c["ms_min"] = float(np.min(ms))
+        t = c["ms"] * 1e-3
+        c["tflops"] = c["flops"] / t / 1e12
+        c["gbps"] = c["bytes"] / t / 1e9
+        t_hbm = c["bytes"] / PEAK_HBM
+        t_cmp = c["flops"] / PEAK_FLOPS[c["dtype"]]
+        c["bound"] = "hbm" if t_hbm >= t_cmp else "mfma"
+        c["frac"] = max(t_hbm, t_cmp) / t
+    del A, Cbuf
+    torch.cuda.empty_cache()
+    return wall / args.steps
+
+
+def cpu_baseline(args, budget_s):
+    """The oracle's `combined` restatement over OpenMP + a CBLAS, on the
+    same 24-case sweep with every extent divided by 4 (A 256 x 256 x 128)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+    if not orc.LIB.exists():
+        orc.build(with_ref=False)
+    blas = orc.bind_blas()
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        avail = os.cpu_count() or 1
+    threads = max(1, min(orc.cores(), avail, args.cpu_threads))
+    ext = [256, 256, 128]
+    rng = np.random.default_rng(1)
+    data = {}
+    for dt, npdt in (("f32", np.float32), ("f64", np.float64)):
+        data[dt] = rng.uniform(-1, 1, size=int(np.prod(ext))).astype(npdt)
+    cases = case_list(["f32", "f64"])
+    prepared = []
+    for c in cases:
+        nq = ext[c["q"] - 1]
+        b = rng.uniform(-1, 1, size=c["m"] * nq).astype(data[c["dtype"]].dtype)
+        nc = list(ext)
+        nc[c["q"] - 1] = c["m"]
+        prepared.append((c, b, nc, orc_strides(ext, c["pia"]), orc_strides(nc, c["pia"])))
+    flops_sweep = sum(2.0 * c["m"] * int(np.prod(ext)) for c, *_ in prepared)
+
+    def sweep():
+        branches = []
+        for c, b, nc, wa, wc in prepared:
+            _, br = orc.ttm_reference(c["q"], data[c["dtype"]], ext, wa, c["pia"], b, [c["m"], ext[c["q"] - 1]],
+                                      [2, 1], nc, wc, use_blas=True, cores=threads)
+            branches.append(br)
+        return branches
+
+    branches = sweep()  # warm-up
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        sweep()
+        n += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = (time.perf_counter() - t0) / n
+    return {
+        "value": flops_sweep / dt / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
+        "sample": (f"the same 24-product sweep (f32+f64, first/last-order, q=1..3, m in {{16,1024}}) on A "
+                   f"256x256x128 (1/64 of the volume), oracle restatement of tlib::ttm `combined` "
+                   f"(OpenMP x {threads} threads + {os.path.basename(blas)}), mean of {n} sweeps of {dt:.2f} s"),
+        "blas": os.path.basename(blas), "branches": sorted(set(branches)),
+    }
+
+
+def orc_strides(n, pi):
+    w = [0] * len(n)
+    s = 1
+    for mode in pi:
+        w[mode - 1] = s
+        s *= n[mode - 1]
+    return w
+
+
+def kernel_symbol(c):
+    """Kernel template instance a case launches (matches the rocprof kernel name)."""
+    t = "float" if c["dtype"] == "f32" else "double"
+    jt = 1 if c["m"] <= 16 else (2 if c["m"] <= 32 else 4)
+    fam = c["kernel"]
+    if fam == "tn_stream":
+        return f"k_tn_stream<{t}, {jt}, true>"
+    if fam == "nn_stream":
+        return f"k_nn_stream<{t}, {jt}>"
+    return f"k_{fam}<{t}>"
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    cases = case_list([d.strip() for d in args.dtypes.split(",") if d.strip()])
+    if args.only:
+        keep = set(args.only.split(","))
+        cases = [c for c in cases if c["label"] in keep]
+    step_s = gpu_bench(args, cases)
+    total_flops = sum(c["flops"] for c in cases)
+    total_bytes = sum(c["bytes"] for c in cases)
+    kern_time = sum(c["ms"] for c in cases) * 1e-3
+
+    # dominant kernel symbol by total time
+    groups = {}
+    for c in cases:
+        groups.setdefault(kernel_symbol(c), []).append(c)
+    sym, members = max(groups.items(), key=lambda kv: sum(c["ms"] for c in kv[1]))
+    t_sym = sum(c["ms"] for c in members) * 1e-3
+    bounds = {c["bound"] for c in members}
+    bound = "mfma" if bounds == {"mfma"} else "hbm"
+    dt = members[0]["dtype"]
+    if bound == "mfma":
+        achieved = sum(c["flops"] for c in members) / t_sym / 1e12
+        peak, unit = PEAK_FLOPS[dt] / 1e12, "TFLOP/s"
+    else:
+        achieved = sum(c["bytes"] for c in members) / t_sym / 1e9
+        peak, unit = PEAK_HBM / 1e9, "GB/s"
+    roofline = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+                "frac": round(achieved / peak, 4), "traffic": None, "kernel": sym,
+                "launches_per_step": len(members), "avg_launch_ms": round(t_sym * 1e3 / len(members), 4),
+                "cases": [c["label"] for c in members]}
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(args, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": round(total_flops / kern_time / 1e12, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "+".join(sorted({c["dtype"] for c in cases})),
+        "data": "synthetic (uniform[-1,1], fixed seed)",
+        "config": {"workload": "TTM sweep: A 1024x1024x512, layouts first/last-order, q in {1,2,3}, "
+                               "m in {16,1024}, B m x n_q row-major",
+                   "extents": EXTENTS, "layouts": list(LAYOUTS), "q": [1, 2, 3], "m": MS,
+                   "products_per_step": len(cases)},
+        "hbm_gbps": round(total_bytes / kern_time / 1e9, 1),
+        "step_kernel_ms": round(kern_time * 1e3, 3),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "cases": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in c.items()
+                   if k in ("label", "kernel", "ms", "ms_min", "tflops", "gbps", "bound", "frac")}
+                  for c in cases],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
