@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--only", default="", help="comma list of case labels to run (profiling)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--force", default="", help="force a kernel family (comparison runs only)")
     return ap.parse_args()
 
 
@@ -94,6 +95,8 @@ def gpu_bench(args, cases):
         c["wc"] = ttm_amd.dense_strides(c["nc"], c["pia"])
         c["kernel"] = ttm_amd.plan_kernel(c["q"], EXTENTS, c["pia"], [c["m"], nq], [2, 1],
                                           is_double=c["dtype"] == "f64")
+        c["symbol"] = ttm_amd.plan_kernel_symbol(c["q"], EXTENTS, c["pia"], [c["m"], nq], [2, 1],
+                                                 is_double=c["dtype"] == "f64")
         c["flops"], c["bytes"] = work(EXTENTS, c["q"], c["m"], 4 if c["dtype"] == "f32" else 8)
     torch.cuda.synchronize()
 
@@ -196,24 +199,15 @@ def orc_strides(n, pi):
     return w
 
 
-def kernel_symbol(c):
-    """Kernel template instance a case launches (matches the rocprof kernel name)."""
-    t = "float" if c["dtype"] == "f32" else "double"
-    jt = 1 if c["m"] <= 16 else (2 if c["m"] <= 32 else 4)
-    fam = c["kernel"]
-    if fam == "tn_stream":
-        return f"k_tn_stream<{t}, {jt}, true>"
-    if fam == "nn_stream":
-        return f"k_nn_stream<{t}, {jt}>"
-    return f"k_{fam}<{t}>"
-
-
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.force:
+        import ttm_amd
+        ttm_amd.force_kernel(args.force)
     cases = case_list([d.strip() for d in args.dtypes.split(",") if d.strip()])
     if args.only:
         keep = set(args.only.split(","))
@@ -226,7 +220,7 @@ def main():
     # dominant kernel symbol by total time
     groups = {}
     for c in cases:
-        groups.setdefault(kernel_symbol(c), []).append(c)
+        groups.setdefault(c["symbol"], []).append(c)
     sym, members = max(groups.items(), key=lambda kv: sum(c["ms"] for c in kv[1]))
     t_sym = sum(c["ms"] for c in members) * 1e-3
     bounds = {c["bound"] for c in members}
@@ -269,7 +263,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cases": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in c.items()
-                   if k in ("label", "kernel", "ms", "ms_min", "tflops", "gbps", "bound", "frac")}
+                   if k in ("label", "symbol", "ms", "ms_min", "tflops", "gbps", "bound", "frac")}
                   for c in cases],
     }
     print(json.dumps(line), flush=True)

@@ -134,6 +134,11 @@ int ttm_is_device_pointer(const void *ptr);
 const char *ttm_plan_kernel(unsigned q, unsigned p, int is_double, const size_t *na,
                             const size_t *pia, const size_t *nb, const size_t *pib,
                             ttm_precision_t precision);
+/* Template instance of that kernel as profilers name it, e.g.
+ * "k_gemm<float, 4, 4, 2, 16, false, true>" (thread-local string). */
+const char *ttm_plan_kernel_symbol(unsigned q, unsigned p, int is_double, const size_t *na,
+                                   const size_t *pia, const size_t *nb, const size_t *pib,
+                                   ttm_precision_t precision);
 /* Library version string. */
 const char *ttm_version(void);
 

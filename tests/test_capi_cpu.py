@@ -140,10 +140,10 @@ def test_plan_reference_configs(ttmlib):
                 for dbl in (False, True):
                     k = ttm_amd.plan_kernel(q, na, pia, nb, [1, 2], is_double=dbl)
                     assert k, (pia, q, m)
-                    if qh == 1:
-                        assert k.startswith("tn"), (pia, q, m, k)
-                    else:
-                        assert k.startswith("nn"), (pia, q, m, k)
+                    assert k == "gemm", (pia, q, m, k)
+                    sym = ttm_amd.plan_kernel_symbol(q, na, pia, nb, [1, 2], is_double=dbl)
+                    assert sym.startswith("k_gemm<" + ("double" if dbl else "float"))
+                    assert sym.endswith(("true, true>" if qh == 1 else "false, true>")), sym
     # odd shapes fall back to the generic kernel
     assert ttm_amd.plan_kernel(2, [3, 5, 7], [1, 2, 3], [4, 5], [1, 2]) == "generic"
     assert ttm_amd.plan_kernel(1, [3, 5, 7], [1, 2, 3], [4, 3], [1, 2]) == "generic"

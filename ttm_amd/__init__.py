@@ -8,9 +8,10 @@ arrays and ROCm torch tensors:
 * :func:`ttm`     -- tensor form, any dense layout;
 * :mod:`ttm_amd.ttmpy` -- the reference's ``ttmpy`` module (``ttm``, ``ttms``).
 """
-from ._lib import LIB_PATH, Precision, Status, TTMError, force_kernel, lib, plan_kernel, version
+from ._lib import (LIB_PATH, Precision, Status, TTMError, force_kernel, lib, plan_kernel,
+                   plan_kernel_symbol, version)
 from .api import dense_strides, layout_from_strides, ttm, ttm_raw
 from . import ttmpy
 
-__all__ = ["LIB_PATH", "Precision", "Status", "TTMError", "force_kernel", "lib", "plan_kernel",
+__all__ = ["LIB_PATH", "Precision", "Status", "TTMError", "force_kernel", "lib", "plan_kernel", "plan_kernel_symbol",
            "version", "dense_strides", "layout_from_strides", "ttm", "ttm_raw", "ttmpy"]

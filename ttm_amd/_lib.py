@@ -76,6 +76,8 @@ SIGNATURES = [
     ("ttm_is_device_pointer", ctypes.c_int, [_vp]),
     ("ttm_plan_kernel", ctypes.c_char_p,
      [ctypes.c_uint, ctypes.c_uint, ctypes.c_int, _size_p, _size_p, _size_p, _size_p, ctypes.c_int]),
+    ("ttm_plan_kernel_symbol", ctypes.c_char_p,
+     [ctypes.c_uint, ctypes.c_uint, ctypes.c_int, _size_p, _size_p, _size_p, _size_p, ctypes.c_int]),
     ("ttm_version", ctypes.c_char_p, []),
     ("ttm_force_kernel", ctypes.c_int, [ctypes.c_char_p]),
 ]
@@ -131,6 +133,13 @@ def plan_kernel(q: int, na, pia, nb, pib, is_double: bool = False, precision: in
     p = len(na)
     return lib().ttm_plan_kernel(q, p, int(is_double), sizes(na), sizes(pia), sizes(nb), sizes(pib),
                                  int(precision)).decode()
+
+
+def plan_kernel_symbol(q: int, na, pia, nb, pib, is_double: bool = False, precision: int = 0) -> str:
+    """Kernel template instance the planner launches (as rocprof names it)."""
+    p = len(na)
+    return lib().ttm_plan_kernel_symbol(q, p, int(is_double), sizes(na), sizes(pia), sizes(nb), sizes(pib),
+                                        int(precision)).decode()
 
 
 def version() -> str:

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <string>
 #include <type_traits>
 
 #include "ttm_amd.h"
@@ -179,26 +180,45 @@ const char *ttm_status_message(ttm_status_t status) { return status_text(status)
 
 int ttm_is_device_pointer(const void *ptr) { return is_device_ptr(ptr) ? 1 : 0; }
 
-const char *ttm_plan_kernel(unsigned q, unsigned p, int is_double, const size_t *na, const size_t *pia,
-                            const size_t *nb, const size_t *pib, ttm_precision_t precision)
+namespace
 {
-    if (!na || !pia || !nb || !pib || p == 0 || p > 64 || q == 0 || q > p) return "";
+// Fold of a dense problem described by extents and layouts only.
+bool fold_dense(unsigned q, unsigned p, const size_t *na, const size_t *pia, const size_t *nb, const size_t *pib,
+                Fold &f)
+{
+    if (!na || !pia || !nb || !pib || p == 0 || p > 64 || q == 0 || q > p) return false;
     size_t nc[64], wa[64], wc[64];
     for (unsigned r = 0; r < p; ++r) nc[r] = na[r];
     nc[q - 1] = nb[0];
-    // dense strides of the layout
     size_t sa = 1, sc = 1;
     for (unsigned r = 0; r < p; ++r) {
-        if (pia[r] == 0 || pia[r] > p) return "";
+        if (pia[r] == 0 || pia[r] > p) return false;
         wa[pia[r] - 1] = sa;
         wc[pia[r] - 1] = sc;
         sa *= na[pia[r] - 1];
         sc *= nc[pia[r] - 1];
     }
     static const char dummy = 0;
+    return plan(q, p, &dummy, na, wa, pia, &dummy, nb, pib, &dummy, nc, wc, f) == TTM_SUCCESS;
+}
+thread_local std::string g_symbol;
+} // namespace
+
+const char *ttm_plan_kernel(unsigned q, unsigned p, int is_double, const size_t *na, const size_t *pia,
+                            const size_t *nb, const size_t *pib, ttm_precision_t precision)
+{
     Fold f;
-    if (plan(q, p, &dummy, na, wa, pia, &dummy, nb, pib, &dummy, nc, wc, f) != TTM_SUCCESS) return "";
+    if (!fold_dense(q, p, na, pia, nb, pib, f)) return "";
     return kernel_name(choose_kernel(f, is_double != 0, precision, true, g_force));
+}
+
+const char *ttm_plan_kernel_symbol(unsigned q, unsigned p, int is_double, const size_t *na, const size_t *pia,
+                                   const size_t *nb, const size_t *pib, ttm_precision_t precision)
+{
+    Fold f;
+    if (!fold_dense(q, p, na, pia, nb, pib, f)) return "";
+    g_symbol = kernel_symbol(choose_kernel(f, is_double != 0, precision, true, g_force), f, is_double != 0);
+    return g_symbol.c_str();
 }
 
 const char *ttm_version(void) { return "ttm_amd 0.1.0 (gfx950)"; }

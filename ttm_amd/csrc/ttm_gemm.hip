@@ -1,0 +1,436 @@
+// LDS-staged MFMA kernels for the folded product (plain FP32 / FP64)
+//
+//     D[j][n] = sum_l B(j, l) * At[l][n]       j < m, n < ncols, l < nq
+//
+// with two views of the column index n:
+//   NN (w % VEC == 0):  n = k*w + i,  At[l][n] = A[k*nq*w + l*w + i],  C at k*m*w + j*w + i
+//   TN (w == 1):        n = k,        At[l][n] = A[k*nq + l],          C at k*m + j
+//
+// A workgroup of 4 waves (WAVES_M x WAVES_N) computes a BM x BN block of D,
+// stepping through the contraction in BK-slices that are staged in LDS with
+// double buffering (one barrier per slice; the next slice's global loads are
+// in flight while the current one is multiplied).
+//
+// Fragment interleaving.  A wave owns 16*JT rows and 16*NT columns.  MFMA
+// tile jt holds the rows {JT*x + jt : x < 16} and tile nt the columns
+// {NT*y + nt : y < 16}, so the JT (NT) fragment values a lane needs for one
+// k are adjacent in LDS and arrive with one vector read; and a lane's NT
+// results of one row are adjacent columns (NN: one 16-byte store) and its JT
+// results of one column adjacent rows (TN: one 16-byte store).
+//
+// MFMA lane conventions (16x16x4, lane = 16*lr + lc): srcA holds
+// Afrag[lc][lr], srcB holds Bfrag[lr][lc], accumulator element r holds
+// D[acc_row(lr, r)][lc] with acc_row = 4*lr + r (f32) or 4*r + lr (f64).
+#include "ttm_gemm.hpp"
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+namespace ttm_amd
+{
+namespace
+{
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+struct G;
+template <>
+struct G<float>
+{
+    static constexpr int VEC = 4;
+    using vec = f32x4;
+    using acc = f32x4;
+    __device__ static constexpr int acc_row(int lr, int r) { return 4 * lr + r; }
+    __device__ static f32x4 mfma(float a, float b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+};
+template <>
+struct G<double>
+{
+    static constexpr int VEC = 2;
+    using vec = f64x2;
+    using acc = f64x4;
+    __device__ static constexpr int acc_row(int lr, int r) { return 4 * r + lr; }
+    __device__ static f64x4 mfma(double a, double b, f64x4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+};
+
+template <class V>
+__device__ __forceinline__ V vzero()
+{
+    V v;
+#pragma unroll
+    for (int e = 0; e < (int)(sizeof(V) / sizeof(v[0])); ++e) v[e] = 0;
+    return v;
+}
+
+// Read N consecutive T from LDS with 16-byte (or 8-byte) vector reads.
+template <class T, int N>
+__device__ __forceinline__ void lds_read(const T *p, T (&out)[N])
+{
+    constexpr int VEC = G<T>::VEC;
+    if constexpr (N % VEC == 0) {
+#pragma unroll
+        for (int e = 0; e < N; e += VEC) {
+            auto v = *reinterpret_cast<const typename G<T>::vec *>(p + e);
+#pragma unroll
+            for (int x = 0; x < VEC; ++x) out[e + x] = v[x];
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < N; ++e) out[e] = p[e];
+    }
+}
+
+// Workgroup id -> (row block, column block).  Consecutive workgroup ids are
+// dispatched round-robin over the 8 XCDs; regroup them so that the row
+// blocks sharing one column block run on the same XCD (shared L2 for that
+// slice of A).  Row blocks vary fastest.
+__device__ __forceinline__ void block_coords(unsigned nj, unsigned total, unsigned &jb, unsigned &nb)
+{
+    unsigned b = blockIdx.x;
+    unsigned const full = total & ~7u;
+    if (b < full) b = (b & 7u) * (full >> 3) + (b >> 3);
+    jb = b % nj;
+    nb = b / nj;
+}
+
+template <class T, int JT, int NT, int WAVES_M, int BK, bool TN, bool BCOL>
+__global__ __launch_bounds__(256) void k_gemm(const T *__restrict__ a, const T *__restrict__ b,
+                                              T *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w,
+                                              uint64_t m, unsigned nj, unsigned total)
+{
+    using Gt = G<T>;
+    using V = typename Gt::vec;
+    using Acc = typename Gt::acc;
+    constexpr int VEC = Gt::VEC;
+    constexpr int WAVES_N = 4 / WAVES_M;
+    constexpr int BM = WAVES_M * 16 * JT;
+    constexpr int BN = WAVES_N * 16 * NT;
+    constexpr int PAD = VEC;            // row padding of the LDS tiles (one 16-byte vector)
+    constexpr int LDA = BN + PAD;       // At tile row stride
+    constexpr int LDB = BM + PAD;       // B tile row stride
+    static_assert(BK % 4 == 0 && BK % VEC == 0, "BK");
+
+    __shared__ __attribute__((aligned(16))) T sA[2][BK * LDA];
+    __shared__ __attribute__((aligned(16))) T sB[2][BK * LDB];
+
+    unsigned jb, nb;
+    block_coords(nj, total, jb, nb);
+    uint64_t const j0 = (uint64_t)jb * BM;
+    uint64_t const n0 = (uint64_t)nb * BN;
+
+    int const tid = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
+    int const lc = lane & 15;
+    int const lr = lane >> 4;
+    int const wm = wave % WAVES_M;
+    int const wn = wave / WAVES_M;
+
+    // ---- global -> register loaders ------------------------------------
+    // At tile: BK x BN.  NN: rows of BN contiguous columns (vectors along n).
+    //                     TN: BN columns of BK contiguous l (vectors along l).
+    constexpr int A_VECS = BK * BN / VEC;
+    constexpr int A_PER = (A_VECS + 255) / 256;
+    // B tile: BK x BM.  BCOL: B(j,l) at j + l*m (vectors along j);
+    //                   else  at j*nq + l (vectors along l).
+    constexpr int B_VECS = BK * BM / VEC;
+    constexpr int B_PER = (B_VECS + 255) / 256;
+
+    // per-thread fixed part of the A addresses
+    uint64_t a_base[A_PER];
+    bool a_ok[A_PER];
+    int a_row[A_PER], a_col[A_PER];
+#pragma unroll
+    for (int e = 0; e < A_PER; ++e) {
+        int const v = tid + 256 * e;
+        a_ok[e] = v < A_VECS;
+        if constexpr (!TN) {
+            a_row[e] = v / (BN / VEC);
+            a_col[e] = (v % (BN / VEC)) * VEC;
+            uint64_t const n = n0 + a_col[e];
+            a_ok[e] = a_ok[e] && n < ncols;
+            uint64_t const k = a_ok[e] ? n / w : 0;
+            a_base[e] = k * nq * w + (a_ok[e] ? n - k * w : 0);
+        } else {
+            a_col[e] = v / (BK / VEC);          // column n within the tile
+            a_row[e] = (v % (BK / VEC)) * VEC;  // first l within the slice
+            uint64_t const n = n0 + a_col[e];
+            a_ok[e] = a_ok[e] && n < ncols;
+            a_base[e] = a_ok[e] ? n * nq : 0;
+        }
+    }
+    int b_row[B_PER], b_col[B_PER];
+    bool b_in[B_PER];
+#pragma unroll
+    for (int e = 0; e < B_PER; ++e) {
+        int const v = tid + 256 * e;
+        b_in[e] = v < B_VECS;
+        if constexpr (BCOL) {
+            b_row[e] = v / (BM / VEC);
+            b_col[e] = (v % (BM / VEC)) * VEC;
+        } else {
+            b_col[e] = v / (BK / VEC);
+            b_row[e] = (v % (BK / VEC)) * VEC;
+        }
+    }
+    bool const b_vec_ok = BCOL ? (m % VEC == 0) : (nq % VEC == 0);
+
+    V ra[A_PER], rb[B_PER];
+
+    auto load_tiles = [&](uint64_t l0) {
+#pragma unroll
+        for (int e = 0; e < A_PER; ++e) {
+            ra[e] = vzero<V>();
+            if constexpr (!TN) {
+                uint64_t const l = l0 + a_row[e];
+                if (a_ok[e] && l < nq) ra[e] = *reinterpret_cast<const V *>(a + a_base[e] + l * w);
+            } else {
+                uint64_t const l = l0 + a_row[e];
+                // nq % VEC == 0 for TN, so a vector is wholly in or out of range
+                if (a_ok[e] && l < nq) ra[e] = *reinterpret_cast<const V *>(a + a_base[e] + l);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < B_PER; ++e) {
+            rb[e] = vzero<V>();
+            if (!b_in[e]) continue;
+            if constexpr (BCOL) {
+                uint64_t const l = l0 + b_row[e];
+                uint64_t const j = j0 + b_col[e];
+                if (l < nq) {
+                    if (b_vec_ok && j + VEC <= m) {
+                        rb[e] = *reinterpret_cast<const V *>(b + j + l * m);
+                    } else {
+#pragma unroll
+                        for (int x = 0; x < VEC; ++x)
+                            if (j + x < m) rb[e][x] = b[j + x + l * m];
+                    }
+                }
+            } else {
+                uint64_t const j = j0 + b_col[e];
+                uint64_t const l = l0 + b_row[e];
+                if (j < m) {
+                    if (b_vec_ok && l + VEC <= nq) {
+                        rb[e] = *reinterpret_cast<const V *>(b + j * nq + l);
+                    } else {
+#pragma unroll
+                        for (int x = 0; x < VEC; ++x)
+                            if (l + x < nq) rb[e][x] = b[j * nq + l + x];
+                    }
+                }
+            }
+        }
+    };
+
+    auto store_tiles = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < A_PER; ++e) {
+            if (tid + 256 * e >= A_VECS) continue;
+            if constexpr (!TN) {
+                *reinterpret_cast<V *>(&sA[buf][a_row[e] * LDA + a_col[e]]) = ra[e];
+            } else {
+#pragma unroll
+                for (int x = 0; x < VEC; ++x) sA[buf][(a_row[e] + x) * LDA + a_col[e]] = ra[e][x];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < B_PER; ++e) {
+            if (!b_in[e]) continue;
+            if constexpr (BCOL) {
+                *reinterpret_cast<V *>(&sB[buf][b_row[e] * LDB + b_col[e]]) = rb[e];
+            } else {
+#pragma unroll
+                for (int x = 0; x < VEC; ++x) sB[buf][(b_row[e] + x) * LDB + b_col[e]] = rb[e][x];
+            }
+        }
+    };
+
+    Acc acc[JT][NT];
+#pragma unroll
+    for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[jt][nt] = vzero<Acc>();
+
+    int const a_off = wn * 16 * NT + NT * lc; // this lane's columns within the At tile
+    int const b_off = wm * 16 * JT + JT * lc; // this lane's rows within the B tile
+
+    uint64_t const nslices = (nq + BK - 1) / BK;
+    load_tiles(0);
+    store_tiles(0);
+    __syncthreads();
+    for (uint64_t s = 0; s < nslices; ++s) {
+        int const buf = (int)(s & 1);
+        bool const more = s + 1 < nslices;
+        if (more) load_tiles((s + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK / 4; ++kk) {
+            int const kr = kk * 4 + lr;
+            T fa[JT], fb[NT];
+            lds_read<T, JT>(&sB[buf][kr * LDB + b_off], fa);
+            lds_read<T, NT>(&sA[buf][kr * LDA + a_off], fb);
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) acc[jt][nt] = Gt::mfma(fa[jt], fb[nt], acc[jt][nt]);
+        }
+        if (more) store_tiles(buf ^ 1);
+        __syncthreads();
+    }
+
+    // ---- epilogue ---------------------------------------------------------
+    uint64_t const ncol0 = n0 + a_off; // first of this lane's NT adjacent columns
+    if constexpr (!TN) {
+        // one row j, NT adjacent columns; each VEC-group of them lies in one
+        // slab (w % VEC == 0), but the NT columns may span two slabs.
+        constexpr int NG = NT / VEC;
+        static_assert(NT % VEC == 0, "NT");
+        T *cgroup[NG];
+        bool gok[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            uint64_t const n = ncol0 + g * VEC;
+            gok[g] = n < ncols;
+            uint64_t const k = gok[g] ? n / w : 0;
+            cgroup[g] = c + k * m * w + (gok[g] ? n - k * w : 0);
+        }
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                uint64_t const j = j0 + wm * 16 * JT + JT * Gt::acc_row(lr, r) + jt;
+                if (j >= m) continue;
+#pragma unroll
+                for (int g = 0; g < NG; ++g) {
+                    if (!gok[g]) continue;
+                    V o;
+#pragma unroll
+                    for (int x = 0; x < VEC; ++x) o[x] = acc[jt][g * VEC + x][r];
+                    *reinterpret_cast<V *>(cgroup[g] + j * w) = o;
+                }
+            }
+    } else {
+        // one column (row k of C), JT adjacent rows j
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            uint64_t const k = ncol0 + nt;
+            if (k >= ncols) continue;
+            T *crow = c + k * m;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                uint64_t const jr = j0 + wm * 16 * JT + JT * Gt::acc_row(lr, r);
+                if constexpr (JT % VEC == 0) {
+                    if ((m % VEC) == 0 && jr + JT <= m) {
+#pragma unroll
+                        for (int jt = 0; jt < JT; jt += VEC) {
+                            V o;
+#pragma unroll
+                            for (int x = 0; x < VEC; ++x) o[x] = acc[jt + x][nt][r];
+                            *reinterpret_cast<V *>(crow + jr + jt) = o;
+                        }
+                        continue;
+                    }
+                }
+#pragma unroll
+                for (int jt = 0; jt < JT; ++jt)
+                    if (jr + jt < m) crow[jr + jt] = acc[jt][nt][r];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side: tile configurations
+// ---------------------------------------------------------------------------
+struct Config
+{
+    int jt, nt, waves_m, bk;
+};
+
+// Small m streams A once with wide column blocks; large m uses square blocks.
+Config pick_config(const Fold &f, bool is_double)
+{
+    (void)is_double;
+    if (f.m <= 16) return {1, 4, 1, 16};
+    if (f.m <= 32) return {2, 4, 1, 16};
+    if (f.m <= 64) return {4, 4, 1, 16};
+    return {4, 4, 2, 16};
+}
+
+template <class T, int JT, int NT, int WM, int BK, bool TN, bool BCOL>
+void launch_cfg(const Fold &f, const T *a, const T *b, T *c, hipStream_t s)
+{
+    constexpr int BM = WM * 16 * JT;
+    constexpr int BN = (4 / WM) * 16 * NT;
+    uint64_t const ncols = TN ? f.outer : f.outer * f.w;
+    unsigned const nj = (unsigned)((f.m + BM - 1) / BM);
+    unsigned const nn = (unsigned)((ncols + BN - 1) / BN);
+    unsigned const total = nj * nn;
+    hipLaunchKernelGGL((k_gemm<T, JT, NT, WM, BK, TN, BCOL>), dim3(total), dim3(256), 0, s, a, b, c, ncols, f.nq,
+                       f.w, f.m, nj, total);
+}
+
+template <class T, bool TN, bool BCOL>
+void launch_layout(const Fold &f, const T *a, const T *b, T *c, hipStream_t s, Config cfg)
+{
+    if (cfg.jt == 1) launch_cfg<T, 1, 4, 1, 16, TN, BCOL>(f, a, b, c, s);
+    else if (cfg.jt == 2) launch_cfg<T, 2, 4, 1, 16, TN, BCOL>(f, a, b, c, s);
+    else if (cfg.waves_m == 1) launch_cfg<T, 4, 4, 1, 16, TN, BCOL>(f, a, b, c, s);
+    else launch_cfg<T, 4, 4, 2, 16, TN, BCOL>(f, a, b, c, s);
+}
+
+} // namespace
+
+std::string gemm_symbol(const Fold &f, bool is_double)
+{
+    Config const cfg = pick_config(f, is_double);
+    char buf[128];
+    std::snprintf(buf, sizeof buf, "k_gemm<%s, %d, %d, %d, %d, %s, %s>", is_double ? "double" : "float", cfg.jt,
+                  cfg.nt, cfg.waves_m, cfg.bk, f.w == 1 ? "true" : "false", f.b_colmajor ? "true" : "false");
+    return buf;
+}
+
+bool gemm_supports(const Fold &f, bool is_double, bool aligned)
+{
+    uint64_t const vec = is_double ? 2 : 4;
+    if (!aligned) return false;
+    bool const tn = f.w == 1;
+    if (tn) {
+        if (f.nq % vec != 0) return false;
+    } else if (f.w % vec != 0) {
+        return false;
+    }
+    uint64_t const ncols = tn ? f.outer : f.outer * f.w;
+    // grid size limits (the grid is one-dimensional)
+    uint64_t const blocks = ((f.m + 15) / 16) * ((ncols + 127) / 128);
+    return blocks < (1ull << 31);
+}
+
+template <class T>
+ttm_status_t launch_gemm(const Fold &f, const T *a, const T *b, T *c, hipStream_t s)
+{
+    Config const cfg = pick_config(f, sizeof(T) == 8);
+    bool const tn = f.w == 1;
+    if (tn) {
+        if (f.b_colmajor) launch_layout<T, true, true>(f, a, b, c, s, cfg);
+        else launch_layout<T, true, false>(f, a, b, c, s, cfg);
+    } else {
+        if (f.b_colmajor) launch_layout<T, false, true>(f, a, b, c, s, cfg);
+        else launch_layout<T, false, false>(f, a, b, c, s, cfg);
+    }
+    hipError_t const e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(TTM_ERR_HIP, hipGetErrorString(e));
+        return TTM_ERR_HIP;
+    }
+    return TTM_SUCCESS;
+}
+
+template ttm_status_t launch_gemm<float>(const Fold &, const float *, const float *, float *, hipStream_t);
+template ttm_status_t launch_gemm<double>(const Fold &, const double *, const double *, double *, hipStream_t);
+
+} // namespace ttm_amd

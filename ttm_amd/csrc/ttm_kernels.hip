@@ -15,6 +15,7 @@
 // In every kernel here srcA is a fragment of the small matrix B (rows j) and
 // srcB a fragment of the tensor A, so D rows are rows j of C.
 #include "ttm_kernels.hpp"
+#include "ttm_gemm.hpp"
 
 #include <cstring>
 
@@ -355,6 +356,7 @@ ttm_status_t launch_plain(const Fold &f, const T *a, const T *b, T *c, hipStream
 #undef TTM_LAUNCH_TN
         break;
     }
+    case Kernel::gemm: return launch_gemm<T>(f, a, b, c, s);
     default: set_error(TTM_ERR_ARGUMENT, "no kernel for this problem"); return TTM_ERR_ARGUMENT;
     }
     hipError_t const e = hipGetLastError();
@@ -365,7 +367,7 @@ ttm_status_t launch_plain(const Fold &f, const T *a, const T *b, T *c, hipStream
     return TTM_SUCCESS;
 }
 
-const char *const k_kernel_names[] = {"none", "generic", "nn_stream", "tn_stream"};
+const char *const k_kernel_names[] = {"none", "generic", "nn_stream", "tn_stream", "gemm"};
 
 } // namespace
 
@@ -373,6 +375,19 @@ const char *kernel_name(Kernel k)
 {
     int const i = static_cast<int>(k);
     return (i >= 0 && i < static_cast<int>(Kernel::count)) ? k_kernel_names[i] : "none";
+}
+
+std::string kernel_symbol(Kernel k, const Fold &f, bool is_double)
+{
+    std::string const t = is_double ? "double" : "float";
+    std::string const jt = std::to_string(pick_jt(f.m));
+    switch (k) {
+    case Kernel::generic: return "k_generic<" + t + ">";
+    case Kernel::nn_stream: return "k_nn_stream<" + t + ", " + jt + ">";
+    case Kernel::tn_stream: return "k_tn_stream<" + t + ", " + jt + ", " + (f.b_colmajor ? "false" : "true") + ">";
+    case Kernel::gemm: return gemm_symbol(f, is_double);
+    default: return "";
+    }
 }
 
 Kernel kernel_from_name(const char *name)
@@ -391,9 +406,12 @@ Kernel choose_kernel(const Fold &f, bool is_double, ttm_precision_t prec, bool a
     bool const grid_ok = (f.m + 15) / 16 <= 65535;
     bool const nn_ok = aligned && grid_ok && f.w % vec == 0;
     bool const tn_ok = aligned && grid_ok && f.w == 1 && f.nq % vec == 0;
+    bool const gemm_ok = gemm_supports(f, is_double, aligned);
     if (force == Kernel::generic) return Kernel::generic;
     if (force == Kernel::nn_stream && nn_ok) return Kernel::nn_stream;
     if (force == Kernel::tn_stream && tn_ok) return Kernel::tn_stream;
+    if (force == Kernel::gemm && gemm_ok) return Kernel::gemm;
+    if (gemm_ok) return Kernel::gemm;
     if (tn_ok) return Kernel::tn_stream;
     if (nn_ok) return Kernel::nn_stream;
     return Kernel::generic;

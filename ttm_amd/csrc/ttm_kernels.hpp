@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 
 #include "ttm_amd.h"
 #include "ttm_plan.hpp"
@@ -17,6 +18,7 @@ enum class Kernel : int
     generic,    // one thread per C element, any shape / alignment
     nn_stream,  // MFMA 16x16x4, w % vec == 0: streams A slabs along the contiguous block
     tn_stream,  // MFMA 16x16x4, w == 1: rows of A contiguous along the contraction mode
+    gemm,       // LDS-staged MFMA blocks (ttm_gemm.hip), NN (w % vec == 0) or TN (w == 1) view
     count
 };
 
@@ -26,6 +28,9 @@ Kernel kernel_from_name(const char *name);
 // Kernel the planner picks for a fold (aligned: all base pointers 16-byte
 // aligned).  `force` (may be Kernel::none) overrides when applicable.
 Kernel choose_kernel(const Fold &f, bool is_double, ttm_precision_t prec, bool aligned, Kernel force);
+
+// Template instance of the family's kernel for a fold (as profilers print it).
+std::string kernel_symbol(Kernel k, const Fold &f, bool is_double);
 
 // Launch the product on `stream`; returns TTM_SUCCESS or TTM_ERR_HIP.
 ttm_status_t launch_f32(const Fold &f, const float *a, const float *b, float *c, ttm_precision_t prec,
