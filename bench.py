@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--force", default="", help="force a kernel family (comparison runs only)")
+    ap.add_argument("--no-bf16x3", dest="bf16x3", action="store_false",
+                    help="skip the BF16X3-vs-FP32 comparison on the compute-bound float32 products")
     return ap.parse_args()
 
 
@@ -131,9 +133,65 @@ def gpu_bench(args, cases):
         t_cmp = c["flops"] / PEAK_FLOPS[c["dtype"]]
         c["bound"] = "hbm" if t_hbm >= t_cmp else "mfma"
         c["frac"] = max(t_hbm, t_cmp) / t
+    split = None
+    if args.bf16x3 and "f32" in A:
+        split = bf16x3_vs_fp32(args, cases, A["f32"], Cbuf["f32"], stream)
     del A, Cbuf
     torch.cuda.empty_cache()
-    return wall / args.steps
+    return wall / args.steps, split
+
+
+def bf16x3_vs_fp32(args, cases, A, C, stream):
+    """The compute-bound float32 products again in BF16X3: time, and the
+    normwise relative error of both arithmetics against the float64
+    definition on 4096 sampled entries of C."""
+    import torch
+    import ttm_amd
+    rng = np.random.default_rng(17)
+    rows = []
+    for c in cases:
+        if c["dtype"] != "f32" or c["m"] != 1024:
+            continue
+        q = c["q"]
+        nq = EXTENTS[q - 1]
+
+        def run(prec):
+            ttm_amd.ttm_raw(q, 3, A, c["na"], c["wa"], c["pia"], c["B"], [c["m"], nq], [2, 1], C, c["nc"], c["wc"],
+                            precision=prec)
+
+        res = {}
+        for prec in ("default", "bf16x3"):
+            for _ in range(2):
+                run(prec)
+            ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+            for e0, e1 in ev:
+                e0.record(stream)
+                run(prec)
+                e1.record(stream)
+            torch.cuda.synchronize()
+            ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
+            # sampled error against the float64 definition
+            idx = [rng.integers(0, c["nc"][r], size=4096) for r in range(3)]
+            offc = sum(idx[r] * c["wc"][r] for r in range(3))
+            base = sum(idx[r] * c["wa"][r] for r in range(3) if r != q - 1)
+            fib = torch.from_numpy((base[:, None] + np.arange(nq)[None, :] * c["wa"][q - 1]).astype(np.int64))
+            a_f = A[fib.to(A.device)].double().cpu().numpy()
+            b_r = c["B"].double().cpu().numpy()[idx[q - 1]]
+            expect = np.sum(a_f * b_r, axis=1)
+            got = C[torch.from_numpy(offc.astype(np.int64)).to(A.device)].double().cpu().numpy()
+            res[prec] = (ms, float(np.linalg.norm(got - expect) / np.linalg.norm(expect)))
+        rows.append({"case": c["label"], "fp32_ms": round(res["default"][0], 4),
+                     "bf16x3_ms": round(res["bf16x3"][0], 4),
+                     "speedup": round(res["default"][0] / res["bf16x3"][0], 3),
+                     "fp32_normwise_err": res["default"][1], "bf16x3_normwise_err": res["bf16x3"][1],
+                     "bf16x3_fp32equiv_tflops": round(c["flops"] / res["bf16x3"][0] / 1e9, 2)})
+    if not rows:
+        return None
+    sp = [r["speedup"] for r in rows]
+    return {"speedup_geomean": round(float(np.exp(np.mean(np.log(sp)))), 3),
+            "max_bf16x3_normwise_err": max(r["bf16x3_normwise_err"] for r in rows),
+            "max_fp32_normwise_err": max(r["fp32_normwise_err"] for r in rows),
+            "error_reference": "float64 definition on 4096 sampled entries", "cases": rows}
 
 
 def cpu_baseline(args, budget_s):
@@ -212,7 +270,7 @@ def main():
     if args.only:
         keep = set(args.only.split(","))
         cases = [c for c in cases if c["label"] in keep]
-    step_s = gpu_bench(args, cases)
+    step_s, split = gpu_bench(args, cases)
     total_flops = sum(c["flops"] for c in cases)
     total_bytes = sum(c["bytes"] for c in cases)
     kern_time = sum(c["ms"] for c in cases) * 1e-3
@@ -262,6 +320,7 @@ def main():
         "step_kernel_ms": round(kern_time * 1e3, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "bf16x3_vs_fp32": split,
         "cases": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in c.items()
                    if k in ("label", "symbol", "ms", "ms_min", "tflops", "gbps", "bound", "frac")}
                   for c in cases],

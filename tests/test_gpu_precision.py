@@ -1,0 +1,192 @@
+"""Reduced-precision variants of the float32 product on the GPU.
+
+Each variant is checked twice:
+* against the CPU emulation of its own arithmetic (tests/lowp_emul.py),
+  which it must match up to FP32 accumulation: |dC| <= 4 n_q 2^-24 sum|b||a|
+  elementwise over the rounded operands;
+* against the FP64 oracle (brute force of the definition), reporting max and
+  normwise relative errors (the error table of the config-3 study), with
+  BF16X3 held to the 1e-6 normwise target.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import lowp_emul as le
+import refcases as rc
+
+pytestmark = pytest.mark.gpu
+
+PRECS = ["bf16x3", "bf16", "fp16", "fp8"]
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run(torch, dev, q, a, na, wa, pia, b, nb, pib, nc, wc, precision, kernel=None):
+    import ttm_amd
+    A = torch.from_numpy(a).to(dev)
+    B = torch.from_numpy(b).to(dev)
+    C = torch.full((int(np.prod(nc)),), float("nan"), dtype=torch.float32, device=dev)
+    ttm_amd.force_kernel(kernel)
+    try:
+        ttm_amd.ttm_raw(q, len(na), A, na, wa, pia, B, nb, pib, C, nc, wc, precision=precision)
+        torch.cuda.synchronize()
+    finally:
+        ttm_amd.force_kernel(None)
+    return C.cpu().numpy()
+
+
+def folded(a, na, pia, q):
+    """A as (outer, nq, w) in the folded order of its layout."""
+    qh = pia.index(q) + 1
+    w = int(np.prod([na[m - 1] for m in pia[: qh - 1]])) if qh > 1 else 1
+    outer = int(np.prod([na[m - 1] for m in pia[qh:]])) if qh < len(pia) else 1
+    return a.reshape(outer, na[q - 1], w)
+
+
+def case(rng, na, pia, q, m, colmajor, dist):
+    wa = rc.dense_strides(na, pia)
+    n = int(np.prod(na))
+    if dist == "uniform":
+        a = rng.uniform(-1, 1, n).astype(np.float32)
+        Bm = rng.uniform(-1, 1, (m, na[q - 1])).astype(np.float32)
+    else:  # sign-random log-uniform magnitudes in [1e-3, 1e3]
+        a = (rng.choice([-1, 1], n) * 10.0 ** rng.uniform(-3, 3, n)).astype(np.float32)
+        Bm = (rng.choice([-1, 1], (m, na[q - 1])) * 10.0 ** rng.uniform(-3, 3, (m, na[q - 1]))).astype(np.float32)
+    b = (Bm.T if colmajor else Bm).reshape(-1).copy()
+    nc = list(na)
+    nc[q - 1] = m
+    wc = rc.dense_strides(nc, pia)
+    return a, wa, Bm, b, [m, na[q - 1]], ([1, 2] if colmajor else [2, 1]), nc, wc
+
+
+def check_vs_emulation(c, a, na, pia, q, Bm, prec, label):
+    A3 = folded(a, na, pia, q)
+    ref = le.emulate(prec, A3, Bm)                       # (outer, m, w)
+    mag = {"bf16": lambda x: np.abs(le.bf16(x)), "fp16": lambda x: np.abs(le.fp16(x)),
+           "bf16x3": lambda x: np.abs(x.astype(np.float64)) * 1.001,
+           "fp8": lambda x: np.abs(x.astype(np.float64)) * 1.2}[prec]
+    # FP32 accumulation bound; the scaled FP8 instruction sums 128 products per
+    # step inside the matrix core with undocumented intermediate rounding, so
+    # it gets a 4x wider (still ~1000x tighter than FP8 rounding) allowance.
+    factor = 16 if prec == "fp8" else 4
+    bound = factor * na[q - 1] * 2.0 ** -24 * np.einsum("jl,kli->kji", mag(Bm), mag(A3)) + 1e-30
+    if prec == "fp8":
+        # plus the smallest normal (8 subnormal steps) per operand and product:
+        # the matrix core may flush e4m3fn subnormal inputs, which this
+        # comparison does not pin (a wrong layout or scale errs by the values
+        # themselves, orders of magnitude more)
+        qa, qb = 8 * le.fp8_block_quantum(A3, axis=1), 8 * le.fp8_block_quantum(Bm, axis=1)
+        bound = bound + np.einsum("jl,kli->kji", qb, mag(A3)) + np.einsum("jl,kli->kji", mag(Bm), qa)
+    got = c.reshape(ref.shape)
+    err = np.abs(got - ref)
+    worst = float(np.max(err / bound))
+    assert np.all(err <= bound), f"{label}: emulation mismatch, max err/bound {worst:.2f}"
+    return worst
+
+
+def errors_vs_fp64(c, c64):
+    c = c.astype(np.float64)
+    return (float(np.max(np.abs(c - c64)) / np.max(np.abs(c64))),
+            float(np.linalg.norm(c - c64) / np.linalg.norm(c64)))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_variant_matches_its_emulation(gpu, prec):
+    import torch
+    rng = np.random.default_rng(11)
+    shapes = [([64, 48, 32], [1, 2, 3], 2, 40), ([16, 24, 20], [3, 2, 1], 3, 64), ([40, 12, 8], [1, 2, 3], 1, 33),
+              ([8, 36, 10], [2, 1, 3], 2, 130), ([20, 8, 5], [1, 3, 2], 2, 7), ([13, 7, 9], [1, 2, 3], 2, 5),
+              ([256, 4, 6], [3, 1, 2], 1, 17)]
+    for na, pia, q, m in shapes:
+        for colmajor in (True, False):
+            for dist in ("uniform", "loguniform"):
+                a, wa, Bm, b, nb, pib, nc, wc = case(rng, na, pia, q, m, colmajor, dist)
+                for kernel in (None, "lowp_generic"):
+                    c = run(torch, gpu, q, a, na, wa, pia, b, nb, pib, nc, wc, prec, kernel)
+                    # reorder C into the folded (outer, m, w) order of its layout
+                    c3 = folded(c, nc, pia, q)
+                    check_vs_emulation(c3, a, na, pia, q, Bm, prec, f"{prec} {na} {pia} q={q} m={m} {dist} {kernel}")
+
+
+def test_config3_error_table(gpu, oracle):
+    """cfg-1 shapes (A 64x48x32, q=2, B 40x48), float32 inputs, U[-1,1] and log-uniform."""
+    import torch
+    na, pia, q, m = [64, 48, 32], [1, 2, 3], 2, 40
+    wa = rc.dense_strides(na, pia)
+    nc = [64, 40, 32]
+    wc = rc.dense_strides(nc, pia)
+    table = {}
+    for dist in ("uniform", "loguniform"):
+        if dist == "uniform":
+            a = oracle.fill_uniform(int(np.prod(na)), 0x77140003)
+            b = oracle.fill_uniform(m * 48, 0x77140013)
+        else:
+            a = oracle.fill_loguniform(int(np.prod(na)), 0x77140023)
+            b = oracle.fill_loguniform(m * 48, 0x77140033)
+        c64 = oracle.ttm_brute_f64(q, a, na, wa, b, m, True, nc, wc)
+        for prec in ["default"] + PRECS:
+            c = run(torch, gpu, q, a, na, wa, pia, b, [m, 48], [1, 2], nc, wc, prec)
+            mx, nw = errors_vs_fp64(c, c64)
+            table[(dist, prec)] = (mx, nw)
+            _say(f"config-3 {dist:10s} {prec:8s} max-rel {mx:.3e}  normwise-rel {nw:.3e}")
+    for dist in ("uniform", "loguniform"):
+        fp32 = table[(dist, "default")][1]
+        assert table[(dist, "bf16x3")][1] <= 1e-6, table[(dist, "bf16x3")]
+        assert table[(dist, "bf16x3")][1] <= 4 * fp32 + 1e-9
+        assert 1e-5 < table[(dist, "bf16")][1] < 2e-2
+        assert 1e-6 < table[(dist, "fp16")][1] < 3e-3
+        assert 1e-3 < table[(dist, "fp8")][1] < 0.15
+
+
+def test_bf16x3_integer_data_is_exact(gpu):
+    """Integers below 2^24 split exactly into three bfloat16 parts: the closed form holds."""
+    import torch
+    for na, pia, q, colmajor in [([4, 8, 12], [1, 2, 3], 2, True), ([16, 12, 8], [3, 2, 1], 3, False),
+                                 ([8, 8, 8], [2, 1, 3], 1, True)]:
+        m = 2 * na[q - 1]
+        a, wa = rc.ttm_init(q, na, pia)
+        b, pib = rc.make_b(m, na[q - 1], "ones", colmajor, np.float32)
+        nc = list(na)
+        nc[q - 1] = m
+        wc = rc.dense_strides(nc, pia)
+        for kernel in (None, "lowp_generic"):
+            c = run(torch, gpu, q, a.astype(np.float32), na, wa, pia, b, [m, na[q - 1]], pib, nc, wc, "bf16x3", kernel)
+            assert np.array_equal(rc.as_nd(c, nc, wc), rc.closed_form(q, na, pia, m, "ones")), (na, pia, q, kernel)
+
+
+def test_bf16x3_full_size_sampled(gpu):
+    """Compute-bound cfg-2 shapes in BF16X3: sampled entries against the float64 definition."""
+    import torch
+    import ttm_amd
+    na = [1024, 1024, 512]
+    A = torch.empty(int(np.prod(na)), dtype=torch.float32, device=gpu).uniform_(-1, 1)
+    rng = np.random.default_rng(3)
+    for pia, q in (([1, 2, 3], 2), ([3, 2, 1], 3), ([1, 2, 3], 3)):
+        m = 1024
+        Bt = torch.empty((m, na[q - 1]), dtype=torch.float32, device=gpu).uniform_(-1, 1)
+        wa = rc.dense_strides(na, pia)
+        nc = list(na)
+        nc[q - 1] = m
+        wc = rc.dense_strides(nc, pia)
+        C = torch.empty(int(np.prod(nc)), dtype=torch.float32, device=gpu)
+        ttm_amd.ttm_raw(q, 3, A, na, wa, pia, Bt, [m, na[q - 1]], [2, 1], C, nc, wc, precision="bf16x3")
+        torch.cuda.synchronize()
+        p = 3
+        idx = [rng.integers(0, nc[r], size=2048) for r in range(p)]
+        offc = sum(idx[r] * wc[r] for r in range(p))
+        base = sum(idx[r] * wa[r] for r in range(p) if r != q - 1)
+        fib_idx = torch.from_numpy((base[:, None] + np.arange(na[q - 1])[None, :] * wa[q - 1]).astype(np.int64))
+        fib = A[fib_idx.to(gpu)].double().cpu().numpy()
+        brow = Bt.double().cpu().numpy()[idx[q - 1]]
+        expect = np.sum(fib * brow, axis=1)
+        got = C[torch.from_numpy(offc.astype(np.int64)).to(gpu)].double().cpu().numpy()
+        rel = np.linalg.norm(got - expect) / np.linalg.norm(expect)
+        _say(f"bf16x3 cfg2 pi={pia} q={q}: sampled normwise-rel {rel:.3e}")
+        assert rel <= 1e-6
+        del C
+    torch.cuda.empty_cache()

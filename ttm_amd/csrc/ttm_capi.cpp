@@ -217,7 +217,7 @@ const char *ttm_plan_kernel_symbol(unsigned q, unsigned p, int is_double, const 
 {
     Fold f;
     if (!fold_dense(q, p, na, pia, nb, pib, f)) return "";
-    g_symbol = kernel_symbol(choose_kernel(f, is_double != 0, precision, true, g_force), f, is_double != 0);
+    g_symbol = kernel_symbol(choose_kernel(f, is_double != 0, precision, true, g_force), f, is_double != 0, precision);
     return g_symbol.c_str();
 }
 

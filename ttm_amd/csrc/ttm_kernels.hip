@@ -16,6 +16,7 @@
 // srcB a fragment of the tensor A, so D rows are rows j of C.
 #include "ttm_kernels.hpp"
 #include "ttm_gemm.hpp"
+#include "ttm_lowp.hpp"
 
 #include <cstring>
 
@@ -367,7 +368,7 @@ ttm_status_t launch_plain(const Fold &f, const T *a, const T *b, T *c, hipStream
     return TTM_SUCCESS;
 }
 
-const char *const k_kernel_names[] = {"none", "generic", "nn_stream", "tn_stream", "gemm"};
+const char *const k_kernel_names[] = {"none", "generic", "nn_stream", "tn_stream", "gemm", "lowp", "lowp_generic"};
 
 } // namespace
 
@@ -377,7 +378,7 @@ const char *kernel_name(Kernel k)
     return (i >= 0 && i < static_cast<int>(Kernel::count)) ? k_kernel_names[i] : "none";
 }
 
-std::string kernel_symbol(Kernel k, const Fold &f, bool is_double)
+std::string kernel_symbol(Kernel k, const Fold &f, bool is_double, ttm_precision_t prec)
 {
     std::string const t = is_double ? "double" : "float";
     std::string const jt = std::to_string(pick_jt(f.m));
@@ -386,6 +387,8 @@ std::string kernel_symbol(Kernel k, const Fold &f, bool is_double)
     case Kernel::nn_stream: return "k_nn_stream<" + t + ", " + jt + ">";
     case Kernel::tn_stream: return "k_tn_stream<" + t + ", " + jt + ", " + (f.b_colmajor ? "false" : "true") + ">";
     case Kernel::gemm: return gemm_symbol(f, is_double);
+    case Kernel::lowp: return lowp_symbol(f, prec, false);
+    case Kernel::lowp_generic: return lowp_symbol(f, prec, true);
     default: return "";
     }
 }
@@ -400,7 +403,12 @@ Kernel kernel_from_name(const char *name)
 
 Kernel choose_kernel(const Fold &f, bool is_double, ttm_precision_t prec, bool aligned, Kernel force)
 {
-    if (prec != TTM_PREC_DEFAULT) return Kernel::none; // precision variants are not built yet
+    if (prec != TTM_PREC_DEFAULT) {
+        if (is_double || prec > TTM_PREC_FP8_E4M3) return Kernel::none;
+        bool const lp_ok = lowp_supports(f, aligned);
+        if (force == Kernel::generic || force == Kernel::lowp_generic || !lp_ok) return Kernel::lowp_generic;
+        return Kernel::lowp;
+    }
     uint64_t const vec = is_double ? 2 : 4;
     // grid.y of the MFMA kernels covers m in blocks of >= 16 rows
     bool const grid_ok = (f.m + 15) / 16 <= 65535;
@@ -428,6 +436,8 @@ ttm_status_t launch_f32(const Fold &f, const float *a, const float *b, float *c,
         set_error(TTM_ERR_PRECISION);
         return TTM_ERR_PRECISION;
     }
+    if (k == Kernel::lowp || k == Kernel::lowp_generic)
+        return launch_lowp(f, a, b, c, prec, stream, k == Kernel::lowp_generic);
     return launch_plain<float>(f, a, b, c, stream, k);
 }
 

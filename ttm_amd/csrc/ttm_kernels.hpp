@@ -19,6 +19,8 @@ enum class Kernel : int
     nn_stream,  // MFMA 16x16x4, w % vec == 0: streams A slabs along the contiguous block
     tn_stream,  // MFMA 16x16x4, w == 1: rows of A contiguous along the contraction mode
     gemm,       // LDS-staged MFMA blocks (ttm_gemm.hip), NN (w % vec == 0) or TN (w == 1) view
+    lowp,       // reduced-precision MFMA blocks (ttm_lowp.hip): bf16, fp16, bf16x3, fp8
+    lowp_generic, // reduced precision, one thread per element (any shape / alignment)
     count
 };
 
@@ -30,7 +32,7 @@ Kernel kernel_from_name(const char *name);
 Kernel choose_kernel(const Fold &f, bool is_double, ttm_precision_t prec, bool aligned, Kernel force);
 
 // Template instance of the family's kernel for a fold (as profilers print it).
-std::string kernel_symbol(Kernel k, const Fold &f, bool is_double);
+std::string kernel_symbol(Kernel k, const Fold &f, bool is_double, ttm_precision_t prec);
 
 // Launch the product on `stream`; returns TTM_SUCCESS or TTM_ERR_HIP.
 ttm_status_t launch_f32(const Fold &f, const float *a, const float *b, float *c, ttm_precision_t prec,

@@ -1,0 +1,633 @@
+// Reduced-precision arithmetic for float32 products (no counterpart in the
+// reference, which is float/double only).  Inputs and output stay float32;
+// the operands are converted on the fly while a slice is staged in LDS, and
+// the matrix cores accumulate in FP32.
+//
+//   BF16    x -> bf16(x)                                   1 product
+//   FP16    x -> binary16(x)                               1 product
+//   BF16X3  x = x0 + x1 + x2, x0 = bf16(x), x1 = bf16(x - x0),
+//           x2 = bf16(x - x0 - x1); products x0y0 + x0y1 + x1y0 + x0y2 + x1y1 + x2y0
+//           (the six terms of order >= 2^-16 relative; Henry, Tang &
+//           Heinecke, "Leveraging the bfloat16 artificial intelligence
+//           datatype for higher-precision computations", ARITH 2019)
+//   FP8     OCP e4m3fn operands with one power-of-two (E8M0) scale per slice
+//           of 32 consecutive contraction indices of each fibre of A and each
+//           row of B, applied by the matrix core (v_mfma_scale_*_f8f6f4).
+//
+// Same folded product and NN / TN views as ttm_gemm.hip:
+//   NN (w % 4 == 0): n = k*w + i,  At[l][n] = A[k*nq*w + l*w + i]
+//   TN (w == 1):     n = k,        At[l][n] = A[k*nq + l]
+//
+// MFMA layouts (measured with a lane-layout probe, lane = 16*lg + lc):
+//   16x16x32 bf16/f16: srcA lane holds Afrag[lc][8*lg + e], e < 8;
+//                      srcB lane holds Bfrag[8*lg + e][lc];
+//   16x16x128 f8f6f4:  32 bytes per lane, byte e at k = 16*lg + e (e < 16) and
+//                      k = 64 + 16*lg + (e - 16) (e >= 16); the lane's E8M0
+//                      scale applies to its row (column) and k in [32*lg, 32*lg + 32);
+//   accumulator element r holds D[4*lg + r][lc].
+// LDS tiles are therefore stored K-contiguous: s[row][k].
+#include "ttm_lowp.hpp"
+
+#include <cstdio>
+#include <string>
+
+namespace ttm_amd
+{
+namespace
+{
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+enum LpMode : int
+{
+    LP_BF16 = 0,
+    LP_FP16 = 1,
+    LP_BF16X3 = 2
+};
+
+template <int MODE>
+struct Lp;
+template <>
+struct Lp<LP_BF16>
+{
+    using op = __bf16;
+    using v4 = bf16x4;
+    using v8 = bf16x8;
+    static constexpr int PARTS = 1;
+    __device__ static f32x4 mfma(v8 a, v8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+    __device__ static void split(float x, op (&p)[1]) { p[0] = (__bf16)x; }
+};
+template <>
+struct Lp<LP_FP16>
+{
+    using op = _Float16;
+    using v4 = f16x4;
+    using v8 = f16x8;
+    static constexpr int PARTS = 1;
+    __device__ static f32x4 mfma(v8 a, v8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+    __device__ static void split(float x, op (&p)[1]) { p[0] = (_Float16)x; }
+};
+template <>
+struct Lp<LP_BF16X3>
+{
+    using op = __bf16;
+    using v4 = bf16x4;
+    using v8 = bf16x8;
+    static constexpr int PARTS = 3;
+    __device__ static f32x4 mfma(v8 a, v8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+    // x - float(x0) and the next difference are exact in float32.
+    __device__ static void split(float x, op (&p)[3])
+    {
+        p[0] = (__bf16)x;
+        float const r1 = x - (float)p[0];
+        p[1] = (__bf16)r1;
+        float const r2 = r1 - (float)p[1];
+        p[2] = (__bf16)r2;
+    }
+};
+
+__device__ __forceinline__ f32x4 zero4()
+{
+    f32x4 z = {0.f, 0.f, 0.f, 0.f};
+    return z;
+}
+
+// Workgroup id -> (row block, column block), row blocks fastest, grouped per XCD.
+__device__ __forceinline__ void lp_block_coords(unsigned nj, unsigned total, unsigned &jb, unsigned &nb)
+{
+    unsigned b = blockIdx.x;
+    unsigned const full = total & ~7u;
+    if (b < full) b = (b & 7u) * (full >> 3) + (b >> 3);
+    jb = b % nj;
+    nb = b / nj;
+}
+
+// ---------------------------------------------------------------------------
+// BF16 / FP16 / BF16X3: 128 x 128 blocks, K-slices of 32, 4 waves of 64 x 64.
+// ---------------------------------------------------------------------------
+template <int MODE, bool TN, bool BCOL>
+__global__ __launch_bounds__(256) void k_lowp(const float *__restrict__ a, const float *__restrict__ b,
+                                              float *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w,
+                                              uint64_t m, unsigned nj, unsigned total)
+{
+    using L = Lp<MODE>;
+    using op = typename L::op;
+    using v4 = typename L::v4;
+    using v8 = typename L::v8;
+    constexpr int P = L::PARTS;
+    constexpr int BM = 128, BN = 128, BK = 32;
+    constexpr int LDK = BK + 8; // 80-byte rows: the 16 rows of a fragment read hit distinct banks
+    __shared__ __attribute__((aligned(16))) op sA[P][BN * LDK];
+    __shared__ __attribute__((aligned(16))) op sB[P][BM * LDK];
+
+    unsigned jb, nb;
+    lp_block_coords(nj, total, jb, nb);
+    uint64_t const j0 = (uint64_t)jb * BM;
+    uint64_t const n0 = (uint64_t)nb * BN;
+    int const tid = threadIdx.x;
+    int const lane = tid & 63, wave = tid >> 6;
+    int const lc = lane & 15, lg = lane >> 4;
+    int const wm = wave & 1, wn = wave >> 1;
+
+    // ---- loaders --------------------------------------------------------
+    // "MN" sources (NN A, column-major B): a thread loads a 4 (k) x 4 (row)
+    // micro-block with four 16-byte loads along the rows and writes each
+    // row's 4 k-values as one 8-byte LDS store.  "K" sources (TN A, row-major
+    // B): a thread loads 16 consecutive k of one row and writes 16-byte stores.
+    // A tile: rows n (BN), B tile: rows j (BM); both 128 rows x 32 k.
+    int const mn_c4 = (tid & 31) * 4; // first of 4 rows (columns of the source)
+    int const mn_k4 = (tid >> 5) * 4; // first of 4 k
+    int const k_row = tid >> 1;       // K sources: row
+    int const k_k16 = (tid & 1) * 16; // K sources: first of 16 k
+
+    // A addresses
+    uint64_t a_col_base = 0;
+    bool a_ok;
+    if constexpr (!TN) {
+        uint64_t const n = n0 + mn_c4;
+        a_ok = n < ncols;
+        uint64_t const k = a_ok ? n / w : 0;
+        a_col_base = k * nq * w + (a_ok ? n - k * w : 0);
+    } else {
+        uint64_t const n = n0 + k_row;
+        a_ok = n < ncols;
+        a_col_base = a_ok ? n * nq : 0;
+    }
+    bool const b_vec = BCOL ? (m % 4 == 0) : (nq % 4 == 0);
+
+    f32x4 ra[4], rb[4];
+    auto load = [&](uint64_t l0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            ra[e] = zero4();
+            rb[e] = zero4();
+        }
+        if constexpr (!TN) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                uint64_t const l = l0 + mn_k4 + e;
+                if (a_ok && l < nq) ra[e] = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(a + a_col_base + l * w));
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                uint64_t const l = l0 + k_k16 + 4 * e;
+                if (a_ok && l < nq) ra[e] = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(a + a_col_base + l));
+            }
+        }
+        if constexpr (BCOL) {
+            uint64_t const j = j0 + mn_c4;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                uint64_t const l = l0 + mn_k4 + e;
+                if (l >= nq) continue;
+                if (b_vec && j + 4 <= m) {
+                    rb[e] = *reinterpret_cast<const f32x4 *>(b + j + l * m);
+                } else {
+#pragma unroll
+                    for (int x = 0; x < 4; ++x)
+                        if (j + x < m) rb[e][x] = b[j + x + l * m];
+                }
+            }
+        } else {
+            uint64_t const j = j0 + k_row;
+            if (j < m) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    uint64_t const l = l0 + k_k16 + 4 * e;
+                    if (b_vec && l + 4 <= nq) {
+                        rb[e] = *reinterpret_cast<const f32x4 *>(b + j * nq + l);
+                    } else {
+#pragma unroll
+                        for (int x = 0; x < 4; ++x)
+                            if (l + x < nq) rb[e][x] = b[j * nq + l + x];
+                    }
+                }
+            }
+        }
+    };
+
+    // Convert and store: MN micro-block (ra[e][x] = row x, k e) or K run.
+    auto store_mn = [&](f32x4 (&r)[4], op (*s)[BN * LDK]) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+            v4 out[P];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                op parts[P];
+                L::split(r[e][x], parts);
+#pragma unroll
+                for (int p = 0; p < P; ++p) out[p][e] = parts[p];
+            }
+#pragma unroll
+            for (int p = 0; p < P; ++p) *reinterpret_cast<v4 *>(&s[p][(mn_c4 + x) * LDK + mn_k4]) = out[p];
+        }
+    };
+    auto store_k = [&](f32x4 (&r)[4], op (*s)[BN * LDK]) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            v8 out[P];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                op parts[P];
+                L::split(r[2 * h + e / 4][e % 4], parts);
+#pragma unroll
+                for (int p = 0; p < P; ++p) out[p][e] = parts[p];
+            }
+#pragma unroll
+            for (int p = 0; p < P; ++p) *reinterpret_cast<v8 *>(&s[p][k_row * LDK + k_k16 + 8 * h]) = out[p];
+        }
+    };
+
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[jt][nt] = zero4();
+
+    // Interleaved fragments: tile jt holds rows 4*x + jt (x < 16) of the
+    // wave's 64 rows, tile nt columns 4*y + nt, so a lane's 4 results of a
+    // row (column) are adjacent and leave with one 16-byte store.
+    int const arow = wn * 64 + 4 * lc; // + nt
+    int const brow = wm * 64 + 4 * lc; // + jt
+
+    uint64_t const nslices = (nq + BK - 1) / BK;
+    load(0);
+    for (uint64_t s = 0; s < nslices; ++s) {
+        if constexpr (!TN) store_mn(ra, sA);
+        else store_k(ra, sA);
+        if constexpr (BCOL) store_mn(rb, sB);
+        else store_k(rb, sB);
+        __syncthreads();
+        if (s + 1 < nslices) load((s + 1) * BK);
+        v8 fa[P][4];
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) fa[p][nt] = *reinterpret_cast<const v8 *>(&sA[p][(arow + nt) * LDK + 8 * lg]);
+#pragma unroll
+        for (int jt = 0; jt < 4; ++jt) {
+            v8 fb[P];
+#pragma unroll
+            for (int p = 0; p < P; ++p) fb[p] = *reinterpret_cast<const v8 *>(&sB[p][(brow + jt) * LDK + 8 * lg]);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                if constexpr (P == 3) {
+                    acc[jt][nt] = L::mfma(fb[1], fa[1][nt], acc[jt][nt]);
+                    acc[jt][nt] = L::mfma(fb[0], fa[2][nt], acc[jt][nt]);
+                    acc[jt][nt] = L::mfma(fb[2], fa[0][nt], acc[jt][nt]);
+                    acc[jt][nt] = L::mfma(fb[0], fa[1][nt], acc[jt][nt]);
+                    acc[jt][nt] = L::mfma(fb[1], fa[0][nt], acc[jt][nt]);
+                }
+                acc[jt][nt] = L::mfma(fb[0], fa[0][nt], acc[jt][nt]);
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- epilogue ------------------------------------------------------------
+    uint64_t const ncol0 = n0 + wn * 64 + 4 * lc; // this lane's 4 adjacent columns
+    if constexpr (!TN) {
+        if (ncol0 >= ncols) return;
+        uint64_t const k = ncol0 / w;
+        float *cb = c + k * m * w + (ncol0 - k * w);
+#pragma unroll
+        for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                uint64_t const j = j0 + wm * 64 + 4 * (4 * lg + r) + jt;
+                if (j >= m) continue;
+                f32x4 o = {acc[jt][0][r], acc[jt][1][r], acc[jt][2][r], acc[jt][3][r]};
+                *reinterpret_cast<f32x4 *>(cb + j * w) = o;
+            }
+    } else {
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            uint64_t const k = ncol0 + nt;
+            if (k >= ncols) continue;
+            float *crow = c + k * m;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                uint64_t const jr = j0 + wm * 64 + 4 * (4 * lg + r);
+                if ((m & 3) == 0 && jr + 4 <= m) {
+                    f32x4 o = {acc[0][nt][r], acc[1][nt][r], acc[2][nt][r], acc[3][nt][r]};
+                    *reinterpret_cast<f32x4 *>(crow + jr) = o;
+                } else {
+#pragma unroll
+                    for (int jt = 0; jt < 4; ++jt)
+                        if (jr + jt < m) crow[jr + jt] = acc[jt][nt][r];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP8 e4m3fn with per-slice E8M0 scales: 64 x 64 blocks, K-slices of 128.
+// ---------------------------------------------------------------------------
+constexpr float FP8_MAX = 448.f;
+
+// Power-of-two exponent e with amax * 2^-e <= 448 (0 for an all-zero slice).
+__device__ __forceinline__ int slice_exponent(float amax)
+{
+    if (!(amax > 0.f)) return 0;
+    int e = (int)ceilf(log2f(amax / FP8_MAX));
+    if (ldexpf(amax, -e) > FP8_MAX) ++e;
+    return e < -126 ? -126 : (e > 127 ? 127 : e);
+}
+
+// 32 floats -> 32 e4m3fn bytes scaled by 2^-e (clamped: the conversion
+// instruction returns NaN beyond +-448 instead of saturating).
+__device__ __forceinline__ void quantize32(const float (&x)[32], int e, i32x8 &out)
+{
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        float v[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) v[t] = fminf(fmaxf(ldexpf(x[4 * q + t], -e), -FP8_MAX), FP8_MAX);
+        int w = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+        w = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], w, true);
+        out[q] = w;
+    }
+}
+
+template <bool TN, bool BCOL>
+__global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const float *__restrict__ b,
+                                             float *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w,
+                                             uint64_t m, unsigned nj, unsigned total)
+{
+    constexpr int BM = 64, BN = 64, BK = 128;
+    constexpr int LDK = BK + 16; // bytes per row
+    __shared__ __attribute__((aligned(16))) unsigned char sA[BN * LDK];
+    __shared__ __attribute__((aligned(16))) unsigned char sB[BM * LDK];
+    __shared__ unsigned char sAs[BN * 4];
+    __shared__ unsigned char sBs[BM * 4];
+
+    unsigned jb, nb;
+    lp_block_coords(nj, total, jb, nb);
+    uint64_t const j0 = (uint64_t)jb * BM;
+    uint64_t const n0 = (uint64_t)nb * BN;
+    int const tid = threadIdx.x;
+    int const lane = tid & 63, wave = tid >> 6;
+    int const lc = lane & 15, lg = lane >> 4;
+    int const wm = wave & 1, wn = wave >> 1;
+
+    // every thread owns one (row, K-block) of each tile: row = tid % 64, block = tid / 64
+    int const row = tid & 63, kb = tid >> 6;
+    uint64_t const n = n0 + row;
+    bool const a_ok = n < ncols;
+    uint64_t a_base;
+    if constexpr (!TN) {
+        uint64_t const k = a_ok ? n / w : 0;
+        a_base = k * nq * w + (a_ok ? n - k * w : 0);
+    } else {
+        a_base = a_ok ? n * nq : 0;
+    }
+    uint64_t const j = j0 + row;
+    bool const b_ok = j < m;
+
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = zero4();
+
+    for (uint64_t l0 = 0; l0 < nq; l0 += BK) {
+        float xa[32], xb[32];
+        uint64_t const lb = l0 + 32 * kb;
+#pragma unroll
+        for (int t = 0; t < 32; ++t) {
+            uint64_t const l = lb + t;
+            bool const lv = l < nq;
+            if constexpr (!TN) xa[t] = (a_ok && lv) ? a[a_base + l * w] : 0.f;
+            else xa[t] = (a_ok && lv) ? a[a_base + l] : 0.f;
+            if constexpr (BCOL) xb[t] = (b_ok && lv) ? b[j + l * m] : 0.f;
+            else xb[t] = (b_ok && lv) ? b[j * nq + l] : 0.f;
+        }
+        float amax = 0.f, bmax = 0.f;
+#pragma unroll
+        for (int t = 0; t < 32; ++t) {
+            amax = fmaxf(amax, fabsf(xa[t]));
+            bmax = fmaxf(bmax, fabsf(xb[t]));
+        }
+        int const ea = slice_exponent(amax), eb = slice_exponent(bmax);
+        i32x8 qa, qb;
+        quantize32(xa, ea, qa);
+        quantize32(xb, eb, qb);
+        __syncthreads(); // the previous slice's fragments have been read
+        *reinterpret_cast<i32x4 *>(&sA[row * LDK + 32 * kb]) = {qa[0], qa[1], qa[2], qa[3]};
+        *reinterpret_cast<i32x4 *>(&sA[row * LDK + 32 * kb + 16]) = {qa[4], qa[5], qa[6], qa[7]};
+        *reinterpret_cast<i32x4 *>(&sB[row * LDK + 32 * kb]) = {qb[0], qb[1], qb[2], qb[3]};
+        *reinterpret_cast<i32x4 *>(&sB[row * LDK + 32 * kb + 16]) = {qb[4], qb[5], qb[6], qb[7]};
+        sAs[row * 4 + kb] = (unsigned char)(ea + 127);
+        sBs[row * 4 + kb] = (unsigned char)(eb + 127);
+        __syncthreads();
+        // Fragments: the lane's 32 bytes are k = 16*lg + [0,16) and
+        // k = 64 + 16*lg + [0,16); its scale covers k = 32*lg + [0,32).
+        int sbv[2], sav[2];
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt) {
+            int const rj = wm * 32 + jt * 16 + lc;
+            i32x4 lo = *reinterpret_cast<const i32x4 *>(&sB[rj * LDK + 16 * lg]);
+            i32x4 hi = *reinterpret_cast<const i32x4 *>(&sB[rj * LDK + 64 + 16 * lg]);
+            i32x8 fb = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+            sbv[jt] = sBs[rj * 4 + lg];
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) {
+                int const rn = wn * 32 + nt * 16 + lc;
+                i32x4 alo = *reinterpret_cast<const i32x4 *>(&sA[rn * LDK + 16 * lg]);
+                i32x4 ahi = *reinterpret_cast<const i32x4 *>(&sA[rn * LDK + 64 + 16 * lg]);
+                i32x8 fa = {alo[0], alo[1], alo[2], alo[3], ahi[0], ahi[1], ahi[2], ahi[3]};
+                sav[nt] = sAs[rn * 4 + lg];
+                acc[jt][nt] =
+                    __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fb, fa, acc[jt][nt], 0, 0, 0, sbv[jt], 0, sav[nt]);
+            }
+        }
+        // Keep the scale registers allocated until the slice is done, so that
+        // no later instruction of the slice reuses a register a matrix
+        // instruction may still read its scale from.
+        __syncthreads();
+        asm volatile("" ::"v"(sbv[0]), "v"(sbv[1]), "v"(sav[0]), "v"(sav[1]));
+    }
+
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        uint64_t const col = n0 + wn * 32 + nt * 16 + lc;
+        if (col >= ncols) continue;
+        float *cbase;
+        uint64_t stride;
+        if constexpr (!TN) {
+            uint64_t const k = col / w;
+            cbase = c + k * m * w + (col - k * w);
+            stride = w;
+        } else {
+            cbase = c + col * m;
+            stride = 1;
+        }
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                uint64_t const jj = j0 + wm * 32 + jt * 16 + 4 * lg + r;
+                if (jj < m) cbase[jj * stride] = acc[jt][nt][r];
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Generic reduced-precision kernel (any shape / alignment): one thread per
+// element of C, the same operand rounding, FP32 accumulation.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float fp8_value(float x, int e)
+{
+    float v = fminf(fmaxf(ldexpf(x, -e), -FP8_MAX), FP8_MAX);
+    int const wq = __builtin_amdgcn_cvt_pk_fp8_f32(v, 0.f, 0, false);
+    return ldexpf(__builtin_amdgcn_cvt_f32_fp8(wq, 0), e);
+}
+
+__global__ __launch_bounds__(256) void k_lowp_generic(const float *__restrict__ a, const float *__restrict__ b,
+                                                      float *__restrict__ c, uint64_t outer, uint64_t nq, uint64_t w,
+                                                      uint64_t m, uint64_t bsj, uint64_t bsl, int prec)
+{
+    uint64_t const total = outer * m * w;
+    uint64_t const step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += step) {
+        uint64_t const i = idx % w;
+        uint64_t const kj = idx / w;
+        uint64_t const j = kj % m;
+        uint64_t const k = kj / m;
+        float const *ap = a + k * nq * w + i;
+        float const *bp = b + j * bsj;
+        float s = 0.f;
+        for (uint64_t l0 = 0; l0 < nq; l0 += 32) {
+            uint64_t const l1 = l0 + 32 < nq ? l0 + 32 : nq;
+            int ea = 0, eb = 0;
+            if (prec == TTM_PREC_FP8_E4M3) {
+                float amax = 0.f, bmax = 0.f;
+                for (uint64_t l = l0; l < l1; ++l) {
+                    amax = fmaxf(amax, fabsf(ap[l * w]));
+                    bmax = fmaxf(bmax, fabsf(bp[l * bsl]));
+                }
+                ea = slice_exponent(amax);
+                eb = slice_exponent(bmax);
+            }
+            for (uint64_t l = l0; l < l1; ++l) {
+                float const x = ap[l * w], y = bp[l * bsl];
+                switch (prec) {
+                case TTM_PREC_BF16: s = fmaf((float)(__bf16)y, (float)(__bf16)x, s); break;
+                case TTM_PREC_FP16: s = fmaf((float)(_Float16)y, (float)(_Float16)x, s); break;
+                case TTM_PREC_FP8_E4M3: s = fmaf(fp8_value(y, eb), fp8_value(x, ea), s); break;
+                default: {
+                    __bf16 px[3], py[3];
+                    Lp<LP_BF16X3>::split(x, px);
+                    Lp<LP_BF16X3>::split(y, py);
+                    float t = (float)py[1] * (float)px[1] + (float)py[0] * (float)px[2] + (float)py[2] * (float)px[0];
+                    t += (float)py[0] * (float)px[1] + (float)py[1] * (float)px[0];
+                    s += t + (float)py[0] * (float)px[0];
+                }
+                }
+            }
+        }
+        c[idx] = s;
+    }
+}
+
+template <int MODE, bool TN, bool BCOL>
+void launch_lp(const Fold &f, const float *a, const float *b, float *c, hipStream_t s)
+{
+    uint64_t const ncols = TN ? f.outer : f.outer * f.w;
+    unsigned const nj = (unsigned)((f.m + 127) / 128);
+    unsigned const total = nj * (unsigned)((ncols + 127) / 128);
+    hipLaunchKernelGGL((k_lowp<MODE, TN, BCOL>), dim3(total), dim3(256), 0, s, a, b, c, ncols, f.nq, f.w, f.m, nj,
+                       total);
+}
+
+template <bool TN, bool BCOL>
+void launch_fp8(const Fold &f, const float *a, const float *b, float *c, hipStream_t s)
+{
+    uint64_t const ncols = TN ? f.outer : f.outer * f.w;
+    unsigned const nj = (unsigned)((f.m + 63) / 64);
+    unsigned const total = nj * (unsigned)((ncols + 63) / 64);
+    hipLaunchKernelGGL((k_fp8<TN, BCOL>), dim3(total), dim3(256), 0, s, a, b, c, ncols, f.nq, f.w, f.m, nj, total);
+}
+
+template <int MODE>
+void launch_lp_mode(const Fold &f, const float *a, const float *b, float *c, hipStream_t s)
+{
+    bool const tn = f.w == 1;
+    if (tn) {
+        if (f.b_colmajor) launch_lp<MODE, true, true>(f, a, b, c, s);
+        else launch_lp<MODE, true, false>(f, a, b, c, s);
+    } else {
+        if (f.b_colmajor) launch_lp<MODE, false, true>(f, a, b, c, s);
+        else launch_lp<MODE, false, false>(f, a, b, c, s);
+    }
+}
+
+} // namespace
+
+bool lowp_supports(const Fold &f, bool aligned)
+{
+    if (!aligned) return false;
+    bool const shape_ok = f.w == 1 ? f.nq % 4 == 0 : f.w % 4 == 0;
+    if (!shape_ok) return false;
+    uint64_t const ncols = f.w == 1 ? f.outer : f.outer * f.w;
+    uint64_t const blocks = ((f.m + 63) / 64) * ((ncols + 63) / 64);
+    return blocks < (1ull << 31);
+}
+
+std::string lowp_symbol(const Fold &f, ttm_precision_t prec, bool generic)
+{
+    char const *tn = f.w == 1 ? "true" : "false";
+    char const *bc = f.b_colmajor ? "true" : "false";
+    char buf[96];
+    if (generic) return "k_lowp_generic";
+    if (prec == TTM_PREC_FP8_E4M3) {
+        std::snprintf(buf, sizeof buf, "k_fp8<%s, %s>", tn, bc);
+    } else {
+        int const mode = prec == TTM_PREC_BF16 ? LP_BF16 : (prec == TTM_PREC_FP16 ? LP_FP16 : LP_BF16X3);
+        std::snprintf(buf, sizeof buf, "k_lowp<%d, %s, %s>", mode, tn, bc);
+    }
+    return buf;
+}
+
+ttm_status_t launch_lowp(const Fold &f, const float *a, const float *b, float *c, ttm_precision_t prec,
+                         hipStream_t s, bool generic)
+{
+    if (generic) {
+        uint64_t const total = f.numel_c();
+        uint64_t blocks = (total + 255) / 256;
+        if (blocks > (1u << 20)) blocks = 1u << 20;
+        hipLaunchKernelGGL(k_lowp_generic, dim3((unsigned)blocks), dim3(256), 0, s, a, b, c, f.outer, f.nq, f.w,
+                           f.m, f.b_stride_j(), f.b_stride_l(), (int)prec);
+    } else if (prec == TTM_PREC_FP8_E4M3) {
+        bool const tn = f.w == 1;
+        if (tn) {
+            if (f.b_colmajor) launch_fp8<true, true>(f, a, b, c, s);
+            else launch_fp8<true, false>(f, a, b, c, s);
+        } else {
+            if (f.b_colmajor) launch_fp8<false, true>(f, a, b, c, s);
+            else launch_fp8<false, false>(f, a, b, c, s);
+        }
+    } else if (prec == TTM_PREC_BF16) {
+        launch_lp_mode<LP_BF16>(f, a, b, c, s);
+    } else if (prec == TTM_PREC_FP16) {
+        launch_lp_mode<LP_FP16>(f, a, b, c, s);
+    } else {
+        launch_lp_mode<LP_BF16X3>(f, a, b, c, s);
+    }
+    hipError_t const e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(TTM_ERR_HIP, hipGetErrorString(e));
+        return TTM_ERR_HIP;
+    }
+    return TTM_SUCCESS;
+}
+
+} // namespace ttm_amd

@@ -1,0 +1,25 @@
+// Reduced-precision arithmetic (BF16, FP16, BF16X3, FP8 e4m3fn) for float32 products.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "ttm_amd.h"
+#include "ttm_plan.hpp"
+
+namespace ttm_amd
+{
+
+// True when the MFMA kernels can run the fold: 16-byte aligned buffers and
+// either w == 1 with nq % 4 == 0 (TN view) or w % 4 == 0 (NN view).
+bool lowp_supports(const Fold &f, bool aligned);
+
+// Template instance launched for a fold and precision (as profilers print it).
+std::string lowp_symbol(const Fold &f, ttm_precision_t prec, bool generic);
+
+// `generic` selects the one-thread-per-element kernel (any shape/alignment).
+ttm_status_t launch_lowp(const Fold &f, const float *a, const float *b, float *c, ttm_precision_t prec,
+                         hipStream_t s, bool generic);
+
+} // namespace ttm_amd
