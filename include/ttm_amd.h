@@ -122,6 +122,48 @@ ttm_status_t ttm_sttm_host_ex(unsigned q, unsigned p,
                               const size_t *pib, float *c, const size_t *nc, const size_t *wc,
                               ttm_precision_t precision);
 
+/* --- batched multi-TTM chains (Tucker-style) ------------------------------ */
+/* No single reference counterpart: generalises ttmpy.ttms (the sequential
+ * multi-TTM chain of ttmpy/src/wrapped_ttm.cpp:81-222) to many tensors of
+ * uneven extents in one plan.  Tensor t of a plan computes
+ *     C = A x_{k1} U_{k1} x_{k2} U_{k2} ...   over every mode k with u[k-1] != NULL
+ * where U_k is the row-major r[k-1] x n[k-1] factor of mode k; modes with
+ * u[k-1] == NULL keep their extent.  A and C are dense in layout pi, in device
+ * memory.  Modes are contracted in layout order; every stage reads rows that
+ * are contiguous along the contracted mode and writes that mode slowest, so the
+ * result lands back in layout pi without any transposition. */
+#define TTM_CHAIN_MAX_ORDER 8
+typedef struct ttm_chain_desc_t {
+    unsigned p;                         /* order, 1..TTM_CHAIN_MAX_ORDER */
+    size_t n[TTM_CHAIN_MAX_ORDER];      /* extents of A */
+    size_t pi[TTM_CHAIN_MAX_ORDER];     /* one-based layout of A and C, fastest first */
+    const void *a;                      /* A (float or double, as the plan) */
+    const void *u[TTM_CHAIN_MAX_ORDER]; /* factor of mode k+1, row-major r[k] x n[k], or NULL */
+    size_t r[TTM_CHAIN_MAX_ORDER];      /* rows of u[k] = extent of mode k+1 in C */
+    void *c;                            /* C: extents r[k] where u[k] != NULL, else n[k] */
+} ttm_chain_desc_t;
+
+typedef struct ttm_chain_plan_s *ttm_chain_plan_t;
+
+/* Validate `count` descriptors and prepare a plan (allocates device memory for
+ * descriptors, tile maps and intermediates).  precision: TTM_PREC_DEFAULT
+ * (float or double as is_double says) or, for float, TTM_PREC_BF16X3. */
+ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, int is_double,
+                                   ttm_precision_t precision, ttm_chain_plan_t *plan);
+/* Run every chain of the plan: one persistent grouped launch per stage,
+ * ordered on `stream`; allocates nothing. */
+ttm_status_t ttm_chain_run(ttm_chain_plan_t plan, ttm_stream_t stream);
+/* The same as one hipGraph: captured on the first call, replayed afterwards. */
+ttm_status_t ttm_chain_run_graph(ttm_chain_plan_t plan, ttm_stream_t stream);
+/* Time each compiled tile shape per stage on the plan's own buffers, keep the
+ * fastest (also for later plans of the same shape class in this process, and
+ * in the file named by TTM_AMD_TUNE_CACHE when set).  Drops a captured graph. */
+ttm_status_t ttm_chain_tune(ttm_chain_plan_t plan, ttm_stream_t stream);
+/* Stages, launches and the tile shape (kernel instance) of stage `s`. */
+unsigned ttm_chain_stages(ttm_chain_plan_t plan);
+const char *ttm_chain_stage_kernel(ttm_chain_plan_t plan, unsigned s);
+void ttm_chain_plan_destroy(ttm_chain_plan_t plan);
+
 /* --- diagnostics -------------------------------------------------------- */
 /* Message of the last failing call on this thread ("" after a success). */
 const char *ttm_last_error(void);

@@ -1,0 +1,45 @@
+"""Validation of batched multi-TTM chain descriptors (no GPU: every call fails before device use)."""
+import ctypes
+
+import pytest
+
+
+def _desc(p=3, n=(4, 5, 6), pi=(1, 2, 3), r=(2, 3, 4), a=0x1000, c=0x2000, factors=True):
+    from ttm_amd.chain import ChainDesc
+    d = ChainDesc()
+    d.p = p
+    for k in range(min(p, 8)):
+        d.n[k] = n[k] if k < len(n) else 1
+        d.pi[k] = pi[k] if k < len(pi) else 0
+        d.r[k] = r[k] if k < len(r) else 1
+        d.u[k] = 0x3000 + 64 * k if factors else None
+    d.a = a
+    d.c = c
+    return d
+
+
+@pytest.mark.parametrize("kwargs,status", [
+    (dict(p=0), 25), (dict(p=9), 25), (dict(a=None), 25), (dict(c=None), 25),
+    (dict(pi=(1, 1, 3)), 16), (dict(pi=(0, 2, 3)), 16), (dict(pi=(1, 2, 4)), 16),
+    (dict(n=(4, 0, 6)), 14), (dict(r=(2, 0, 4)), 15),
+])
+def test_invalid_descriptors(ttmlib, kwargs, status):
+    from ttm_amd import chain
+    L = chain._bind()
+    descs = (chain.ChainDesc * 1)(_desc(**kwargs))
+    h = ctypes.c_void_p()
+    st = L.ttm_chain_plan_create(1, descs, 0, 0, ctypes.byref(h))
+    assert st == status
+    assert not h.value
+    assert L.ttm_last_error().decode().startswith("Error in tlib::tensor_times_matrix")
+
+
+def test_precision_rules(ttmlib):
+    from ttm_amd import chain
+    L = chain._bind()
+    descs = (chain.ChainDesc * 1)(_desc())
+    h = ctypes.c_void_p()
+    assert L.ttm_chain_plan_create(1, descs, 1, 1, ctypes.byref(h)) == 22   # BF16X3 needs float
+    assert L.ttm_chain_plan_create(1, descs, 0, 2, ctypes.byref(h)) == 22   # only default / BF16X3
+    assert L.ttm_chain_plan_create(1, None, 0, 0, ctypes.byref(h)) == 25
+    assert L.ttm_chain_plan_create(1, descs, 0, 0, None) == 25

@@ -1,0 +1,162 @@
+"""Batched multi-TTM chains on the GPU against the float64 definition.
+
+The reference of every chain is the same product computed one mode at a
+time with numpy.einsum in float64 (the textbook definition; the chain's
+mode order does not change the exact result).  Bars: float64 1e-12 relative,
+float32 normwise 4 (sum n_q) eps_32 relative to ||A|| prod ||U_k||, BF16X3
+normwise-relative 1e-6.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def einsum_chain(A, Us):
+    """A (natural mode order, float64) times U_k in every mode k with U_k not None."""
+    X = A.astype(np.float64)
+    p = X.ndim
+    letters = "abcdefgh"
+    for k, U in enumerate(Us):
+        if U is None:
+            continue
+        sub = letters[:p]
+        out = sub[:k] + "z" + sub[k + 1:]
+        X = np.einsum(f"{sub},z{sub[k]}->{out}", X, U.astype(np.float64))
+    return X
+
+
+def to_layout(torch, dev, X, pi, dtype):
+    """Device tensor with the values of X (natural order) stored dense in layout pi."""
+    order = [m - 1 for m in reversed(pi)]               # slowest mode first
+    T = torch.from_numpy(np.ascontiguousarray(np.transpose(X, order))).to(dev, dtype)
+    inv = np.argsort(order)
+    return T.permute(*[int(i) for i in inv])
+
+
+def random_items(torch, dev, rng, count, dtype, max_ext=12, max_rank=10, orders=(1, 2, 3, 4)):
+    items, refs = [], []
+    for t in range(count):
+        p = int(rng.choice(orders))
+        na = [int(x) for x in rng.integers(1, max_ext + 1, size=p)]
+        pi = [int(x) + 1 for x in rng.permutation(p)]
+        A = rng.uniform(-1, 1, size=na)
+        Us = []
+        for k in range(p):
+            if rng.random() < 0.25:
+                Us.append(None)
+            else:
+                Us.append(rng.uniform(-1, 1, size=(int(rng.integers(1, max_rank + 1)), na[k])))
+        nc = [U.shape[0] if U is not None else n for U, n in zip(Us, na)]
+        At = to_layout(torch, dev, A, pi, dtype)
+        Ut = [torch.from_numpy(U).to(dev, dtype).contiguous() if U is not None else None for U in Us]
+        Ct = to_layout(torch, dev, np.zeros(nc), pi, dtype)
+        items.append((At, Ut, Ct))
+        refs.append((A, Us, einsum_chain(A, Us)))
+    return items, refs
+
+
+def check(items, refs, dtype, label, prec=None):
+    import torch
+    torch.cuda.synchronize()
+    eps = np.finfo(np.float32).eps
+    for (At, Ut, Ct), (A, Us, ref) in zip(items, refs):
+        got = Ct.double().cpu().numpy()
+        err = np.linalg.norm(got - ref)
+        if dtype == torch.float64:
+            assert err <= 1e-12 * max(np.linalg.norm(ref), 1.0), (label, A.shape)
+        elif prec == "bf16x3":
+            assert err <= 1e-6 * max(np.linalg.norm(ref), 1e-30) + 1e-12, (label, A.shape, err)
+        else:
+            nqs = sum(n for n, U in zip(A.shape, Us) if U is not None) + sum(A.shape)
+            scale = np.linalg.norm(A) * np.prod([np.linalg.norm(U) for U in Us if U is not None])
+            assert err <= 4 * nqs * eps * scale, (label, A.shape, err / scale)
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32", "bf16x3"])
+def test_random_batch(gpu, dt):
+    import torch
+    from ttm_amd.chain import ChainPlan
+    dtype = torch.float64 if dt == "f64" else torch.float32
+    rng = np.random.default_rng({"f64": 1, "f32": 2, "bf16x3": 3}[dt])
+    items, refs = random_items(torch, gpu, rng, 24, dtype)
+    plan = ChainPlan(items, precision="bf16x3" if dt == "bf16x3" else None)
+    _say(dt, "stage kernels", plan.stage_kernels)
+    plan.run()
+    check(items, refs, dtype, f"{dt} run", prec=dt)
+    plan.close()
+
+
+def test_graph_replay_matches_eager(gpu):
+    """Replays of the captured chain, with eager work in between, equal the eager run."""
+    import torch
+    from ttm_amd.chain import ChainPlan
+    rng = np.random.default_rng(7)
+    items, refs = random_items(torch, gpu, rng, 16, torch.float32, max_ext=20, max_rank=12, orders=(3,))
+    plan = ChainPlan(items)
+    plan.run()
+    torch.cuda.synchronize()
+    eager = [C.clone() for _, _, C in items]
+    for rep in range(4):
+        for _, _, C in items:
+            C.fill_(float("nan"))
+        torch.cuda.synchronize()
+        plan.run_graph()
+        torch.cuda.synchronize()
+        junk = torch.randn(1000, device=gpu).sum()  # eager work between replays
+        for (_, _, C), E in zip(items, eager):
+            assert torch.equal(C, E), f"replay {rep}"
+        del junk
+    check(items, refs, torch.float32, "graph")
+    plan.close()
+
+
+def test_tune_keeps_results(gpu, tmp_path, monkeypatch):
+    import torch
+    from ttm_amd.chain import ChainPlan
+    monkeypatch.setenv("TTM_AMD_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    rng = np.random.default_rng(9)
+    items, refs = random_items(torch, gpu, rng, 12, torch.float32, max_ext=40, max_rank=40, orders=(3,))
+    plan = ChainPlan(items)
+    plan.tune()
+    _say("tuned stage kernels", plan.stage_kernels)
+    for _, _, C in items:
+        C.zero_()
+    plan.run()
+    check(items, refs, torch.float32, "tuned")
+    plan.close()
+
+
+def orthonormal_rows(rng, r, n):
+    Q, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    return Q[:, :r].T.copy()  # r x n with orthonormal rows
+
+
+@pytest.mark.parametrize("prec", [None, "bf16x3"])
+def test_config5_reduced(gpu, prec):
+    """Tucker chains A x1 U1 x2 U2 x3 U3 over uneven first-order tensors (config 5 at reduced size)."""
+    import torch
+    from ttm_amd.chain import ChainPlan
+    rng = np.random.default_rng(0x77140005)
+    items, refs = [], []
+    for t in range(32):
+        na = [int(x) for x in rng.integers(16, 65, size=3)]
+        rk = [int(x) for x in rng.integers(4, 17, size=3)]
+        A = rng.uniform(-1, 1, size=na)
+        Us = [orthonormal_rows(rng, r, n) for r, n in zip(rk, na)]
+        At = to_layout(torch, gpu, A, [1, 2, 3], torch.float32)
+        Ut = [torch.from_numpy(U).to(gpu, torch.float32).contiguous() for U in Us]
+        Ct = to_layout(torch, gpu, np.zeros(rk), [1, 2, 3], torch.float32)
+        items.append((At, Ut, Ct))
+        refs.append((A, Us, einsum_chain(A.astype(np.float32), [U.astype(np.float32) for U in Us])))
+    plan = ChainPlan(items, precision=prec)
+    plan.run_graph()
+    check(items, refs, torch.float32, f"cfg5 {prec}", prec=prec)
+    plan.close()

@@ -15,3 +15,6 @@ from . import ttmpy
 
 __all__ = ["LIB_PATH", "Precision", "Status", "TTMError", "force_kernel", "lib", "plan_kernel", "plan_kernel_symbol",
            "version", "dense_strides", "layout_from_strides", "ttm", "ttm_raw", "ttmpy"]
+from . import chain  # noqa: E402  (batched multi-TTM chains)
+
+__all__.append("chain")

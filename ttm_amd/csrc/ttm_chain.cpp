@@ -1,0 +1,477 @@
+// Batched multi-TTM chains: plan, persistent grouped launches, hipGraph
+// replay and the tile-shape autotuner (declared in include/ttm_amd.h).
+//
+// A chain over the modes of one tensor is run in layout order pi_1, pi_2, ...:
+// stage s reads the current tensor as rows that are contiguous along mode
+// pi_s (it is the fastest mode), multiplies them by U_{pi_s} and writes the
+// result with that mode slowest (the TT view of ttm_gemm.hip).  The layout
+// therefore rotates by one position per stage and after p stages is pi again.
+// A mode without a factor is carried through by the identity.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "ttm_amd.h"
+#include "ttm_gemm.hpp"
+#include "ttm_lowp.hpp"
+#include "ttm_plan.hpp"
+
+using namespace ttm_amd;
+
+namespace
+{
+
+struct Stage
+{
+    std::vector<GroupProblem> probs;   // tile_begin / nj filled for `cfg`
+    std::vector<unsigned> tile_prob;
+    GroupConfig cfg{4, 4, 1, 16};
+    unsigned total = 0;
+    unsigned grid = 0;
+    GroupProblem *d_probs = nullptr;   // inside the plan's tile allocation
+    unsigned *d_tile_prob = nullptr;
+    std::string key;                   // shape class for the tuning cache
+};
+
+// ---- tuning cache ---------------------------------------------------------
+std::mutex g_tune_mutex;
+std::map<std::string, int> g_tune_cache;
+bool g_tune_loaded = false;
+
+const char *tune_file() { return std::getenv("TTM_AMD_TUNE_CACHE"); }
+
+void tune_load_locked()
+{
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    const char *path = tune_file();
+    if (!path) return;
+    std::ifstream in(path);
+    std::string key;
+    int idx;
+    while (in >> key >> idx)
+        if (idx >= 0 && idx < group_config_count()) g_tune_cache[key] = idx;
+}
+
+void tune_store_locked()
+{
+    const char *path = tune_file();
+    if (!path) return;
+    std::ofstream out(path, std::ios::trunc);
+    for (auto const &kv : g_tune_cache) out << kv.first << ' ' << kv.second << '\n';
+}
+
+int log2_class(uint64_t x)
+{
+    int c = 0;
+    while ((1ull << c) < x && c < 63) ++c;
+    return c;
+}
+
+} // namespace
+
+struct ttm_chain_plan_s
+{
+    bool is_double = false;
+    ttm_precision_t prec = TTM_PREC_DEFAULT;
+    std::vector<Stage> stages;
+    void *d_static = nullptr;   // identities + intermediates
+    void *d_tiles = nullptr;    // descriptors + tile maps of all stages
+    size_t tiles_bytes = 0;
+    int num_cu = 0;
+    hipGraphExec_t graph = nullptr;
+    std::string symbol;
+};
+
+namespace
+{
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+ttm_status_t fail(ttm_status_t st, const char *detail)
+{
+    set_error(st, detail);
+    return st;
+}
+
+ttm_status_t hip_check(hipError_t e)
+{
+    if (e == hipSuccess) return TTM_SUCCESS;
+    (void)hipGetLastError();
+    set_error(TTM_ERR_HIP, hipGetErrorString(e));
+    return TTM_ERR_HIP;
+}
+
+// Heuristic tile shape for a stage: rows just above the largest rank, wide
+// column blocks (the stages stream their input once).
+GroupConfig default_config(uint64_t max_m, bool bf16x3)
+{
+    if (bf16x3) return GroupConfig{4, 4, 2, 32}; // the reduced-precision kernel has one shape
+    if (max_m <= 16) return group_config(0);
+    if (max_m <= 32) return group_config(1);
+    if (max_m <= 64) return group_config(2);
+    return group_config(3);
+}
+
+bool config_allowed(const GroupConfig &cfg, bool is_double)
+{
+    (void)is_double;
+    return cfg.nt * 16 * (4 / cfg.waves_m) >= 64;
+}
+
+// (Re)build the tile list of a stage for its configuration and upload it.
+ttm_status_t layout_tiles(ttm_chain_plan_t plan)
+{
+    size_t bytes = 0;
+    for (Stage &st : plan->stages) {
+        unsigned const bm = (unsigned)st.cfg.bm(), bn = (unsigned)st.cfg.bn();
+        st.tile_prob.clear();
+        unsigned begin = 0;
+        for (size_t i = 0; i < st.probs.size(); ++i) {
+            GroupProblem &P = st.probs[i];
+            unsigned const nj = (unsigned)((P.m + bm - 1) / bm);
+            uint64_t const nn = (P.ncols + bn - 1) / bn;
+            uint64_t const tiles = (uint64_t)nj * nn;
+            if (begin + tiles >= (1ull << 31)) return fail(TTM_ERR_ARGUMENT, "too many tiles in one stage");
+            P.tile_begin = begin;
+            P.nj = nj;
+            st.tile_prob.insert(st.tile_prob.end(), (size_t)tiles, (unsigned)i);
+            begin += (unsigned)tiles;
+        }
+        st.total = begin;
+        int occ = plan->prec == TTM_PREC_BF16X3 ? lowp_group_occupancy() : group_occupancy(st.cfg, plan->is_double);
+        if (occ <= 0) occ = 1;
+        uint64_t const resident = (uint64_t)occ * (uint64_t)std::max(plan->num_cu, 1);
+        st.grid = (unsigned)std::min<uint64_t>(st.total, resident);
+        bytes = align_up(bytes, 256) + st.probs.size() * sizeof(GroupProblem);
+        bytes = align_up(bytes, 256) + st.tile_prob.size() * sizeof(unsigned);
+    }
+    if (plan->d_tiles && bytes > plan->tiles_bytes) {
+        (void)hipFree(plan->d_tiles);
+        plan->d_tiles = nullptr;
+    }
+    if (!plan->d_tiles) {
+        if (hipMalloc(&plan->d_tiles, std::max<size_t>(bytes, 256)) != hipSuccess) {
+            (void)hipGetLastError();
+            plan->d_tiles = nullptr;
+            return fail(TTM_ERR_ALLOC, "chain tile maps");
+        }
+        plan->tiles_bytes = std::max<size_t>(bytes, 256);
+    }
+    size_t off = 0;
+    char *base = static_cast<char *>(plan->d_tiles);
+    for (Stage &st : plan->stages) {
+        off = align_up(off, 256);
+        st.d_probs = reinterpret_cast<GroupProblem *>(base + off);
+        ttm_status_t s = hip_check(
+            hipMemcpy(st.d_probs, st.probs.data(), st.probs.size() * sizeof(GroupProblem), hipMemcpyHostToDevice));
+        if (s) return s;
+        off += st.probs.size() * sizeof(GroupProblem);
+        off = align_up(off, 256);
+        st.d_tile_prob = reinterpret_cast<unsigned *>(base + off);
+        s = hip_check(hipMemcpy(st.d_tile_prob, st.tile_prob.data(), st.tile_prob.size() * sizeof(unsigned),
+                                hipMemcpyHostToDevice));
+        if (s) return s;
+        off += st.tile_prob.size() * sizeof(unsigned);
+    }
+    return TTM_SUCCESS;
+}
+
+ttm_status_t launch_stage(ttm_chain_plan_t plan, const Stage &st, hipStream_t s)
+{
+    if (st.total == 0) return TTM_SUCCESS;
+    if (plan->prec == TTM_PREC_BF16X3) return launch_lowp_grouped(st.d_probs, st.d_tile_prob, st.total, st.grid, s);
+    if (plan->is_double)
+        return launch_gemm_grouped<double>(st.cfg, st.d_probs, st.d_tile_prob, st.total, st.grid, s);
+    return launch_gemm_grouped<float>(st.cfg, st.d_probs, st.d_tile_prob, st.total, st.grid, s);
+}
+
+void drop_graph(ttm_chain_plan_t plan)
+{
+    if (plan->graph) {
+        (void)hipGraphExecDestroy(plan->graph);
+        plan->graph = nullptr;
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, int is_double,
+                                   ttm_precision_t precision, ttm_chain_plan_t *plan_out)
+{
+    if (!plan_out) return fail(TTM_ERR_ARGUMENT, "plan pointer is null");
+    *plan_out = nullptr;
+    if (count > 0 && !descs) return fail(TTM_ERR_ARGUMENT, "descriptor array is null");
+    if (precision != TTM_PREC_DEFAULT && (is_double || precision != TTM_PREC_BF16X3))
+        return fail(TTM_ERR_PRECISION, "chains run in the plain precision or, for float, BF16X3");
+    size_t const esz = is_double ? sizeof(double) : sizeof(float);
+    size_t const vec = 16 / esz;
+
+    // ---- validation and sizes ------------------------------------------------
+    size_t max_p = 0, ws_total = 0;
+    std::vector<size_t> ws_of(count, 0);
+    std::vector<uint64_t> ident_sizes; // distinct extents that need an identity factor
+    for (size_t t = 0; t < count; ++t) {
+        ttm_chain_desc_t const &d = descs[t];
+        char msg[96];
+        std::snprintf(msg, sizeof msg, "chain descriptor %zu", t);
+        if (d.p == 0 || d.p > TTM_CHAIN_MAX_ORDER) return fail(TTM_ERR_ARGUMENT, msg);
+        if (!d.a || !d.c) return fail(TTM_ERR_ARGUMENT, msg);
+        bool seen[TTM_CHAIN_MAX_ORDER + 1] = {false};
+        for (unsigned r = 0; r < d.p; ++r) {
+            if (d.pi[r] == 0 || d.pi[r] > d.p || seen[d.pi[r]]) return fail(TTM_ERR_LAYOUT_A, msg);
+            seen[d.pi[r]] = true;
+            if (d.n[r] == 0) return fail(TTM_ERR_SHAPE_A, msg);
+            if (d.u[r] && d.r[r] == 0) return fail(TTM_ERR_SHAPE_C, msg);
+        }
+        max_p = std::max<size_t>(max_p, d.p);
+        // intermediate sizes after each stage but the last
+        uint64_t cur[TTM_CHAIN_MAX_ORDER];
+        for (unsigned r = 0; r < d.p; ++r) cur[r] = d.n[r];
+        uint64_t biggest = 0;
+        for (unsigned s = 0; s < d.p; ++s) {
+            unsigned const mode = (unsigned)d.pi[s] - 1;
+            if (!d.u[mode] && std::find(ident_sizes.begin(), ident_sizes.end(), d.n[mode]) == ident_sizes.end())
+                ident_sizes.push_back(d.n[mode]);
+            cur[mode] = d.u[mode] ? d.r[mode] : d.n[mode];
+            if (s + 1 < d.p) {
+                uint64_t sz = 1;
+                for (unsigned r = 0; r < d.p; ++r) sz *= cur[r];
+                biggest = std::max(biggest, sz);
+            }
+        }
+        ws_of[t] = align_up((size_t)biggest * esz, 256);
+        ws_total += 2 * ws_of[t];
+    }
+
+    auto *plan = new ttm_chain_plan_s;
+    plan->is_double = is_double != 0;
+    plan->prec = precision;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        delete plan;
+        return fail(TTM_ERR_HIP, "device query failed");
+    }
+    plan->num_cu = prop.multiProcessorCount;
+
+    // ---- static allocation: identities, then ping-pong intermediates --------
+    size_t ident_bytes = 0;
+    for (uint64_t n : ident_sizes) ident_bytes = align_up(ident_bytes + (size_t)(n * n) * esz, 256);
+    size_t const static_bytes = std::max<size_t>(ident_bytes + ws_total, 256);
+    if (hipMalloc(&plan->d_static, static_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        delete plan;
+        return fail(TTM_ERR_ALLOC, "chain intermediates");
+    }
+    char *sbase = static_cast<char *>(plan->d_static);
+    std::map<uint64_t, const void *> ident_ptr;
+    {
+        size_t off = 0;
+        for (uint64_t n : ident_sizes) {
+            std::vector<char> host((size_t)(n * n) * esz, 0);
+            for (uint64_t i = 0; i < n; ++i) {
+                if (is_double) reinterpret_cast<double *>(host.data())[i * n + i] = 1.0;
+                else reinterpret_cast<float *>(host.data())[i * n + i] = 1.0f;
+            }
+            if (hipMemcpy(sbase + off, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+                (void)hipGetLastError();
+                ttm_chain_plan_destroy(plan);
+                return fail(TTM_ERR_HIP, "identity upload");
+            }
+            ident_ptr[n] = sbase + off;
+            off = align_up(off + host.size(), 256);
+        }
+    }
+
+    // ---- stages ---------------------------------------------------------------
+    plan->stages.resize(max_p);
+    std::vector<uint64_t> stage_max_m(max_p, 0), stage_cols(max_p, 0), stage_nq(max_p, 0);
+    size_t ws_off = ident_bytes;
+    for (size_t t = 0; t < count; ++t) {
+        ttm_chain_desc_t const &d = descs[t];
+        char *buf[2] = {sbase + ws_off, sbase + ws_off + ws_of[t]};
+        ws_off += 2 * ws_of[t];
+        uint64_t cur[TTM_CHAIN_MAX_ORDER];
+        for (unsigned r = 0; r < d.p; ++r) cur[r] = d.n[r];
+        const void *in = d.a;
+        for (unsigned s = 0; s < d.p; ++s) {
+            unsigned const mode = (unsigned)d.pi[s] - 1;
+            GroupProblem P{};
+            P.a = in;
+            P.nq = cur[mode];
+            P.m = d.u[mode] ? d.r[mode] : d.n[mode];
+            P.b = d.u[mode] ? d.u[mode] : ident_ptr[d.n[mode]];
+            uint64_t ncols = 1;
+            for (unsigned r = 0; r < d.p; ++r)
+                if (r != mode) ncols *= cur[r];
+            P.ncols = ncols;
+            void *out = (s + 1 == d.p) ? d.c : static_cast<void *>(buf[s & 1]);
+            P.c = out;
+            bool const a_al = (reinterpret_cast<uintptr_t>(P.a) & 15u) == 0;
+            bool const c_al = (reinterpret_cast<uintptr_t>(P.c) & 15u) == 0;
+            P.flags = ((a_al && P.nq % vec == 0) ? TILE_A_VEC : 0u) | ((c_al && ncols % vec == 0) ? TILE_C_VEC : 0u);
+            plan->stages[s].probs.push_back(P);
+            stage_max_m[s] = std::max(stage_max_m[s], P.m);
+            stage_cols[s] += ncols;
+            stage_nq[s] = std::max(stage_nq[s], P.nq);
+            cur[mode] = P.m;
+            in = out;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        tune_load_locked();
+        for (size_t s = 0; s < max_p; ++s) {
+            Stage &st = plan->stages[s];
+            std::ostringstream key;
+            key << (is_double ? "f64" : "f32") << "/p" << (int)precision << "/m" << log2_class(stage_max_m[s])
+                << "/nq" << log2_class(stage_nq[s]) << "/cols" << log2_class(stage_cols[s]) << "/n"
+                << log2_class(st.probs.size());
+            st.key = key.str();
+            st.cfg = default_config(stage_max_m[s], precision == TTM_PREC_BF16X3);
+            auto it = g_tune_cache.find(st.key);
+            if (precision != TTM_PREC_BF16X3 && it != g_tune_cache.end()) st.cfg = group_config(it->second);
+        }
+    }
+    ttm_status_t const st = layout_tiles(plan);
+    if (st != TTM_SUCCESS) {
+        ttm_chain_plan_destroy(plan);
+        return st;
+    }
+    *plan_out = plan;
+    clear_error();
+    return TTM_SUCCESS;
+}
+
+ttm_status_t ttm_chain_run(ttm_chain_plan_t plan, ttm_stream_t stream)
+{
+    if (!plan) return fail(TTM_ERR_ARGUMENT, "plan is null");
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    for (Stage const &st : plan->stages) {
+        ttm_status_t const r = launch_stage(plan, st, s);
+        if (r != TTM_SUCCESS) return r;
+    }
+    clear_error();
+    return TTM_SUCCESS;
+}
+
+ttm_status_t ttm_chain_run_graph(ttm_chain_plan_t plan, ttm_stream_t stream)
+{
+    if (!plan) return fail(TTM_ERR_ARGUMENT, "plan is null");
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    if (!plan->graph) {
+        // capture on a private stream so the caller's stream may be the null stream
+        hipStream_t cap = nullptr;
+        ttm_status_t r = hip_check(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        if (r) return r;
+        hipGraph_t g = nullptr;
+        r = hip_check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        if (!r) {
+            for (Stage const &st : plan->stages) {
+                r = launch_stage(plan, st, cap);
+                if (r) break;
+            }
+            ttm_status_t const e = hip_check(hipStreamEndCapture(cap, &g));
+            if (!r) r = e;
+        }
+        if (!r) r = hip_check(hipGraphInstantiate(&plan->graph, g, nullptr, nullptr, 0));
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipStreamDestroy(cap);
+        if (r) {
+            plan->graph = nullptr;
+            return r;
+        }
+    }
+    ttm_status_t const r = hip_check(hipGraphLaunch(plan->graph, s));
+    if (!r) clear_error();
+    return r;
+}
+
+ttm_status_t ttm_chain_tune(ttm_chain_plan_t plan, ttm_stream_t stream)
+{
+    if (!plan) return fail(TTM_ERR_ARGUMENT, "plan is null");
+    if (plan->prec == TTM_PREC_BF16X3) return TTM_SUCCESS; // one compiled shape
+    drop_graph(plan);
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    hipEvent_t e0, e1;
+    ttm_status_t r = hip_check(hipEventCreate(&e0));
+    if (r) return r;
+    r = hip_check(hipEventCreate(&e1));
+    if (r) {
+        (void)hipEventDestroy(e0);
+        return r;
+    }
+    for (size_t si = 0; si < plan->stages.size() && !r; ++si) {
+        float best_ms = 1e30f;
+        int best = -1;
+        for (int ci = 0; ci < group_config_count() && !r; ++ci) {
+            GroupConfig const cfg = group_config(ci);
+            if (!config_allowed(cfg, plan->is_double)) continue;
+            plan->stages[si].cfg = cfg;
+            r = layout_tiles(plan);
+            if (r) break;
+            Stage const &st = plan->stages[si];
+            r = launch_stage(plan, st, s); // warm-up
+            if (r) break;
+            r = hip_check(hipEventRecord(e0, s));
+            for (int it = 0; it < 3 && !r; ++it) r = launch_stage(plan, st, s);
+            if (!r) r = hip_check(hipEventRecord(e1, s));
+            if (!r) r = hip_check(hipEventSynchronize(e1));
+            float ms = 0.f;
+            if (!r) r = hip_check(hipEventElapsedTime(&ms, e0, e1));
+            if (!r && ms < best_ms) {
+                best_ms = ms;
+                best = ci;
+            }
+        }
+        if (!r && best >= 0) {
+            plan->stages[si].cfg = group_config(best);
+            std::lock_guard<std::mutex> lock(g_tune_mutex);
+            tune_load_locked();
+            g_tune_cache[plan->stages[si].key] = best;
+            tune_store_locked();
+        }
+    }
+    if (!r) r = layout_tiles(plan);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!r) clear_error();
+    return r;
+}
+
+unsigned ttm_chain_stages(ttm_chain_plan_t plan) { return plan ? (unsigned)plan->stages.size() : 0u; }
+
+const char *ttm_chain_stage_kernel(ttm_chain_plan_t plan, unsigned s)
+{
+    if (!plan || s >= plan->stages.size()) return "";
+    plan->symbol = plan->prec == TTM_PREC_BF16X3 ? lowp_group_symbol()
+                                                 : group_symbol(plan->stages[s].cfg, plan->is_double);
+    return plan->symbol.c_str();
+}
+
+void ttm_chain_plan_destroy(ttm_chain_plan_t plan)
+{
+    if (!plan) return;
+    drop_graph(plan);
+    // launches of this plan may still read its descriptors and intermediates
+    if (plan->d_tiles || plan->d_static) (void)hipDeviceSynchronize();
+    if (plan->d_tiles) (void)hipFree(plan->d_tiles);
+    if (plan->d_static) (void)hipFree(plan->d_static);
+    delete plan;
+}
+
+} // extern "C"

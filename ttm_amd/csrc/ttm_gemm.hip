@@ -98,30 +98,48 @@ __device__ __forceinline__ void block_coords(unsigned nj, unsigned total, unsign
     nb = b / nj;
 }
 
-template <class T, int JT, int NT, int WAVES_M, int BK, bool TN, bool BCOL>
-__global__ __launch_bounds__(256) void k_gemm(const T *__restrict__ a, const T *__restrict__ b,
-                                              T *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w,
-                                              uint64_t m, unsigned nj, unsigned total)
+// Views of the column index (see the file header).  TT loads like TN (rows of
+// A contiguous along l) and stores like NN with a single slab of width ncols:
+// C[j * ncols + n], i.e. the new mode becomes the slowest one.
+enum View : int
+{
+    VIEW_NN = 0,
+    VIEW_TN = 1,
+    VIEW_TT = 2
+};
+template <class T, int JT, int NT, int WAVES_M, int BK>
+struct TileShape
+{
+    static constexpr int VEC = G<T>::VEC;
+    static constexpr int BM = WAVES_M * 16 * JT;
+    static constexpr int BN = (4 / WAVES_M) * 16 * NT;
+    static constexpr int LDA = BN + VEC; // At tile row stride (one 16-byte vector of padding)
+    static constexpr int LDB = BM + VEC; // B tile row stride
+};
+
+// One BM x BN block of D starting at (j0, n0); sA / sB are the block's two
+// LDS buffers per operand.  Begins and ends with all waves synchronised.
+template <class T, int JT, int NT, int WAVES_M, int BK, int VIEW, bool BCOL>
+__device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__restrict__ b, T *__restrict__ c,
+                                          uint64_t ncols, uint64_t nq, uint64_t w, uint64_t m, uint64_t j0,
+                                          uint64_t n0, unsigned flags,
+                                          T (*sA)[BK * TileShape<T, JT, NT, WAVES_M, BK>::LDA],
+                                          T (*sB)[BK * TileShape<T, JT, NT, WAVES_M, BK>::LDB])
 {
     using Gt = G<T>;
     using V = typename Gt::vec;
     using Acc = typename Gt::acc;
+    using S = TileShape<T, JT, NT, WAVES_M, BK>;
     constexpr int VEC = Gt::VEC;
-    constexpr int WAVES_N = 4 / WAVES_M;
-    constexpr int BM = WAVES_M * 16 * JT;
-    constexpr int BN = WAVES_N * 16 * NT;
-    constexpr int PAD = VEC;            // row padding of the LDS tiles (one 16-byte vector)
-    constexpr int LDA = BN + PAD;       // At tile row stride
-    constexpr int LDB = BM + PAD;       // B tile row stride
+    constexpr int BM = S::BM;
+    constexpr int BN = S::BN;
+    constexpr int LDA = S::LDA;
+    constexpr int LDB = S::LDB;
+    constexpr bool TN = VIEW != VIEW_NN;     // rows of A contiguous along l
+    constexpr bool TSTORE = VIEW == VIEW_TN; // C[n][j]
     static_assert(BK % 4 == 0 && BK % VEC == 0, "BK");
-
-    __shared__ __attribute__((aligned(16))) T sA[2][BK * LDA];
-    __shared__ __attribute__((aligned(16))) T sB[2][BK * LDB];
-
-    unsigned jb, nb;
-    block_coords(nj, total, jb, nb);
-    uint64_t const j0 = (uint64_t)jb * BM;
-    uint64_t const n0 = (uint64_t)nb * BN;
+    bool const a_vec = (flags & TILE_A_VEC) != 0;
+    bool const c_vec = (flags & TILE_C_VEC) != 0;
 
     int const tid = threadIdx.x;
     int const lane = tid & 63;
@@ -191,8 +209,16 @@ __global__ __launch_bounds__(256) void k_gemm(const T *__restrict__ a, const T *
                 if (a_ok[e] && l < nq) ra[e] = *reinterpret_cast<const V *>(a + a_base[e] + l * w);
             } else {
                 uint64_t const l = l0 + a_row[e];
-                // nq % VEC == 0 for TN, so a vector is wholly in or out of range
-                if (a_ok[e] && l < nq) ra[e] = *reinterpret_cast<const V *>(a + a_base[e] + l);
+                if (a_ok[e] && l < nq) {
+                    if (a_vec) {
+                        // nq % VEC == 0: a vector is wholly in or out of range
+                        ra[e] = *reinterpret_cast<const V *>(a + a_base[e] + l);
+                    } else {
+#pragma unroll
+                        for (int x = 0; x < VEC; ++x)
+                            if (l + x < nq) ra[e][x] = a[a_base[e] + l + x];
+                    }
+                }
             }
         }
 #pragma unroll
@@ -284,7 +310,7 @@ __global__ __launch_bounds__(256) void k_gemm(const T *__restrict__ a, const T *
 
     // ---- epilogue ---------------------------------------------------------
     uint64_t const ncol0 = n0 + a_off; // first of this lane's NT adjacent columns
-    if constexpr (!TN) {
+    if constexpr (!TSTORE) {
         // one row j, NT adjacent columns; each VEC-group of them lies in one
         // slab (w % VEC == 0), but the NT columns may span two slabs.
         constexpr int NG = NT / VEC;
@@ -307,10 +333,17 @@ __global__ __launch_bounds__(256) void k_gemm(const T *__restrict__ a, const T *
 #pragma unroll
                 for (int g = 0; g < NG; ++g) {
                     if (!gok[g]) continue;
-                    V o;
+                    if (c_vec) {
+                        V o;
 #pragma unroll
-                    for (int x = 0; x < VEC; ++x) o[x] = acc[jt][g * VEC + x][r];
-                    *reinterpret_cast<V *>(cgroup[g] + j * w) = o;
+                        for (int x = 0; x < VEC; ++x) o[x] = acc[jt][g * VEC + x][r];
+                        *reinterpret_cast<V *>(cgroup[g] + j * w) = o;
+                    } else {
+                        // TT with ncols % VEC != 0: the group may run past the end
+#pragma unroll
+                        for (int x = 0; x < VEC; ++x)
+                            if (ncol0 + g * VEC + x < ncols) cgroup[g][j * w + x] = acc[jt][g * VEC + x][r];
+                    }
                 }
             }
     } else {
@@ -340,6 +373,42 @@ __global__ __launch_bounds__(256) void k_gemm(const T *__restrict__ a, const T *
                     if (jr + jt < m) crow[jr + jt] = acc[jt][nt][r];
             }
         }
+    }
+}
+
+// One product: a grid of BM x BN blocks (row blocks fastest, XCD-grouped).
+template <class T, int JT, int NT, int WAVES_M, int BK, bool TN, bool BCOL>
+__global__ __launch_bounds__(256) void k_gemm(const T *__restrict__ a, const T *__restrict__ b,
+                                              T *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w,
+                                              uint64_t m, unsigned nj, unsigned total)
+{
+    using S = TileShape<T, JT, NT, WAVES_M, BK>;
+    __shared__ __attribute__((aligned(16))) T sA[2][BK * S::LDA];
+    __shared__ __attribute__((aligned(16))) T sB[2][BK * S::LDB];
+    unsigned jb, nb;
+    block_coords(nj, total, jb, nb);
+    gemm_tile<T, JT, NT, WAVES_M, BK, TN ? VIEW_TN : VIEW_NN, BCOL>(
+        a, b, c, ncols, nq, w, m, (uint64_t)jb * S::BM, (uint64_t)nb * S::BN, TILE_A_VEC | TILE_C_VEC, sA, sB);
+}
+
+// Many products in one persistent launch: workgroup g processes tiles
+// g, g + gridDim.x, ... of the concatenated tile lists of all problems.
+// Every problem is a TT product with a row-major B (a stage of a multi-TTM
+// chain); tile_prob[t] names the problem of tile t.
+template <class T, int JT, int NT, int WAVES_M, int BK>
+__global__ __launch_bounds__(256) void k_gemm_grouped(const GroupProblem *__restrict__ probs,
+                                                      const unsigned *__restrict__ tile_prob, unsigned total)
+{
+    using S = TileShape<T, JT, NT, WAVES_M, BK>;
+    __shared__ __attribute__((aligned(16))) T sA[2][BK * S::LDA];
+    __shared__ __attribute__((aligned(16))) T sB[2][BK * S::LDB];
+    for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
+        GroupProblem const &P = probs[tile_prob[t]];
+        unsigned const local = t - P.tile_begin;
+        unsigned const jb = local % P.nj, nb = local / P.nj;
+        gemm_tile<T, JT, NT, WAVES_M, BK, VIEW_TT, false>(
+            static_cast<const T *>(P.a), static_cast<const T *>(P.b), static_cast<T *>(P.c), P.ncols, P.nq,
+            P.ncols, P.m, (uint64_t)jb * S::BM, (uint64_t)nb * S::BN, P.flags, sA, sB);
     }
 }
 
@@ -432,5 +501,101 @@ ttm_status_t launch_gemm(const Fold &f, const T *a, const T *b, T *c, hipStream_
 
 template ttm_status_t launch_gemm<float>(const Fold &, const float *, const float *, float *, hipStream_t);
 template ttm_status_t launch_gemm<double>(const Fold &, const double *, const double *, double *, hipStream_t);
+
+// ---- grouped launches -------------------------------------------------------
+namespace
+{
+// (jt, nt, waves_m, bk): rows 16/32/64 with 256 columns, 128 x 128, and 64 or 32
+// rows with 128 columns for short problems.
+constexpr GroupConfig k_group_configs[] = {
+    {1, 4, 1, 16}, {2, 4, 1, 16}, {4, 4, 1, 16}, {4, 4, 2, 16}, {2, 4, 2, 16}, {1, 4, 2, 16},
+};
+constexpr int k_num_group_configs = sizeof(k_group_configs) / sizeof(k_group_configs[0]);
+
+template <class T, int JT, int NT, int WM, int BK>
+const void *grouped_fn()
+{
+    return reinterpret_cast<const void *>(&k_gemm_grouped<T, JT, NT, WM, BK>);
+}
+
+template <class T>
+const void *grouped_kernel(int index)
+{
+    switch (index) {
+    case 0: return grouped_fn<T, 1, 4, 1, 16>();
+    case 1: return grouped_fn<T, 2, 4, 1, 16>();
+    case 2: return grouped_fn<T, 4, 4, 1, 16>();
+    case 3: return grouped_fn<T, 4, 4, 2, 16>();
+    case 4: return grouped_fn<T, 2, 4, 2, 16>();
+    case 5: return grouped_fn<T, 1, 4, 2, 16>();
+    default: return nullptr;
+    }
+}
+
+int config_index(const GroupConfig &cfg)
+{
+    for (int i = 0; i < k_num_group_configs; ++i) {
+        GroupConfig const &g = k_group_configs[i];
+        if (g.jt == cfg.jt && g.nt == cfg.nt && g.waves_m == cfg.waves_m && g.bk == cfg.bk) return i;
+    }
+    return -1;
+}
+} // namespace
+
+int group_config_count() { return k_num_group_configs; }
+
+GroupConfig group_config(int index) { return k_group_configs[index]; }
+
+std::string group_symbol(const GroupConfig &cfg, bool is_double)
+{
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "k_gemm_grouped<%s, %d, %d, %d, %d>", is_double ? "double" : "float", cfg.jt,
+                  cfg.nt, cfg.waves_m, cfg.bk);
+    return buf;
+}
+
+int group_occupancy(const GroupConfig &cfg, bool is_double)
+{
+    int const idx = config_index(cfg);
+    if (idx < 0) return 0;
+    const void *fn = is_double ? grouped_kernel<double>(idx) : grouped_kernel<float>(idx);
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, 256, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return blocks;
+}
+
+template <class T>
+ttm_status_t launch_gemm_grouped(const GroupConfig &cfg, const GroupProblem *d_probs, const unsigned *d_tile_prob,
+                                 unsigned total, unsigned grid, hipStream_t s)
+{
+    int const idx = config_index(cfg);
+    if (idx < 0 || grid == 0) {
+        set_error(TTM_ERR_ARGUMENT, "unknown grouped tile configuration");
+        return TTM_ERR_ARGUMENT;
+    }
+    switch (idx) {
+    case 0: hipLaunchKernelGGL((k_gemm_grouped<T, 1, 4, 1, 16>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total); break;
+    case 1: hipLaunchKernelGGL((k_gemm_grouped<T, 2, 4, 1, 16>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total); break;
+    case 2: hipLaunchKernelGGL((k_gemm_grouped<T, 4, 4, 1, 16>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total); break;
+    case 3: hipLaunchKernelGGL((k_gemm_grouped<T, 4, 4, 2, 16>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total); break;
+    case 4: hipLaunchKernelGGL((k_gemm_grouped<T, 2, 4, 2, 16>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total); break;
+    case 5: hipLaunchKernelGGL((k_gemm_grouped<T, 1, 4, 2, 16>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total); break;
+    default: break;
+    }
+    hipError_t const e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(TTM_ERR_HIP, hipGetErrorString(e));
+        return TTM_ERR_HIP;
+    }
+    return TTM_SUCCESS;
+}
+
+template ttm_status_t launch_gemm_grouped<float>(const GroupConfig &, const GroupProblem *, const unsigned *,
+                                                 unsigned, unsigned, hipStream_t);
+template ttm_status_t launch_gemm_grouped<double>(const GroupConfig &, const GroupProblem *, const unsigned *,
+                                                  unsigned, unsigned, hipStream_t);
 
 } // namespace ttm_amd

@@ -111,25 +111,36 @@ __device__ __forceinline__ void lp_block_coords(unsigned nj, unsigned total, uns
 // ---------------------------------------------------------------------------
 // BF16 / FP16 / BF16X3: 128 x 128 blocks, K-slices of 32, 4 waves of 64 x 64.
 // ---------------------------------------------------------------------------
-template <int MODE, bool TN, bool BCOL>
-__global__ __launch_bounds__(256) void k_lowp(const float *__restrict__ a, const float *__restrict__ b,
-                                              float *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w,
-                                              uint64_t m, unsigned nj, unsigned total)
+// Views (as in ttm_gemm.hip): NN, TN (C[n][j]) and TT (TN loads, C[j * ncols + n]).
+enum LpView : int
+{
+    LPV_NN = 0,
+    LPV_TN = 1,
+    LPV_TT = 2
+};
+constexpr int LP_BM = 128, LP_BN = 128, LP_BK = 32;
+constexpr int LP_LDK = LP_BK + 8; // 80-byte rows: the 16 rows of a fragment read hit distinct banks
+
+// One 128 x 128 block of D at (j0, n0).  flags: TILE_A_VEC / TILE_C_VEC as in
+// ttm_gemm.hpp (16-byte loads of A rows, 16-byte stores of C rows).
+template <int MODE, int VIEW, bool BCOL>
+__device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const float *__restrict__ b,
+                                          float *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w, uint64_t m,
+                                          uint64_t j0, uint64_t n0, unsigned flags,
+                                          typename Lp<MODE>::op (*sA)[LP_BN * LP_LDK],
+                                          typename Lp<MODE>::op (*sB)[LP_BM * LP_LDK])
 {
     using L = Lp<MODE>;
     using op = typename L::op;
     using v4 = typename L::v4;
     using v8 = typename L::v8;
     constexpr int P = L::PARTS;
-    constexpr int BM = 128, BN = 128, BK = 32;
-    constexpr int LDK = BK + 8; // 80-byte rows: the 16 rows of a fragment read hit distinct banks
-    __shared__ __attribute__((aligned(16))) op sA[P][BN * LDK];
-    __shared__ __attribute__((aligned(16))) op sB[P][BM * LDK];
-
-    unsigned jb, nb;
-    lp_block_coords(nj, total, jb, nb);
-    uint64_t const j0 = (uint64_t)jb * BM;
-    uint64_t const n0 = (uint64_t)nb * BN;
+    constexpr int BM = LP_BM, BN = LP_BN, BK = LP_BK;
+    constexpr int LDK = LP_LDK;
+    constexpr bool TN = VIEW != LPV_NN;
+    constexpr bool TSTORE = VIEW == LPV_TN;
+    bool const a_vec = (flags & TILE_A_VEC) != 0;
+    bool const c_vec = (flags & TILE_C_VEC) != 0;
     int const tid = threadIdx.x;
     int const lane = tid & 63, wave = tid >> 6;
     int const lc = lane & 15, lg = lane >> 4;
@@ -178,7 +189,15 @@ __global__ __launch_bounds__(256) void k_lowp(const float *__restrict__ a, const
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 uint64_t const l = l0 + k_k16 + 4 * e;
-                if (a_ok && l < nq) ra[e] = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(a + a_col_base + l));
+                if (a_ok && l < nq) {
+                    if (a_vec) {
+                        ra[e] = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(a + a_col_base + l));
+                    } else {
+#pragma unroll
+                        for (int x = 0; x < 4; ++x)
+                            if (l + x < nq) ra[e][x] = a[a_col_base + l + x];
+                    }
+                }
             }
         }
         if constexpr (BCOL) {
@@ -293,7 +312,7 @@ __global__ __launch_bounds__(256) void k_lowp(const float *__restrict__ a, const
 
     // ---- epilogue ------------------------------------------------------------
     uint64_t const ncol0 = n0 + wn * 64 + 4 * lc; // this lane's 4 adjacent columns
-    if constexpr (!TN) {
+    if constexpr (!TSTORE) {
         if (ncol0 >= ncols) return;
         uint64_t const k = ncol0 / w;
         float *cb = c + k * m * w + (ncol0 - k * w);
@@ -303,8 +322,15 @@ __global__ __launch_bounds__(256) void k_lowp(const float *__restrict__ a, const
             for (int r = 0; r < 4; ++r) {
                 uint64_t const j = j0 + wm * 64 + 4 * (4 * lg + r) + jt;
                 if (j >= m) continue;
-                f32x4 o = {acc[jt][0][r], acc[jt][1][r], acc[jt][2][r], acc[jt][3][r]};
-                *reinterpret_cast<f32x4 *>(cb + j * w) = o;
+                if (c_vec) {
+                    f32x4 o = {acc[jt][0][r], acc[jt][1][r], acc[jt][2][r], acc[jt][3][r]};
+                    *reinterpret_cast<f32x4 *>(cb + j * w) = o;
+                } else {
+                    // TT with ncols % 4 != 0: the 4 columns may run past the end
+#pragma unroll
+                    for (int nt = 0; nt < 4; ++nt)
+                        if (ncol0 + nt < ncols) cb[j * w + nt] = acc[jt][nt][r];
+                }
             }
     } else {
 #pragma unroll
@@ -325,6 +351,39 @@ __global__ __launch_bounds__(256) void k_lowp(const float *__restrict__ a, const
                 }
             }
         }
+    }
+}
+
+// One product: a grid of 128 x 128 blocks.
+template <int MODE, bool TN, bool BCOL>
+__global__ __launch_bounds__(256) void k_lowp(const float *__restrict__ a, const float *__restrict__ b,
+                                              float *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w,
+                                              uint64_t m, unsigned nj, unsigned total)
+{
+    using op = typename Lp<MODE>::op;
+    __shared__ __attribute__((aligned(16))) op sA[Lp<MODE>::PARTS][LP_BN * LP_LDK];
+    __shared__ __attribute__((aligned(16))) op sB[Lp<MODE>::PARTS][LP_BM * LP_LDK];
+    unsigned jb, nb;
+    lp_block_coords(nj, total, jb, nb);
+    lowp_tile<MODE, TN ? LPV_TN : LPV_NN, BCOL>(a, b, c, ncols, nq, w, m, (uint64_t)jb * LP_BM,
+                                                (uint64_t)nb * LP_BN, TILE_A_VEC | TILE_C_VEC, sA, sB);
+}
+
+// Many TT products (stages of multi-TTM chains) in one persistent launch.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_lowp_grouped(const GroupProblem *__restrict__ probs,
+                                                      const unsigned *__restrict__ tile_prob, unsigned total)
+{
+    using op = typename Lp<MODE>::op;
+    __shared__ __attribute__((aligned(16))) op sA[Lp<MODE>::PARTS][LP_BN * LP_LDK];
+    __shared__ __attribute__((aligned(16))) op sB[Lp<MODE>::PARTS][LP_BM * LP_LDK];
+    for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
+        GroupProblem const &P = probs[tile_prob[t]];
+        unsigned const local = t - P.tile_begin;
+        unsigned const jb = local % P.nj, nb = local / P.nj;
+        lowp_tile<MODE, LPV_TT, false>(static_cast<const float *>(P.a), static_cast<const float *>(P.b),
+                                       static_cast<float *>(P.c), P.ncols, P.nq, P.ncols, P.m,
+                                       (uint64_t)jb * LP_BM, (uint64_t)nb * LP_BN, P.flags, sA, sB);
     }
 }
 
@@ -622,6 +681,36 @@ ttm_status_t launch_lowp(const Fold &f, const float *a, const float *b, float *c
     } else {
         launch_lp_mode<LP_BF16X3>(f, a, b, c, s);
     }
+    hipError_t const e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(TTM_ERR_HIP, hipGetErrorString(e));
+        return TTM_ERR_HIP;
+    }
+    return TTM_SUCCESS;
+}
+
+} // namespace ttm_amd
+
+namespace ttm_amd
+{
+
+int lowp_group_occupancy()
+{
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &blocks, reinterpret_cast<const void *>(&k_lowp_grouped<LP_BF16X3>), 256, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return blocks;
+}
+
+std::string lowp_group_symbol() { return "k_lowp_grouped<2>"; }
+
+ttm_status_t launch_lowp_grouped(const GroupProblem *d_probs, const unsigned *d_tile_prob, unsigned total,
+                                 unsigned grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_lowp_grouped<LP_BF16X3>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
     hipError_t const e = hipGetLastError();
     if (e != hipSuccess) {
         set_error(TTM_ERR_HIP, hipGetErrorString(e));

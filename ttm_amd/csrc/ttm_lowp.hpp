@@ -6,6 +6,7 @@
 #include <string>
 
 #include "ttm_amd.h"
+#include "ttm_gemm.hpp"
 #include "ttm_plan.hpp"
 
 namespace ttm_amd
@@ -21,5 +22,12 @@ std::string lowp_symbol(const Fold &f, ttm_precision_t prec, bool generic);
 // `generic` selects the one-thread-per-element kernel (any shape/alignment).
 ttm_status_t launch_lowp(const Fold &f, const float *a, const float *b, float *c, ttm_precision_t prec,
                          hipStream_t s, bool generic);
+
+// Grouped BF16X3 launches of TT products (stages of multi-TTM chains) in
+// 128 x 128 tiles; tile lists as for launch_gemm_grouped.
+int lowp_group_occupancy();
+std::string lowp_group_symbol();
+ttm_status_t launch_lowp_grouped(const GroupProblem *d_probs, const unsigned *d_tile_prob, unsigned total,
+                                 unsigned grid, hipStream_t s);
 
 } // namespace ttm_amd
