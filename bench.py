@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--force", default="", help="force a kernel family (comparison runs only)")
     ap.add_argument("--no-bf16x3", dest="bf16x3", action="store_false",
                     help="skip the BF16X3-vs-FP32 comparison on the compute-bound float32 products")
+    ap.add_argument("--tucker-count", type=int, default=256,
+                    help="tensors of the batched multi-TTM (config 5) section (0 = skip)")
     return ap.parse_args()
 
 
@@ -194,6 +196,66 @@ def bf16x3_vs_fp32(args, cases, A, C, stream):
             "error_reference": "float64 definition on 4096 sampled entries", "cases": rows}
 
 
+def tucker_batch(args):
+    """Config 5: 256 first-order float32 tensors with extents drawn from
+    U{16..256}, each multiplied in all three modes by factors with orthonormal
+    rows and ranks drawn from U{4..64} (QR of a Gaussian matrix); the whole
+    batch as one plan, replayed as one hipGraph (FP32 and BF16X3)."""
+    import torch
+    from ttm_amd.chain import ChainPlan
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(0x77140005)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0x77140005)
+    items = []
+    flops = 0.0
+    compulsory = 0.0
+    for _ in range(args.tucker_count):
+        na = [int(x) for x in rng.integers(16, 257, size=3)]
+        # a factor with orthonormal rows needs r <= n: ranks drawn above the
+        # extent are clamped to it
+        rk = [min(int(x), n) for x, n in zip(rng.integers(4, 65, size=3), na)]
+        # A dense in first-order layout: natural shape, strides (1, n1, n1*n2)
+        A = torch.empty(na[::-1], device=dev, dtype=torch.float32).uniform_(-1, 1, generator=gen).permute(2, 1, 0)
+        Us = []
+        for r, n in zip(rk, na):
+            Q, _ = np.linalg.qr(rng.standard_normal((n, r)))
+            Us.append(torch.from_numpy(np.ascontiguousarray(Q[:, :r].T)).to(dev, torch.float32))
+        C = torch.empty(rk[::-1], device=dev, dtype=torch.float32).permute(2, 1, 0)
+        items.append((A, Us, C))
+        cur = list(na)
+        for k in range(3):  # stages in layout order 1, 2, 3
+            flops += 2.0 * rk[k] * float(np.prod(cur))
+            cur[k] = rk[k]
+        compulsory += 4.0 * (np.prod(na) + sum(r * n for r, n in zip(rk, na)) + np.prod(rk))
+    out = {"workload": f"{args.tucker_count} tensors, extents U{{16..256}}^3, ranks U{{4..64}}, "
+                       "A x1 U1 x2 U2 x3 U3, orthonormal rows, one hipGraph replay per step",
+           "gflop": round(flops / 1e9, 2), "compulsory_gb": round(compulsory / 1e9, 3)}
+    stream = torch.cuda.current_stream()
+    for prec in ("fp32", "bf16x3"):
+        plan = ChainPlan(items, precision=None if prec == "fp32" else "bf16x3")
+        if prec == "fp32":
+            plan.tune()
+        plan.run_graph()
+        plan.run_graph()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            plan.run_graph()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        out[prec] = {"ms": round(ms, 3), "tflops": round(flops / ms / 1e9, 3),
+                     "compulsory_gbps": round(compulsory / ms / 1e6, 1),
+                     "hbm_frac_of_compulsory": round(compulsory / ms / 1e6 / (PEAK_HBM / 1e9), 4),
+                     "stage_kernels": plan.stage_kernels}
+        plan.close()
+    del items
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(args, budget_s):
     """The oracle's `combined` restatement over OpenMP + a CBLAS, on the
     same 24-case sweep with every extent divided by 4 (A 256 x 256 x 128)."""
@@ -295,6 +357,8 @@ def main():
                 "launches_per_step": len(members), "avg_launch_ms": round(t_sym * 1e3 / len(members), 4),
                 "cases": [c["label"] for c in members]}
 
+    tucker = tucker_batch(args) if args.tucker_count > 0 else None
+
     cpu = None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(args, args.cpu_seconds)
@@ -321,6 +385,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "bf16x3_vs_fp32": split,
+        "tucker_batch": tucker,
         "cases": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in c.items()
                    if k in ("label", "symbol", "ms", "ms_min", "tflops", "gbps", "bound", "frac")}
                   for c in cases],

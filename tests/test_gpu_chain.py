@@ -58,7 +58,7 @@ def random_items(torch, dev, rng, count, dtype, max_ext=12, max_rank=10, orders=
         At = to_layout(torch, dev, A, pi, dtype)
         Ut = [torch.from_numpy(U).to(dev, dtype).contiguous() if U is not None else None for U in Us]
         Ct = to_layout(torch, dev, np.zeros(nc), pi, dtype)
-        items.append((At, Ut, Ct))
+        items.append((At, Ut, Ct, pi))
         refs.append((A, Us, einsum_chain(A, Us)))
     return items, refs
 
@@ -67,8 +67,8 @@ def check(items, refs, dtype, label, prec=None):
     import torch
     torch.cuda.synchronize()
     eps = np.finfo(np.float32).eps
-    for (At, Ut, Ct), (A, Us, ref) in zip(items, refs):
-        got = Ct.double().cpu().numpy()
+    for item, (A, Us, ref) in zip(items, refs):
+        got = item[2].double().cpu().numpy()
         err = np.linalg.norm(got - ref)
         if dtype == torch.float64:
             assert err <= 1e-12 * max(np.linalg.norm(ref), 1.0), (label, A.shape)
@@ -103,16 +103,16 @@ def test_graph_replay_matches_eager(gpu):
     plan = ChainPlan(items)
     plan.run()
     torch.cuda.synchronize()
-    eager = [C.clone() for _, _, C in items]
+    eager = [it[2].clone() for it in items]
     for rep in range(4):
-        for _, _, C in items:
-            C.fill_(float("nan"))
+        for it in items:
+            it[2].fill_(float("nan"))
         torch.cuda.synchronize()
         plan.run_graph()
         torch.cuda.synchronize()
         junk = torch.randn(1000, device=gpu).sum()  # eager work between replays
-        for (_, _, C), E in zip(items, eager):
-            assert torch.equal(C, E), f"replay {rep}"
+        for it, E in zip(items, eager):
+            assert torch.equal(it[2], E), f"replay {rep}"
         del junk
     check(items, refs, torch.float32, "graph")
     plan.close()
@@ -127,8 +127,8 @@ def test_tune_keeps_results(gpu, tmp_path, monkeypatch):
     plan = ChainPlan(items)
     plan.tune()
     _say("tuned stage kernels", plan.stage_kernels)
-    for _, _, C in items:
-        C.zero_()
+    for it in items:
+        it[2].zero_()
     plan.run()
     check(items, refs, torch.float32, "tuned")
     plan.close()

@@ -54,8 +54,16 @@ def _bind():
     return L
 
 
+def _dense_in(shape, strides, layout):
+    """Dense in `layout`, ignoring the (free) strides of extent-1 modes."""
+    want = dense_strides(shape, layout)
+    return all(n == 1 or s == w for n, s, w in zip(shape, strides, want))
+
+
 def make_desc(A, factors, C, layout=None):
-    """Descriptor for one tensor (torch tensors on the device)."""
+    """Descriptor for one tensor (torch tensors on the device).  `layout`
+    (one-based, fastest first) defaults to the one A's strides describe; pass
+    it when A has extent-1 modes whose place in C's layout matters."""
     import torch
     p = A.dim()
     if not 1 <= p <= MAX_ORDER:
@@ -66,7 +74,7 @@ def make_desc(A, factors, C, layout=None):
         raise TypeError("A and C must both be float32 or both float64")
     na = list(A.shape)
     pia = list(layout) if layout is not None else layout_from_strides(na, list(A.stride()))
-    if list(A.stride()) != dense_strides(na, pia):
+    if not _dense_in(na, list(A.stride()), pia):
         raise ValueError("A must be dense in its layout")
     d = ChainDesc()
     d.p = p
@@ -83,7 +91,7 @@ def make_desc(A, factors, C, layout=None):
             d.u[k] = None
             d.r[k] = na[k]
     nc = [int(d.r[k]) for k in range(p)]
-    if list(C.shape) != nc or list(C.stride()) != dense_strides(nc, pia):
+    if list(C.shape) != nc or not _dense_in(nc, list(C.stride()), pia):
         raise ValueError("C must have the chain's output extents, dense in A's layout")
     d.a = A.data_ptr()
     d.c = C.data_ptr()
@@ -91,7 +99,8 @@ def make_desc(A, factors, C, layout=None):
 
 
 class ChainPlan:
-    """A plan over many (A, factors, C) items; keeps references to the tensors."""
+    """A plan over many (A, factors, C) or (A, factors, C, layout) items;
+    keeps references to the tensors."""
 
     def __init__(self, items, precision=None):
         import torch

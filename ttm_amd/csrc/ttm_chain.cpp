@@ -145,6 +145,7 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
             if (begin + tiles >= (1ull << 31)) return fail(TTM_ERR_ARGUMENT, "too many tiles in one stage");
             P.tile_begin = begin;
             P.nj = nj;
+            P.flags = (P.flags & ~TILE_STREAM) | (nj == 1 ? TILE_STREAM : 0u); // A read once
             st.tile_prob.insert(st.tile_prob.end(), (size_t)tiles, (unsigned)i);
             begin += (unsigned)tiles;
         }

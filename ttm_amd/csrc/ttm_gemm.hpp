@@ -27,7 +27,8 @@ ttm_status_t launch_gemm(const Fold &f, const T *a, const T *b, T *c, hipStream_
 enum TileFlags : unsigned
 {
     TILE_A_VEC = 1u, // rows of A may be read with 16-byte loads (nq % vec == 0, aligned)
-    TILE_C_VEC = 2u  // rows of C may be written with 16-byte stores (ncols % vec == 0, aligned)
+    TILE_C_VEC = 2u, // rows of C may be written with 16-byte stores (ncols % vec == 0, aligned)
+    TILE_STREAM = 4u // A is read exactly once (one row block): non-temporal loads of A and stores of C
 };
 
 // One product of a grouped launch, in the TT view:
