@@ -37,6 +37,20 @@ LAYOUTS = {"first": [1, 2, 3], "last": [3, 2, 1]}
 MS = [16, 1024]
 METRIC = "TTM TFLOP/s + HBM GB/s vs roofline; BF16x3 vs FP32 at equal FP64-oracle error"
 
+# HBM traffic per launch from two rocprofv3 --pmc passes (FETCH_SIZE, then
+# WRITE_SIZE; KiB per dispatch, mean over the dispatches of the case), keyed by
+# kernel symbol and case.  Both counters are scaled by what they read for a
+# plain 1 GiB device copy (`bench.py --copy-calibration`): FETCH_SIZE reads
+# 524,300 KiB for 1,048,576 KiB fetched (scale 2.0), WRITE_SIZE 1,048,576 KiB
+# for 1,048,576 KiB written (scale 1.0).  Summaries and commands:
+# profiles/ttm_hbm_traffic_pmc.md.  A symbol or case without an entry
+# reports traffic null.
+PMC_FETCH_SCALE = 2.0
+PMC_WRITE_SCALE = 1.0
+PMC_TRAFFIC_KIB = {
+    "k_gemm<double, 4, 4, 2, 16, false, false>": {"f64-first-q2-m1024": (6791506.4, 4197019.4)},
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -52,7 +66,25 @@ def parse():
                     help="skip the BF16X3-vs-FP32 comparison on the compute-bound float32 products")
     ap.add_argument("--tucker-count", type=int, default=256,
                     help="tensors of the batched multi-TTM (config 5) section (0 = skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (scale for FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """Three launches of a plain device-to-device copy of 1 GiB (1 GiB read,
+    1 GiB written per launch), the known-size kernel against which the
+    FETCH_SIZE / WRITE_SIZE counter readings are scaled to HBM bytes."""
+    import torch
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n,
+                      "launches": 3}), flush=True)
 
 
 def case_list(dtypes):
@@ -325,6 +357,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.force:
         import ttm_amd
         ttm_amd.force_kernel(args.force)
@@ -352,10 +387,20 @@ def main():
     else:
         achieved = sum(c["bytes"] for c in members) / t_sym / 1e9
         peak, unit = PEAK_HBM / 1e9, "GB/s"
+    measured = [PMC_TRAFFIC_KIB.get(sym, {}).get(c["label"]) for c in members]
+    measured = [(c, t) for c, t in zip(members, measured) if t is not None]
+    traffic = traffic_alg = None
+    if measured:
+        traffic = float(np.mean([1024.0 * (PMC_FETCH_SCALE * f + PMC_WRITE_SCALE * wr) for _, (f, wr) in measured]))
+        traffic_alg = float(np.mean([c["bytes"] for c, _ in measured]))
     roofline = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-                "frac": round(achieved / peak, 4), "traffic": None, "kernel": sym,
-                "launches_per_step": len(members), "avg_launch_ms": round(t_sym * 1e3 / len(members), 4),
+                "frac": round(achieved / peak, 4), "traffic": None if traffic is None else round(traffic),
+                "kernel": sym, "launches_per_step": len(members), "avg_launch_ms": round(t_sym * 1e3 / len(members), 4),
                 "cases": [c["label"] for c in members]}
+    if traffic is not None:
+        roofline["traffic_unit"] = "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled)"
+        roofline["traffic_cases"] = [c["label"] for c, _ in measured]
+        roofline["traffic_over_algorithmic"] = round(traffic / traffic_alg, 3)
 
     tucker = tucker_batch(args) if args.tucker_count > 0 else None
 

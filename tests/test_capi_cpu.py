@@ -140,8 +140,13 @@ def test_plan_reference_configs(ttmlib):
                 for dbl in (False, True):
                     k = ttm_amd.plan_kernel(q, na, pia, nb, [1, 2], is_double=dbl)
                     assert k, (pia, q, m)
-                    assert k == "gemm", (pia, q, m, k)
                     sym = ttm_amd.plan_kernel_symbol(q, na, pia, nb, [1, 2], is_double=dbl)
+                    if m == 16 and qh > 1:
+                        # NN view (w % vec == 0), one row block: the streaming kernel
+                        assert k == "nn_stream", (pia, q, m, k)
+                        assert sym.startswith("k_nn_stream<" + ("double" if dbl else "float") + ", 1, 4, ")
+                        continue
+                    assert k == "gemm", (pia, q, m, k)
                     assert sym.startswith("k_gemm<" + ("double" if dbl else "float"))
                     assert sym.endswith(("true, true>" if qh == 1 else "false, true>")), sym
     # odd shapes fall back to the generic kernel

@@ -233,6 +233,45 @@ def test_config2_full_size_sampled(gpu, dtype):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_config2_stream_kernels_sampled(gpu, dtype):
+    """The streaming kernels forced on the bench extents (A 1024x1024x512, first- and
+    last-order, q = 1..3): m = 16 with a row-major B and m = 40 with a column-major B."""
+    import torch
+    import ttm_amd
+    na = [1024, 1024, 512]
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    rng = np.random.default_rng(98)
+    A = torch.empty(int(np.prod(na)), dtype=tdt, device=gpu).uniform_(-1, 1)
+    for pia in ([1, 2, 3], [3, 2, 1]):
+        wa = rc.dense_strides(na, pia)
+        for q in (1, 2, 3):
+            w = int(np.prod([na[r - 1] for r in pia[:pia.index(q)]]))
+            fam = "tn_stream" if w == 1 else "nn_stream"
+            for m, colmajor in ((16, False), (40, True)):
+                Bm = torch.empty((m, na[q - 1]), dtype=tdt, device=gpu).uniform_(-1, 1)
+                b = Bm.t().contiguous() if colmajor else Bm
+                pib = [1, 2] if colmajor else [2, 1]
+                nc = list(na)
+                nc[q - 1] = m
+                wc = rc.dense_strides(nc, pia)
+                C = torch.empty(int(np.prod(nc)), dtype=tdt, device=gpu)
+                ttm_amd.force_kernel(fam)
+                try:
+                    used = ttm_amd.plan_kernel(q, na, pia, [m, na[q - 1]], pib, is_double=dtype == np.float64)
+                    _say(f"stream {np.dtype(dtype).name} pi={pia} q={q} m={m} col={colmajor} kernel={used}")
+                    assert used == fam
+                    ttm_amd.ttm_raw(q, 3, A, na, wa, pia, b, [m, na[q - 1]], pib, C, nc, wc)
+                    torch.cuda.synchronize()
+                finally:
+                    ttm_amd.force_kernel(None)
+                _check_sampled(torch, A, na, wa, Bm, C, nc, wc, q, dtype, rng,
+                               f"stream {np.dtype(dtype).name} pi={pia} q={q} m={m}", samples=1024)
+                del C, Bm, b
+    del A
+    torch.cuda.empty_cache()
+
+
 # ----------------------------------------------------------------------------- python bindings
 def test_ttmpy_reference_vectors_numpy_and_torch(gpu):
     """ttmpy/tests/test.py: exact equality with numpy.einsum, for ttm and all three ttms orders."""

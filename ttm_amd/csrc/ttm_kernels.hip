@@ -105,14 +105,45 @@ __global__ __launch_bounds__(256) void k_generic(const T *__restrict__ a, const 
 }
 
 // ---------------------------------------------------------------------------
-// nn_stream: w % VEC == 0.  Fused column index n' = k*w + i over all slabs.
-// Each wave owns a strip of 16*VEC columns; lane (lc, lr) loads the VEC
-// consecutive columns n' = 16*VEC*strip + VEC*lc + v (one slab, since
-// w % VEC == 0) of A row l0 + lr with one 16-byte load, and feeds element v to
-// the MFMA of column sub-tile v.  The wave covers 16*JT rows j (grid.y blocks
-// the rest).  A is read once per row block; B fragments come through L1/L2.
+// Streaming kernels for small m: A goes from HBM straight into MFMA operand
+// registers (no LDS), B fragments come through L1/L2, and a software pipeline
+// keeps D groups of A loads in flight per wave.  The wave covers every row j
+// of C (16*JT >= m), so A is read exactly once: non-temporal loads and stores.
+//
+// Contraction indices are taken in groups; MFMA step s of a group pairs
+// k-slot lr of both operands with one index l(s, lr), chosen so that each
+// lane's loads are 16-byte vectors.
 // ---------------------------------------------------------------------------
-template <class T, int JT>
+
+template <class V>
+__device__ __forceinline__ V ld_nt(const V *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
+// VEC consecutive B values B(j, l .. l+VEC-1) of one row j.
+template <class T, bool BROW>
+__device__ __forceinline__ typename Traits<T>::vec load_bvec(const T *bj, uint64_t l, uint64_t bsl)
+{
+    using V = typename Traits<T>::vec;
+    if constexpr (BROW) {
+        return *reinterpret_cast<const V *>(bj + l);
+    } else {
+        V r;
+#pragma unroll
+        for (int e = 0; e < Traits<T>::VEC; ++e) r[e] = bj[(l + e) * bsl];
+        return r;
+    }
+}
+
+// nn_stream: w % VEC == 0.  Fused column index n = k*w + i over all slabs.
+// A wave owns a strip of 16*VEC columns; lane (lc, lr) owns the VEC columns
+// n = strip + VEC*lc + v (one slab, since w % VEC == 0), which feed column
+// sub-tile v.  A group is 16 indices l0 .. l0+15; step u (< 4) uses
+// l = l0 + 4*lr + u, so a lane loads A rows l0 + 4*lr + u (one 16-byte load
+// each; one wave instruction covers 4 rows x 16*VEC columns) and the 4
+// consecutive B values B(j, l0 + 4*lr .. l0 + 4*lr + 3).
+template <class T, int JT, int D, bool BROW>
 __global__ __launch_bounds__(256) void k_nn_stream(const T *__restrict__ a, const T *__restrict__ b,
                                                    T *__restrict__ c, uint64_t outer, uint64_t nq,
                                                    uint64_t w, uint64_t m, uint64_t bsj, uint64_t bsl)
@@ -121,7 +152,7 @@ __global__ __launch_bounds__(256) void k_nn_stream(const T *__restrict__ a, cons
     constexpr int VEC = Tr::VEC;
     using V = typename Tr::vec;
     using Acc = typename Tr::acc;
-    constexpr int U = 4; // MFMA k-steps per unrolled iteration
+    constexpr int BV = 4 / VEC; // B vectors per group and row
 
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
@@ -131,18 +162,16 @@ __global__ __launch_bounds__(256) void k_nn_stream(const T *__restrict__ a, cons
     uint64_t const ncols = outer * w;
     uint64_t const col = ((uint64_t)blockIdx.x * 4 + wave) * (16 * VEC) + (uint64_t)lc * VEC;
     bool const valid = col < ncols;
+    // lanes past the end read column 0 (in bounds) and store nothing
     uint64_t const k = valid ? col / w : 0;
     uint64_t const i = valid ? col - k * w : 0;
-    T const *ap = a + k * nq * w + i;
+    T const *ap = a + k * nq * w + i + (uint64_t)(4 * lr) * w;
 
-    uint64_t const j0 = (uint64_t)blockIdx.y * (16 * JT);
     T const *bp[JT];
-    bool jv[JT];
 #pragma unroll
     for (int jt = 0; jt < JT; ++jt) {
-        uint64_t const j = j0 + jt * 16 + lc;
-        jv[jt] = j < m;
-        bp[jt] = b + (jv[jt] ? j : 0) * bsj;
+        uint64_t const j = (uint64_t)jt * 16 + lc;
+        bp[jt] = b + (j < m ? j : 0) * bsj; // rows past m are computed and dropped
     }
 
     Acc acc[JT][VEC];
@@ -151,35 +180,58 @@ __global__ __launch_bounds__(256) void k_nn_stream(const T *__restrict__ a, cons
 #pragma unroll
         for (int v = 0; v < VEC; ++v) acc[jt][v] = splat_zero<Acc>();
 
-    uint64_t l0 = 0;
-    for (; l0 + 4 * U <= nq; l0 += 4 * U) {
-        V av[U];
-        T bv[U][JT];
+    V av[D][4];
+    V bv[D][JT][BV];
+    auto issue = [&](int slot, uint64_t g) {
+        T const *p = ap + g * 16 * w;
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            uint64_t const l = l0 + 4 * u + lr;
-            av[u] = valid ? *reinterpret_cast<V const *>(ap + l * w) : splat_zero<V>();
-#pragma unroll
-            for (int jt = 0; jt < JT; ++jt) bv[u][jt] = jv[jt] ? bp[jt][l * bsl] : T(0);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-#pragma unroll
-            for (int jt = 0; jt < JT; ++jt)
-#pragma unroll
-                for (int v = 0; v < VEC; ++v) acc[jt][v] = mfma4(bv[u][jt], av[u][v], acc[jt][v]);
-    }
-    for (; l0 < nq; l0 += 4) {
-        uint64_t const l = l0 + lr;
-        bool const lv = l < nq;
-        V av = (valid && lv) ? *reinterpret_cast<V const *>(ap + l * w) : splat_zero<V>();
-        T bv[JT];
-#pragma unroll
-        for (int jt = 0; jt < JT; ++jt) bv[jt] = (jv[jt] && lv) ? bp[jt][l * bsl] : T(0);
+        for (int u = 0; u < 4; ++u) av[slot][u] = ld_nt(reinterpret_cast<const V *>(p + u * w));
+        uint64_t const l = g * 16 + 4 * lr;
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
-            for (int v = 0; v < VEC; ++v) acc[jt][v] = mfma4(bv[jt], av[v], acc[jt][v]);
+            for (int h = 0; h < BV; ++h) bv[slot][jt][h] = load_bvec<T, BROW>(bp[jt], l + h * VEC, bsl);
+    };
+    auto compute = [&](int slot) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+                for (int v = 0; v < VEC; ++v)
+                    acc[jt][v] = mfma4(bv[slot][jt][u / VEC][u % VEC], av[slot][u][v], acc[jt][v]);
+    };
+
+    uint64_t const ngroups = nq / 16;
+#pragma unroll
+    for (int d = 0; d < D - 1; ++d)
+        if ((uint64_t)d < ngroups) issue(d, d);
+    for (uint64_t g0 = 0; g0 < ngroups; g0 += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            uint64_t const g = g0 + d;
+            if (g < ngroups) {
+                if (g + D - 1 < ngroups) issue((d + D - 1) % D, g + D - 1);
+                compute(d);
+            }
+        }
+    }
+    // tail: l0 = 16 * ngroups, indices past nq are zero in both operands
+    uint64_t const l0 = ngroups * 16;
+    if (l0 < nq) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            uint64_t const l = l0 + 4 * lr + u;
+            bool const lv = l < nq;
+            V const x = lv ? *reinterpret_cast<const V *>(ap + (l0 + u) * w) : splat_zero<V>();
+            T bl[JT];
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) bl[jt] = lv ? bp[jt][l * bsl] : T(0);
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) acc[jt][v] = mfma4(bl[jt], x[v], acc[jt][v]);
+        }
     }
 
     if (!valid) return;
@@ -188,25 +240,23 @@ __global__ __launch_bounds__(256) void k_nn_stream(const T *__restrict__ a, cons
     for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            uint64_t const j = j0 + jt * 16 + Tr::acc_row(lr, r);
+            uint64_t const j = (uint64_t)jt * 16 + Tr::acc_row(lr, r);
             if (j < m) {
                 V o;
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) o[v] = acc[jt][v][r];
-                *reinterpret_cast<V *>(cp + j * w) = o;
+                __builtin_nontemporal_store(o, reinterpret_cast<V *>(cp + j * w));
             }
         }
 }
 
-// ---------------------------------------------------------------------------
 // tn_stream: w == 1, nq % VEC == 0.  C[k][j] = sum_l A[k*nq + l] * B(j, l).
-// Each wave owns 16 rows k (srcB column lc <-> row k0 + lc) and 16*JT rows j.
-// Lane (lc, lr) loads the VEC contraction indices l0 + VEC*lr + v of its row
-// with one 16-byte load; MFMA step v pairs element v with the same l of B
-// (the k-index of the MFMA is lr for both operands).  With BROW (row-major
-// B) the B fragment is one 16-byte load as well.
-// ---------------------------------------------------------------------------
-template <class T, int JT, bool BROW>
+// A wave owns 16 rows k (column lc of the MFMA <-> row k0 + lc).  A group is
+// 16*VEC indices (256 bytes of each row): lane (lc, lr) loads, for v < 4,
+// the VEC indices l0 + 4*VEC*v + VEC*lr + e (e < VEC) of its row with one
+// 16-byte load, and step (v, e) uses them for k-slot lr; the matching B
+// values are one 16-byte vector per v of a row-major B.
+template <class T, int JT, int D, bool BROW>
 __global__ __launch_bounds__(256) void k_tn_stream(const T *__restrict__ a, const T *__restrict__ b,
                                                    T *__restrict__ c, uint64_t outer, uint64_t nq,
                                                    uint64_t m, uint64_t bsj, uint64_t bsl)
@@ -215,8 +265,7 @@ __global__ __launch_bounds__(256) void k_tn_stream(const T *__restrict__ a, cons
     constexpr int VEC = Tr::VEC;
     using V = typename Tr::vec;
     using Acc = typename Tr::acc;
-    constexpr int U = 2;
-    constexpr int KSTEP = 4 * VEC; // contraction indices per wave-wide load
+    constexpr int GL = 16 * VEC; // indices per group
 
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
@@ -227,61 +276,63 @@ __global__ __launch_bounds__(256) void k_tn_stream(const T *__restrict__ a, cons
     bool const valid = row < outer;
     T const *ap = a + (valid ? row : 0) * nq + (uint64_t)lr * VEC;
 
-    uint64_t const j0 = (uint64_t)blockIdx.y * (16 * JT);
     T const *bp[JT];
-    bool jv[JT];
 #pragma unroll
     for (int jt = 0; jt < JT; ++jt) {
-        uint64_t const j = j0 + jt * 16 + lc;
-        jv[jt] = j < m;
-        bp[jt] = b + (jv[jt] ? j : 0) * bsj + (uint64_t)lr * VEC * bsl;
+        uint64_t const j = (uint64_t)jt * 16 + lc;
+        bp[jt] = b + (j < m ? j : 0) * bsj;
     }
+    uint64_t const lane_l = (uint64_t)lr * VEC;
 
     Acc acc[JT];
 #pragma unroll
     for (int jt = 0; jt < JT; ++jt) acc[jt] = splat_zero<Acc>();
 
-    auto load_b = [&](int jt, uint64_t l, bool ok) -> V {
-        V r = splat_zero<V>();
-        if (ok && jv[jt]) {
-            if constexpr (BROW) {
-                r = *reinterpret_cast<V const *>(bp[jt] + l);
-            } else {
+    V av[D][4];
+    V bv[D][JT][4];
+    auto issue = [&](int slot, uint64_t g) {
 #pragma unroll
-                for (int v = 0; v < VEC; ++v) r[v] = bp[jt][(l + v) * bsl];
-            }
-        }
-        return r;
+        for (int v = 0; v < 4; ++v) av[slot][v] = ld_nt(reinterpret_cast<const V *>(ap + g * GL + 4 * VEC * v));
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                bv[slot][jt][v] = load_bvec<T, BROW>(bp[jt], g * GL + 4 * VEC * v + lane_l, bsl);
+    };
+    auto compute = [&](int slot) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e)
+#pragma unroll
+                for (int jt = 0; jt < JT; ++jt) acc[jt] = mfma4(bv[slot][jt][v][e], av[slot][v][e], acc[jt]);
     };
 
-    uint64_t l0 = 0;
-    for (; l0 + KSTEP * U <= nq; l0 += KSTEP * U) {
-        V av[U];
-        V bv[U][JT];
+    uint64_t const ngroups = nq / GL;
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            av[u] = valid ? *reinterpret_cast<V const *>(ap + l0 + KSTEP * u) : splat_zero<V>();
+    for (int d = 0; d < D - 1; ++d)
+        if ((uint64_t)d < ngroups) issue(d, d);
+    for (uint64_t g0 = 0; g0 < ngroups; g0 += D) {
 #pragma unroll
-            for (int jt = 0; jt < JT; ++jt) bv[u][jt] = load_b(jt, l0 + KSTEP * u, true);
+        for (int d = 0; d < D; ++d) {
+            uint64_t const g = g0 + d;
+            if (g < ngroups) {
+                if (g + D - 1 < ngroups) issue((d + D - 1) % D, g + D - 1);
+                compute(d);
+            }
         }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-#pragma unroll
-            for (int v = 0; v < VEC; ++v)
-#pragma unroll
-                for (int jt = 0; jt < JT; ++jt) acc[jt] = mfma4(bv[u][jt][v], av[u][v], acc[jt]);
     }
-    for (; l0 < nq; l0 += KSTEP) {
-        // nq % VEC == 0, so a vector is either wholly inside or wholly outside
-        bool const lv = l0 + (uint64_t)lr * VEC < nq;
-        V av = (valid && lv) ? *reinterpret_cast<V const *>(ap + l0) : splat_zero<V>();
-        V bv[JT];
+    // tail: nq % VEC == 0, so each lane's vector is wholly inside or outside
+    for (uint64_t l0 = ngroups * GL; l0 < nq; l0 += 4 * VEC) {
+        bool const lv = l0 + lane_l < nq;
+        V const x = lv ? *reinterpret_cast<const V *>(ap + l0) : splat_zero<V>();
+        V bl[JT];
 #pragma unroll
-        for (int jt = 0; jt < JT; ++jt) bv[jt] = load_b(jt, l0, lv);
+        for (int jt = 0; jt < JT; ++jt) bl[jt] = lv ? load_bvec<T, BROW>(bp[jt], l0 + lane_l, bsl) : splat_zero<V>();
 #pragma unroll
-        for (int v = 0; v < VEC; ++v)
+        for (int e = 0; e < VEC; ++e)
 #pragma unroll
-            for (int jt = 0; jt < JT; ++jt) acc[jt] = mfma4(bv[jt][v], av[v], acc[jt]);
+            for (int jt = 0; jt < JT; ++jt) acc[jt] = mfma4(bl[jt][e], x[e], acc[jt]);
     }
 
     if (!valid) return;
@@ -290,9 +341,9 @@ __global__ __launch_bounds__(256) void k_tn_stream(const T *__restrict__ a, cons
     for (int jt = 0; jt < JT; ++jt) {
         if constexpr (Tr::acc_row(1, 0) == 4) {
             // f32: the lane's four rows are consecutive in j
-            uint64_t const jb = j0 + jt * 16 + lr * 4;
+            uint64_t const jb = (uint64_t)jt * 16 + lr * 4;
             if ((m & 3) == 0) {
-                if (jb < m) store_rows4(cp + jb, acc[jt]);
+                if (jb < m) __builtin_nontemporal_store(acc[jt], reinterpret_cast<f32x4 *>(cp + jb));
             } else {
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
@@ -301,18 +352,43 @@ __global__ __launch_bounds__(256) void k_tn_stream(const T *__restrict__ a, cons
         } else {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                uint64_t const j = j0 + jt * 16 + Tr::acc_row(lr, r);
+                uint64_t const j = (uint64_t)jt * 16 + Tr::acc_row(lr, r);
                 if (j < m) cp[j] = acc[jt][r];
             }
         }
     }
 }
-
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 
 int pick_jt(uint64_t m) { return m <= 16 ? 1 : (m <= 32 ? 2 : 4); }
+
+// Pipeline depth (groups of A loads in flight per wave) of the stream
+// kernels; the register cost of a group grows with JT.
+constexpr int stream_depth(int jt, bool tn) { return jt == 1 ? (tn ? 3 : 4) : (jt == 2 ? (tn ? 2 : 3) : 2); }
+
+// 16-byte vectors of B rows: row-major B with nq % VEC == 0.
+bool stream_bvec(const Fold &f, int vec) { return !f.b_colmajor && f.nq % (uint64_t)vec == 0; }
+
+template <class T, int JT, bool BROW>
+void launch_stream(const Fold &f, const T *a, const T *b, T *c, hipStream_t s, bool tn)
+{
+    constexpr int VEC = Traits<T>::VEC;
+    uint64_t const bsj = f.b_stride_j(), bsl = f.b_stride_l();
+    if (tn) {
+        constexpr int D = stream_depth(JT, true);
+        dim3 grid((unsigned)((f.outer + 63) / 64));
+        hipLaunchKernelGGL((k_tn_stream<T, JT, D, BROW>), grid, dim3(256), 0, s, a, b, c, f.outer, f.nq, f.m, bsj,
+                           bsl);
+    } else {
+        constexpr int D = stream_depth(JT, false);
+        uint64_t const ncols = f.outer * f.w;
+        dim3 grid((unsigned)((ncols + 64 * VEC - 1) / (64 * VEC)));
+        hipLaunchKernelGGL((k_nn_stream<T, JT, D, BROW>), grid, dim3(256), 0, s, a, b, c, f.outer, f.nq, f.w, f.m,
+                           bsj, bsl);
+    }
+}
 
 template <class T>
 ttm_status_t launch_plain(const Fold &f, const T *a, const T *b, T *c, hipStream_t s, Kernel kern)
@@ -328,33 +404,15 @@ ttm_status_t launch_plain(const Fold &f, const T *a, const T *b, T *c, hipStream
                            f.m, bsj, bsl);
         break;
     }
-    case Kernel::nn_stream: {
-        int const jt = pick_jt(f.m);
-        uint64_t const ncols = f.outer * f.w;
-        dim3 grid((unsigned)((ncols + 64 * VEC - 1) / (64 * VEC)), (unsigned)((f.m + 16 * jt - 1) / (16 * jt)));
-        if (jt == 1)
-            hipLaunchKernelGGL((k_nn_stream<T, 1>), grid, dim3(256), 0, s, a, b, c, f.outer, f.nq, f.w, f.m, bsj, bsl);
-        else if (jt == 2)
-            hipLaunchKernelGGL((k_nn_stream<T, 2>), grid, dim3(256), 0, s, a, b, c, f.outer, f.nq, f.w, f.m, bsj, bsl);
-        else
-            hipLaunchKernelGGL((k_nn_stream<T, 4>), grid, dim3(256), 0, s, a, b, c, f.outer, f.nq, f.w, f.m, bsj, bsl);
-        break;
-    }
+    case Kernel::nn_stream:
     case Kernel::tn_stream: {
+        // the planner only picks these for m <= 64 (one row block)
+        bool const tn = kern == Kernel::tn_stream;
         int const jt = pick_jt(f.m);
-        dim3 grid((unsigned)((f.outer + 63) / 64), (unsigned)((f.m + 16 * jt - 1) / (16 * jt)));
-#define TTM_LAUNCH_TN(JT, BROW)                                                                          \
-    hipLaunchKernelGGL((k_tn_stream<T, JT, BROW>), grid, dim3(256), 0, s, a, b, c, f.outer, f.nq, f.m, bsj, bsl)
-        if (f.b_colmajor) {
-            if (jt == 1) TTM_LAUNCH_TN(1, false);
-            else if (jt == 2) TTM_LAUNCH_TN(2, false);
-            else TTM_LAUNCH_TN(4, false);
-        } else {
-            if (jt == 1) TTM_LAUNCH_TN(1, true);
-            else if (jt == 2) TTM_LAUNCH_TN(2, true);
-            else TTM_LAUNCH_TN(4, true);
-        }
-#undef TTM_LAUNCH_TN
+        bool const bv = stream_bvec(f, VEC);
+        if (jt == 1) bv ? launch_stream<T, 1, true>(f, a, b, c, s, tn) : launch_stream<T, 1, false>(f, a, b, c, s, tn);
+        else if (jt == 2) bv ? launch_stream<T, 2, true>(f, a, b, c, s, tn) : launch_stream<T, 2, false>(f, a, b, c, s, tn);
+        else bv ? launch_stream<T, 4, true>(f, a, b, c, s, tn) : launch_stream<T, 4, false>(f, a, b, c, s, tn);
         break;
     }
     case Kernel::gemm: return launch_gemm<T>(f, a, b, c, s);
@@ -384,8 +442,14 @@ std::string kernel_symbol(Kernel k, const Fold &f, bool is_double, ttm_precision
     std::string const jt = std::to_string(pick_jt(f.m));
     switch (k) {
     case Kernel::generic: return "k_generic<" + t + ">";
-    case Kernel::nn_stream: return "k_nn_stream<" + t + ", " + jt + ">";
-    case Kernel::tn_stream: return "k_tn_stream<" + t + ", " + jt + ", " + (f.b_colmajor ? "false" : "true") + ">";
+    case Kernel::nn_stream:
+    case Kernel::tn_stream: {
+        bool const tn = k == Kernel::tn_stream;
+        int const j = pick_jt(f.m);
+        return std::string(tn ? "k_tn_stream<" : "k_nn_stream<") + t + ", " + jt + ", " +
+               std::to_string(stream_depth(j, tn)) + ", " + (stream_bvec(f, is_double ? 2 : 4) ? "true" : "false") +
+               ">";
+    }
     case Kernel::gemm: return gemm_symbol(f, is_double);
     case Kernel::lowp: return lowp_symbol(f, prec, false);
     case Kernel::lowp_generic: return lowp_symbol(f, prec, true);
@@ -410,15 +474,18 @@ Kernel choose_kernel(const Fold &f, bool is_double, ttm_precision_t prec, bool a
         return Kernel::lowp;
     }
     uint64_t const vec = is_double ? 2 : 4;
-    // grid.y of the MFMA kernels covers m in blocks of >= 16 rows
-    bool const grid_ok = (f.m + 15) / 16 <= 65535;
-    bool const nn_ok = aligned && grid_ok && f.w % vec == 0;
-    bool const tn_ok = aligned && grid_ok && f.w == 1 && f.nq % vec == 0;
+    // the stream kernels hold every row j of C in one wave: m <= 64
+    bool const small_m = f.m <= 64;
+    bool const nn_ok = aligned && small_m && f.w % vec == 0;
+    bool const tn_ok = aligned && small_m && f.w == 1 && f.nq % vec == 0;
     bool const gemm_ok = gemm_supports(f, is_double, aligned);
     if (force == Kernel::generic) return Kernel::generic;
     if (force == Kernel::nn_stream && nn_ok) return Kernel::nn_stream;
     if (force == Kernel::tn_stream && tn_ok) return Kernel::tn_stream;
     if (force == Kernel::gemm && gemm_ok) return Kernel::gemm;
+    // m <= 16 in the NN view: the register-streaming kernel reads A at
+    // 6.4-6.8 TB/s on the bench shapes, the LDS-staged tile at 5.3-5.8
+    if (nn_ok && f.m <= 16) return Kernel::nn_stream;
     if (gemm_ok) return Kernel::gemm;
     if (tn_ok) return Kernel::tn_stream;
     if (nn_ok) return Kernel::nn_stream;

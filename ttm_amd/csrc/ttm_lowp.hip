@@ -28,17 +28,23 @@
 // LDS tiles are therefore stored K-contiguous: s[row][k].
 #include "ttm_lowp.hpp"
 
+#include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
+#include <vector>
 
 namespace ttm_amd
 {
 namespace
 {
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef int i32x4 __attribute__((ext_vector_type(4)));
@@ -57,26 +63,31 @@ template <>
 struct Lp<LP_BF16>
 {
     using op = __bf16;
+    using v2 = bf16x2;
     using v4 = bf16x4;
     using v8 = bf16x8;
     static constexpr int PARTS = 1;
     __device__ static f32x4 mfma(v8 a, v8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
     __device__ static void split(float x, op (&p)[1]) { p[0] = (__bf16)x; }
+    __device__ static void split2(f32x2 x, v2 (&p)[1]) { p[0] = __builtin_convertvector(x, bf16x2); }
 };
 template <>
 struct Lp<LP_FP16>
 {
     using op = _Float16;
+    using v2 = f16x2;
     using v4 = f16x4;
     using v8 = f16x8;
     static constexpr int PARTS = 1;
     __device__ static f32x4 mfma(v8 a, v8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
     __device__ static void split(float x, op (&p)[1]) { p[0] = (_Float16)x; }
+    __device__ static void split2(f32x2 x, v2 (&p)[1]) { p[0] = __builtin_convertvector(x, f16x2); }
 };
 template <>
 struct Lp<LP_BF16X3>
 {
     using op = __bf16;
+    using v2 = bf16x2;
     using v4 = bf16x4;
     using v8 = bf16x8;
     static constexpr int PARTS = 3;
@@ -89,6 +100,15 @@ struct Lp<LP_BF16X3>
         p[1] = (__bf16)r1;
         float const r2 = r1 - (float)p[1];
         p[2] = (__bf16)r2;
+    }
+    // The same for two values at once (paired conversions).
+    __device__ static void split2(f32x2 x, v2 (&p)[3])
+    {
+        p[0] = __builtin_convertvector(x, bf16x2);
+        f32x2 const r1 = x - __builtin_convertvector(p[0], f32x2);
+        p[1] = __builtin_convertvector(r1, bf16x2);
+        f32x2 const r2 = r1 - __builtin_convertvector(p[1], f32x2);
+        p[2] = __builtin_convertvector(r2, bf16x2);
     }
 };
 
@@ -119,12 +139,31 @@ enum LpView : int
     LPV_TT = 2
 };
 constexpr int LP_BM = 128, LP_BN = 128, LP_BK = 32;
-constexpr int LP_LDK = LP_BK + 8; // 80-byte rows: the 16 rows of a fragment read hit distinct banks
+constexpr int LP_LDK = LP_BK; // 64-byte rows; bank spreading by the row order and chunk swizzle below
 
-// One 128 x 128 block of D at (j0, n0).  flags: TILE_A_VEC / TILE_C_VEC as in
-// ttm_gemm.hpp (16-byte loads of A rows, 16-byte stores of C rows).
-template <int MODE, int VIEW, bool BCOL>
+// LDS layout of a 128-row x 32-k operand tile (one part).  MFMA tile t of a
+// wave reads the logical rows {64h + 4x + t : x < 16} (interleaved
+// fragments), so logical row r = 64h + 4y + x is kept at physical row
+// lds_row(r) = 64h + 16x + y: a fragment read covers 16 consecutive physical
+// rows.  Each physical row P holds four 16-byte chunks (8 k each); chunk c
+// sits at position c ^ lds_swz(P).  With this order every fragment read
+// (16 rows, one chunk), every 4-row x 4-chunk store and every 16-row x
+// 8-byte store touches distinct banks (checked for 32 banks / 8 lanes and
+// 64 banks / 16 lanes per cycle).
+__device__ __forceinline__ int lds_row(int r) { return (r & ~63) | ((r & 3) << 4) | ((r & 63) >> 2); }
+__device__ __forceinline__ int lds_row_inv(int P) { return (P & ~63) | ((P & 15) << 2) | ((P & 63) >> 4); }
+__device__ __forceinline__ int lds_swz(int P) { return ((P >> 1) & 3) ^ ((P >> 3) & 1); }
+// element offset of (physical row P, chunk c) within a part
+__device__ __forceinline__ int lds_at(int P, int c) { return P * LP_LDK + 8 * (c ^ lds_swz(P)); }
+
+// One 128 x 128 block of D at (j0, n0).  flags: TILE_A_VEC / TILE_C_VEC /
+// TILE_STREAM as in ttm_gemm.hpp.  With BSPLIT, B comes pre-converted: P
+// planes of `op` at bs + p * plane, row j at j * ldb, zero padded to
+// multiples of 32 columns and 128 rows (k_lowp_split_b); otherwise B is the
+// row-major float32 m x nq matrix b, converted here.
+template <int MODE, int VIEW, bool BSPLIT>
 __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const float *__restrict__ b,
+                                          const typename Lp<MODE>::op *__restrict__ bs, uint64_t ldb, uint64_t plane,
                                           float *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w, uint64_t m,
                                           uint64_t j0, uint64_t n0, unsigned flags,
                                           typename Lp<MODE>::op (*sA)[LP_BN * LP_LDK],
@@ -132,136 +171,163 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
 {
     using L = Lp<MODE>;
     using op = typename L::op;
+    using op2 = typename L::v2;
     using v4 = typename L::v4;
     using v8 = typename L::v8;
     constexpr int P = L::PARTS;
-    constexpr int BM = LP_BM, BN = LP_BN, BK = LP_BK;
-    constexpr int LDK = LP_LDK;
+    constexpr int BN = LP_BN, BK = LP_BK;
     constexpr bool TN = VIEW != LPV_NN;
     constexpr bool TSTORE = VIEW == LPV_TN;
     bool const a_vec = (flags & TILE_A_VEC) != 0;
     bool const c_vec = (flags & TILE_C_VEC) != 0;
+    bool const stream = (flags & TILE_STREAM) != 0;
     int const tid = threadIdx.x;
     int const lane = tid & 63, wave = tid >> 6;
     int const lc = lane & 15, lg = lane >> 4;
     int const wm = wave & 1, wn = wave >> 1;
 
     // ---- loaders --------------------------------------------------------
-    // "MN" sources (NN A, column-major B): a thread loads a 4 (k) x 4 (row)
-    // micro-block with four 16-byte loads along the rows and writes each
-    // row's 4 k-values as one 8-byte LDS store.  "K" sources (TN A, row-major
-    // B): a thread loads 16 consecutive k of one row and writes 16-byte stores.
-    // A tile: rows n (BN), B tile: rows j (BM); both 128 rows x 32 k.
-    int const mn_c4 = (tid & 31) * 4; // first of 4 rows (columns of the source)
-    int const mn_k4 = (tid >> 5) * 4; // first of 4 k
-    int const k_row = tid >> 1;       // K sources: row
-    int const k_k16 = (tid & 1) * 16; // K sources: first of 16 k
+    // "MN" source (NN A): a thread loads a 4 (k) x 4 (row) micro-block with
+    // four 16-byte loads along the rows (logical rows 64H + 4y + x, x < 4;
+    // k = 8*wave + 4*half + e) and writes each row's 4 k-values as one 8-byte
+    // store.  "K" sources (TN A, row-major B): a thread loads 8 consecutive k
+    // (chunk kc) of the two physical rows R and R + 64 and writes one 16-byte
+    // store per row.  A tile: rows n, B tile: rows j; both 128 rows x 32 k.
+    int const mn_half = lane & 1, mn_y = (lane >> 1) & 15, mn_h = lane >> 5;
+    int const mn_c4 = 64 * mn_h + 4 * mn_y;    // first of the thread's 4 logical rows
+    int const mn_k4 = 8 * wave + 4 * mn_half; // first of its 4 k
+    int const k_c = tid & 3;                  // K sources: chunk (k = 8*k_c .. 8*k_c + 7)
+    int const k_p = tid >> 2;                 // K sources: physical rows k_p and k_p + 64
+    int const k_row = lds_row_inv(k_p);       // logical row of k_p (k_row + 64 for the other)
 
-    // A addresses
-    uint64_t a_col_base = 0;
-    bool a_ok;
+    // A: per-thread base addresses (rows past the end read row 0 and are dropped)
+    uint64_t a_base[2] = {0, 0};
+    bool a_ok[2] = {false, false};
     if constexpr (!TN) {
         uint64_t const n = n0 + mn_c4;
-        a_ok = n < ncols;
-        uint64_t const k = a_ok ? n / w : 0;
-        a_col_base = k * nq * w + (a_ok ? n - k * w : 0);
+        a_ok[0] = n < ncols;
+        uint64_t const k = a_ok[0] ? n / w : 0;
+        a_base[0] = k * nq * w + (a_ok[0] ? n - k * w : 0) + (uint64_t)mn_k4 * w;
     } else {
-        uint64_t const n = n0 + k_row;
-        a_ok = n < ncols;
-        a_col_base = a_ok ? n * nq : 0;
-    }
-    bool const b_vec = BCOL ? (m % 4 == 0) : (nq % 4 == 0);
-
-    f32x4 ra[4], rb[4];
-    auto load = [&](uint64_t l0) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            ra[e] = zero4();
-            rb[e] = zero4();
+        for (int s = 0; s < 2; ++s) {
+            uint64_t const n = n0 + k_row + 64 * s;
+            a_ok[s] = n < ncols;
+            a_base[s] = (a_ok[s] ? n * nq : 0) + 8 * k_c;
         }
-        if constexpr (!TN) {
+    }
+    // every column of the tile in range and whole slices: unguarded 16-byte loads
+    bool const full = n0 + BN <= ncols && nq % BK == 0 && (!TN || a_vec);
+    bool const b_vec = nq % 4 == 0;
+
+    f32x4 ra[4];
+    f32x4 rb[4];
+    i32x4 rbs[BSPLIT ? P : 1][2];
+    auto load_a_vec = [&](const float *p) -> f32x4 {
+        return stream ? __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(p))
+                      : *reinterpret_cast<const f32x4 *>(p);
+    };
+    auto load = [&](uint64_t l0) {
+        if (full) {
+            if constexpr (!TN) {
+                const float *pa = a + a_base[0] + l0 * w;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                uint64_t const l = l0 + mn_k4 + e;
-                if (a_ok && l < nq) ra[e] = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(a + a_col_base + l * w));
+                for (int e = 0; e < 4; ++e) ra[e] = load_a_vec(pa + e * w);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) ra[e] = load_a_vec(a + a_base[e >> 1] + l0 + 4 * (e & 1));
             }
         } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                uint64_t const l = l0 + k_k16 + 4 * e;
-                if (a_ok && l < nq) {
-                    if (a_vec) {
-                        ra[e] = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(a + a_col_base + l));
-                    } else {
+            for (int e = 0; e < 4; ++e) ra[e] = zero4();
+            if constexpr (!TN) {
 #pragma unroll
-                        for (int x = 0; x < 4; ++x)
-                            if (l + x < nq) ra[e][x] = a[a_col_base + l + x];
+                for (int e = 0; e < 4; ++e) {
+                    uint64_t const l = l0 + mn_k4 + e;
+                    if (a_ok[0] && l < nq) ra[e] = load_a_vec(a + a_base[0] + (l0 + e) * w);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    int const s = e >> 1;
+                    uint64_t const l = l0 + 8 * k_c + 4 * (e & 1);
+                    if (a_ok[s] && l < nq) {
+                        if (a_vec) {
+                            ra[e] = load_a_vec(a + a_base[s] + l0 + 4 * (e & 1));
+                        } else {
+#pragma unroll
+                            for (int x = 0; x < 4; ++x)
+                                if (l + x < nq) ra[e][x] = a[a_base[s] + l0 + 4 * (e & 1) + x];
+                        }
                     }
                 }
             }
         }
-        if constexpr (BCOL) {
-            uint64_t const j = j0 + mn_c4;
+        if constexpr (BSPLIT) {
+            // padded planes: every load in range
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+                    rbs[p][s] = *reinterpret_cast<const i32x4 *>(bs + p * plane + (j0 + k_row + 64 * s) * ldb + l0 +
+                                                                 8 * k_c);
+        } else {
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                uint64_t const l = l0 + mn_k4 + e;
-                if (l >= nq) continue;
-                if (b_vec && j + 4 <= m) {
-                    rb[e] = *reinterpret_cast<const f32x4 *>(b + j + l * m);
+                rb[e] = zero4();
+                uint64_t const j = j0 + k_row + 64 * (e >> 1);
+                uint64_t const l = l0 + 8 * k_c + 4 * (e & 1);
+                if (j >= m) continue;
+                if (b_vec && l + 4 <= nq) {
+                    rb[e] = *reinterpret_cast<const f32x4 *>(b + j * nq + l);
                 } else {
 #pragma unroll
                     for (int x = 0; x < 4; ++x)
-                        if (j + x < m) rb[e][x] = b[j + x + l * m];
-                }
-            }
-        } else {
-            uint64_t const j = j0 + k_row;
-            if (j < m) {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    uint64_t const l = l0 + k_k16 + 4 * e;
-                    if (b_vec && l + 4 <= nq) {
-                        rb[e] = *reinterpret_cast<const f32x4 *>(b + j * nq + l);
-                    } else {
-#pragma unroll
-                        for (int x = 0; x < 4; ++x)
-                            if (l + x < nq) rb[e][x] = b[j * nq + l + x];
-                    }
+                        if (l + x < nq) rb[e][x] = b[j * nq + l + x];
                 }
             }
         }
     };
 
-    // Convert and store: MN micro-block (ra[e][x] = row x, k e) or K run.
-    auto store_mn = [&](f32x4 (&r)[4], op (*s)[BN * LDK]) {
+    // Convert and store: MN micro-block (r[e][x] = logical row mn_c4 + x,
+    // k mn_k4 + e) or K rows (r[2s], r[2s+1] = 8 k of physical row k_p + 64s).
+    auto store_mn = [&](f32x4 (&r)[4], op (*s)[BN * LP_LDK]) {
 #pragma unroll
         for (int x = 0; x < 4; ++x) {
-            v4 out[P];
+            op2 lo[P], hi[P];
+            f32x2 const x01 = {r[0][x], r[1][x]};
+            f32x2 const x23 = {r[2][x], r[3][x]};
+            L::split2(x01, lo);
+            L::split2(x23, hi);
+            int const off = lds_at(lds_row(mn_c4 + x), wave) + 4 * mn_half;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                op parts[P];
-                L::split(r[e][x], parts);
-#pragma unroll
-                for (int p = 0; p < P; ++p) out[p][e] = parts[p];
-            }
-#pragma unroll
-            for (int p = 0; p < P; ++p) *reinterpret_cast<v4 *>(&s[p][(mn_c4 + x) * LDK + mn_k4]) = out[p];
+            for (int p = 0; p < P; ++p)
+                *reinterpret_cast<v4 *>(&s[p][off]) = __builtin_shufflevector(lo[p], hi[p], 0, 1, 2, 3);
         }
     };
-    auto store_k = [&](f32x4 (&r)[4], op (*s)[BN * LDK]) {
+    auto store_k = [&](f32x4 (&r)[4], op (*s)[BN * LP_LDK]) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            v8 out[P];
+            op2 q[4][P];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                op parts[P];
-                L::split(r[2 * h + e / 4][e % 4], parts);
-#pragma unroll
-                for (int p = 0; p < P; ++p) out[p][e] = parts[p];
+            for (int t = 0; t < 4; ++t) {
+                f32x2 const x = {r[2 * h + t / 2][2 * (t % 2)], r[2 * h + t / 2][2 * (t % 2) + 1]};
+                L::split2(x, q[t]);
             }
+            int const off = lds_at(k_p + 64 * h, k_c);
 #pragma unroll
-            for (int p = 0; p < P; ++p) *reinterpret_cast<v8 *>(&s[p][k_row * LDK + k_k16 + 8 * h]) = out[p];
+            for (int p = 0; p < P; ++p) {
+                v4 const lo = __builtin_shufflevector(q[0][p], q[1][p], 0, 1, 2, 3);
+                v4 const hi = __builtin_shufflevector(q[2][p], q[3][p], 0, 1, 2, 3);
+                *reinterpret_cast<v8 *>(&s[p][off]) = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+            }
         }
+    };
+    auto store_bs = [&]() {
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) *reinterpret_cast<i32x4 *>(&sB[p][lds_at(k_p + 64 * s, k_c)]) = rbs[p][s];
     };
 
     f32x4 acc[4][4];
@@ -272,16 +338,17 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
 
     // Interleaved fragments: tile jt holds rows 4*x + jt (x < 16) of the
     // wave's 64 rows, tile nt columns 4*y + nt, so a lane's 4 results of a
-    // row (column) are adjacent and leave with one 16-byte store.
-    int const arow = wn * 64 + 4 * lc; // + nt
-    int const brow = wm * 64 + 4 * lc; // + jt
+    // row (column) are adjacent and leave with one 16-byte store.  Physical
+    // rows: tile t, lane lc -> 64*half + 16*t + lc.
+    int const arow = wn * 64 + lc; // + 16 * nt
+    int const brow = wm * 64 + lc; // + 16 * jt
 
     uint64_t const nslices = (nq + BK - 1) / BK;
     load(0);
     for (uint64_t s = 0; s < nslices; ++s) {
         if constexpr (!TN) store_mn(ra, sA);
         else store_k(ra, sA);
-        if constexpr (BCOL) store_mn(rb, sB);
+        if constexpr (BSPLIT) store_bs();
         else store_k(rb, sB);
         __syncthreads();
         if (s + 1 < nslices) load((s + 1) * BK);
@@ -289,12 +356,13 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
 #pragma unroll
         for (int p = 0; p < P; ++p)
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) fa[p][nt] = *reinterpret_cast<const v8 *>(&sA[p][(arow + nt) * LDK + 8 * lg]);
+            for (int nt = 0; nt < 4; ++nt)
+                fa[p][nt] = *reinterpret_cast<const v8 *>(&sA[p][lds_at(arow + 16 * nt, lg)]);
 #pragma unroll
         for (int jt = 0; jt < 4; ++jt) {
             v8 fb[P];
 #pragma unroll
-            for (int p = 0; p < P; ++p) fb[p] = *reinterpret_cast<const v8 *>(&sB[p][(brow + jt) * LDK + 8 * lg]);
+            for (int p = 0; p < P; ++p) fb[p] = *reinterpret_cast<const v8 *>(&sB[p][lds_at(brow + 16 * jt, lg)]);
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
                 if constexpr (P == 3) {
@@ -354,25 +422,61 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     }
 }
 
-// One product: a grid of 128 x 128 blocks.
-template <int MODE, bool TN, bool BCOL>
-__global__ __launch_bounds__(256) void k_lowp(const float *__restrict__ a, const float *__restrict__ b,
-                                              float *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w,
-                                              uint64_t m, unsigned nj, unsigned total)
+// B -> P planes of `op` (the operand conversion of the tile, done once per
+// product): plane p at bs + p * plane, row j at j * ldb, zero outside m x nq.
+// One thread per 8 consecutive l of a row.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_lowp_split_b(const float *__restrict__ b, typename Lp<MODE>::op *__restrict__ bs,
+                                                      uint64_t m, uint64_t nq, uint64_t bsj, uint64_t bsl, uint64_t ldb,
+                                                      uint64_t rows, uint64_t plane)
+{
+    using L = Lp<MODE>;
+    using v2 = typename L::v2;
+    using v8 = typename L::v8;
+    constexpr int P = L::PARTS;
+    uint64_t const groups = ldb / 8;
+    uint64_t const idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= rows * groups) return;
+    uint64_t const j = idx / groups, l0 = (idx - j * groups) * 8;
+    v2 q[4][P];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x2 x;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            uint64_t const l = l0 + 2 * t + e;
+            x[e] = (j < m && l < nq) ? b[j * bsj + l * bsl] : 0.f;
+        }
+        L::split2(x, q[t]);
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        auto const lo = __builtin_shufflevector(q[0][p], q[1][p], 0, 1, 2, 3);
+        auto const hi = __builtin_shufflevector(q[2][p], q[3][p], 0, 1, 2, 3);
+        *reinterpret_cast<v8 *>(bs + p * plane + j * ldb + l0) = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    }
+}
+
+// One product: a grid of 128 x 128 blocks over the pre-converted B.
+template <int MODE, bool TN>
+__global__ __launch_bounds__(256, 2) void k_lowp(const float *__restrict__ a, const typename Lp<MODE>::op *__restrict__ bs,
+                                                 uint64_t ldb, uint64_t plane, float *__restrict__ c, uint64_t ncols,
+                                                 uint64_t nq, uint64_t w, uint64_t m, unsigned nj, unsigned total)
 {
     using op = typename Lp<MODE>::op;
     __shared__ __attribute__((aligned(16))) op sA[Lp<MODE>::PARTS][LP_BN * LP_LDK];
     __shared__ __attribute__((aligned(16))) op sB[Lp<MODE>::PARTS][LP_BM * LP_LDK];
     unsigned jb, nb;
     lp_block_coords(nj, total, jb, nb);
-    lowp_tile<MODE, TN ? LPV_TN : LPV_NN, BCOL>(a, b, c, ncols, nq, w, m, (uint64_t)jb * LP_BM,
-                                                (uint64_t)nb * LP_BN, TILE_A_VEC | TILE_C_VEC, sA, sB);
+    lowp_tile<MODE, TN ? LPV_TN : LPV_NN, true>(a, nullptr, bs, ldb, plane, c, ncols, nq, w, m, (uint64_t)jb * LP_BM,
+                                                (uint64_t)nb * LP_BN,
+                                                TILE_A_VEC | TILE_C_VEC | (nj == 1 ? TILE_STREAM : 0u), sA, sB);
 }
 
 // Many TT products (stages of multi-TTM chains) in one persistent launch.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_lowp_grouped(const GroupProblem *__restrict__ probs,
-                                                      const unsigned *__restrict__ tile_prob, unsigned total)
+__global__ __launch_bounds__(256, 2) void k_lowp_grouped(const GroupProblem *__restrict__ probs,
+                                                         const unsigned *__restrict__ tile_prob, unsigned total)
 {
     using op = typename Lp<MODE>::op;
     __shared__ __attribute__((aligned(16))) op sA[Lp<MODE>::PARTS][LP_BN * LP_LDK];
@@ -381,9 +485,9 @@ __global__ __launch_bounds__(256) void k_lowp_grouped(const GroupProblem *__rest
         GroupProblem const &P = probs[tile_prob[t]];
         unsigned const local = t - P.tile_begin;
         unsigned const jb = local % P.nj, nb = local / P.nj;
-        lowp_tile<MODE, LPV_TT, false>(static_cast<const float *>(P.a), static_cast<const float *>(P.b),
-                                       static_cast<float *>(P.c), P.ncols, P.nq, P.ncols, P.m,
-                                       (uint64_t)jb * LP_BM, (uint64_t)nb * LP_BN, P.flags, sA, sB);
+        lowp_tile<MODE, LPV_TT, false>(static_cast<const float *>(P.a), static_cast<const float *>(P.b), nullptr, 0, 0,
+                                       static_cast<float *>(P.c), P.ncols, P.nq, P.ncols, P.m, (uint64_t)jb * LP_BM,
+                                       (uint64_t)nb * LP_BN, P.flags, sA, sB);
     }
 }
 
@@ -597,14 +701,73 @@ __global__ __launch_bounds__(256) void k_lowp_generic(const float *__restrict__ 
     }
 }
 
-template <int MODE, bool TN, bool BCOL>
-void launch_lp(const Fold &f, const float *a, const float *b, float *c, hipStream_t s)
+// Stream-ordered device workspace (the pre-converted B operand): one pool per
+// device that keeps its memory between calls, so a steady stream of products
+// allocates nothing from the driver after the first.
+hipMemPool_t workspace_pool()
 {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    static std::mutex mu;
+    static std::vector<hipMemPool_t> pools;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)pools.size() <= dev) pools.resize(dev + 1, nullptr);
+    if (!pools[dev]) {
+        hipMemPoolProps props = {};
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t pool = nullptr;
+        if (hipMemPoolCreate(&pool, &props) != hipSuccess) return nullptr;
+        uint64_t keep = UINT64_MAX;
+        if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) {
+            (void)hipMemPoolDestroy(pool);
+            return nullptr;
+        }
+        pools[dev] = pool;
+    }
+    return pools[dev];
+}
+
+// Convert B once into padded planes (ldb = nq rounded up to 32, rows = m
+// rounded up to 128), then run the blocks; the workspace is released in
+// stream order after the product.
+template <int MODE, bool TN>
+ttm_status_t launch_lp(const Fold &f, const float *a, const float *b, float *c, hipStream_t s)
+{
+    using op = typename Lp<MODE>::op;
+    constexpr int P = Lp<MODE>::PARTS;
     uint64_t const ncols = TN ? f.outer : f.outer * f.w;
     unsigned const nj = (unsigned)((f.m + 127) / 128);
     unsigned const total = nj * (unsigned)((ncols + 127) / 128);
-    hipLaunchKernelGGL((k_lowp<MODE, TN, BCOL>), dim3(total), dim3(256), 0, s, a, b, c, ncols, f.nq, f.w, f.m, nj,
-                       total);
+    uint64_t const ldb = (f.nq + 31) / 32 * 32;
+    uint64_t const rows = (uint64_t)nj * 128;
+    uint64_t const plane = rows * ldb;
+    uint64_t const split_blocks = (rows * (ldb / 8) + 255) / 256;
+    if (split_blocks >= (1ull << 31)) {
+        set_error(TTM_ERR_ARGUMENT, "B too large for the reduced-precision kernels");
+        return TTM_ERR_ARGUMENT;
+    }
+    hipMemPool_t const pool = workspace_pool();
+    void *ws = nullptr;
+    if (!pool || hipMallocFromPoolAsync(&ws, P * plane * sizeof(op), pool, s) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error(TTM_ERR_ALLOC, "workspace for the converted B operand");
+        return TTM_ERR_ALLOC;
+    }
+    op *const bs = static_cast<op *>(ws);
+    hipLaunchKernelGGL((k_lowp_split_b<MODE>), dim3((unsigned)split_blocks), dim3(256), 0, s, b, bs, f.m, f.nq,
+                       f.b_stride_j(), f.b_stride_l(), ldb, rows, plane);
+    hipLaunchKernelGGL((k_lowp<MODE, TN>), dim3(total), dim3(256), 0, s, a, bs, ldb, plane, c, ncols, f.nq, f.w, f.m,
+                       nj, total);
+    hipError_t const e = hipGetLastError();
+    (void)hipFreeAsync(ws, s);
+    if (e != hipSuccess) {
+        set_error(TTM_ERR_HIP, hipGetErrorString(e));
+        return TTM_ERR_HIP;
+    }
+    return TTM_SUCCESS;
 }
 
 template <bool TN, bool BCOL>
@@ -617,16 +780,9 @@ void launch_fp8(const Fold &f, const float *a, const float *b, float *c, hipStre
 }
 
 template <int MODE>
-void launch_lp_mode(const Fold &f, const float *a, const float *b, float *c, hipStream_t s)
+ttm_status_t launch_lp_mode(const Fold &f, const float *a, const float *b, float *c, hipStream_t s)
 {
-    bool const tn = f.w == 1;
-    if (tn) {
-        if (f.b_colmajor) launch_lp<MODE, true, true>(f, a, b, c, s);
-        else launch_lp<MODE, true, false>(f, a, b, c, s);
-    } else {
-        if (f.b_colmajor) launch_lp<MODE, false, true>(f, a, b, c, s);
-        else launch_lp<MODE, false, false>(f, a, b, c, s);
-    }
+    return f.w == 1 ? launch_lp<MODE, true>(f, a, b, c, s) : launch_lp<MODE, false>(f, a, b, c, s);
 }
 
 } // namespace
@@ -651,7 +807,7 @@ std::string lowp_symbol(const Fold &f, ttm_precision_t prec, bool generic)
         std::snprintf(buf, sizeof buf, "k_fp8<%s, %s>", tn, bc);
     } else {
         int const mode = prec == TTM_PREC_BF16 ? LP_BF16 : (prec == TTM_PREC_FP16 ? LP_FP16 : LP_BF16X3);
-        std::snprintf(buf, sizeof buf, "k_lowp<%d, %s, %s>", mode, tn, bc);
+        std::snprintf(buf, sizeof buf, "k_lowp<%d, %s>", mode, tn);
     }
     return buf;
 }
@@ -675,11 +831,11 @@ ttm_status_t launch_lowp(const Fold &f, const float *a, const float *b, float *c
             else launch_fp8<false, false>(f, a, b, c, s);
         }
     } else if (prec == TTM_PREC_BF16) {
-        launch_lp_mode<LP_BF16>(f, a, b, c, s);
+        return launch_lp_mode<LP_BF16>(f, a, b, c, s);
     } else if (prec == TTM_PREC_FP16) {
-        launch_lp_mode<LP_FP16>(f, a, b, c, s);
+        return launch_lp_mode<LP_FP16>(f, a, b, c, s);
     } else {
-        launch_lp_mode<LP_BF16X3>(f, a, b, c, s);
+        return launch_lp_mode<LP_BF16X3>(f, a, b, c, s);
     }
     hipError_t const e = hipGetLastError();
     if (e != hipSuccess) {
