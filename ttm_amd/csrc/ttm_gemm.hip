@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 namespace ttm_amd
 {
@@ -207,6 +208,14 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
 
     V ra[A_PER], rb[B_PER];
 
+    // Interior tile (every row and column in range, whole slices, 16-byte
+    // vectors throughout): unguarded loads, all in flight together.
+    bool const full = n0 + BN <= ncols && j0 + BM <= m && nq % BK == 0 && (!TN || a_vec) && b_vec_ok;
+    uint64_t a_fix[A_PER]; // A offset of the thread's vector in slice 0
+#pragma unroll
+    for (int e = 0; e < A_PER; ++e) a_fix[e] = TN ? a_base[e] + a_row[e] : a_base[e] + (uint64_t)a_row[e] * w;
+    uint64_t const a_step = TN ? 1 : w; // A offset per contraction index
+
     auto load_tiles = [&](uint64_t l0) {
 #pragma unroll
         for (int e = 0; e < A_PER; ++e) {
@@ -260,6 +269,24 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
         }
     };
 
+    // Interior tiles: the same loads without guards or branches, so all of a
+    // slice's loads stay in flight while the previous slice is multiplied.
+    // Every thread owns whole A vectors (A_VECS is a multiple of 256); B
+    // vectors past B_VECS (BM * BK < 256 * VEC) re-read the tile's first
+    // vector, B(j0, l0), which is in range, and are not stored.
+    static_assert(A_VECS % 256 == 0, "every thread loads whole A vectors");
+    auto load_full = [&](uint64_t l0) {
+#pragma unroll
+        for (int e = 0; e < A_PER; ++e) ra[e] = load_a(a + a_fix[e] + l0 * a_step);
+#pragma unroll
+        for (int e = 0; e < B_PER; ++e) {
+            uint64_t const jj = b_in[e] ? (uint64_t)b_col[e] : 0;
+            uint64_t const ll = b_in[e] ? (uint64_t)b_row[e] : 0;
+            if constexpr (BCOL) rb[e] = *reinterpret_cast<const V *>(b + j0 + jj + (l0 + ll) * m);
+            else rb[e] = *reinterpret_cast<const V *>(b + (j0 + jj) * nq + l0 + ll);
+        }
+    };
+
     auto store_tiles = [&](int buf) {
 #pragma unroll
         for (int e = 0; e < A_PER; ++e) {
@@ -293,27 +320,36 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     int const b_off = wm * 16 * JT + JT * lc; // this lane's rows within the B tile
 
     uint64_t const nslices = (nq + BK - 1) / BK;
-    load_tiles(0);
-    store_tiles(0);
-    __syncthreads();
-    for (uint64_t s = 0; s < nslices; ++s) {
-        int const buf = (int)(s & 1);
-        bool const more = s + 1 < nslices;
-        if (more) load_tiles((s + 1) * BK);
-#pragma unroll
-        for (int kk = 0; kk < BK / 4; ++kk) {
-            int const kr = kk * 4 + lr;
-            T fa[JT], fb[NT];
-            lds_read<T, JT>(&sB[buf][kr * LDB + b_off], fa);
-            lds_read<T, NT>(&sA[buf][kr * LDA + a_off], fb);
-#pragma unroll
-            for (int jt = 0; jt < JT; ++jt)
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) acc[jt][nt] = Gt::mfma(fa[jt], fb[nt], acc[jt][nt]);
-        }
-        if (more) store_tiles(buf ^ 1);
+    auto kloop = [&](auto full_tag) {
+        constexpr bool FULL = decltype(full_tag)::value;
+        auto load = [&](uint64_t l0) {
+            if constexpr (FULL) load_full(l0);
+            else load_tiles(l0);
+        };
+        load(0);
+        store_tiles(0);
         __syncthreads();
-    }
+        for (uint64_t s = 0; s < nslices; ++s) {
+            int const buf = (int)(s & 1);
+            bool const more = s + 1 < nslices;
+            if (more) load((s + 1) * BK);
+#pragma unroll
+            for (int kk = 0; kk < BK / 4; ++kk) {
+                int const kr = kk * 4 + lr;
+                T fa[JT], fb[NT];
+                lds_read<T, JT>(&sB[buf][kr * LDB + b_off], fa);
+                lds_read<T, NT>(&sA[buf][kr * LDA + a_off], fb);
+#pragma unroll
+                for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt) acc[jt][nt] = Gt::mfma(fa[jt], fb[nt], acc[jt][nt]);
+            }
+            if (more) store_tiles(buf ^ 1);
+            __syncthreads();
+        }
+    };
+    if (full) kloop(std::true_type());
+    else kloop(std::false_type());
 
     // ---- epilogue ---------------------------------------------------------
     uint64_t const ncol0 = n0 + a_off; // first of this lane's NT adjacent columns
@@ -387,7 +423,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
 
 // One product: a grid of BM x BN blocks (row blocks fastest, XCD-grouped).
 template <class T, int JT, int NT, int WAVES_M, int BK, bool TN, bool BCOL>
-__global__ __launch_bounds__(256) void k_gemm(const T *__restrict__ a, const T *__restrict__ b,
+__global__ __launch_bounds__(256, 2) void k_gemm(const T *__restrict__ a, const T *__restrict__ b,
                                               T *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w,
                                               uint64_t m, unsigned nj, unsigned total)
 {
@@ -406,7 +442,7 @@ __global__ __launch_bounds__(256) void k_gemm(const T *__restrict__ a, const T *
 // Every problem is a TT product with a row-major B (a stage of a multi-TTM
 // chain); tile_prob[t] names the problem of tile t.
 template <class T, int JT, int NT, int WAVES_M, int BK>
-__global__ __launch_bounds__(256) void k_gemm_grouped(const GroupProblem *__restrict__ probs,
+__global__ __launch_bounds__(256, 2) void k_gemm_grouped(const GroupProblem *__restrict__ probs,
                                                       const unsigned *__restrict__ tile_prob, unsigned total)
 {
     using S = TileShape<T, JT, NT, WAVES_M, BK>;
