@@ -159,7 +159,8 @@ ttm_status_t ttm_chain_run_graph(ttm_chain_plan_t plan, ttm_stream_t stream);
  * fastest (also for later plans of the same shape class in this process, and
  * in the file named by TTM_AMD_TUNE_CACHE when set).  Drops a captured graph. */
 ttm_status_t ttm_chain_tune(ttm_chain_plan_t plan, ttm_stream_t stream);
-/* Stages, launches and the tile shape (kernel instance) of stage `s`. */
+/* Launches of the plan (each stage of the chains split by row class of the
+ * ranks, in execution order) and the tile shape (kernel instance) of launch `s`. */
 unsigned ttm_chain_stages(ttm_chain_plan_t plan);
 const char *ttm_chain_stage_kernel(ttm_chain_plan_t plan, unsigned s);
 void ttm_chain_plan_destroy(ttm_chain_plan_t plan);

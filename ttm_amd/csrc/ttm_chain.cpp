@@ -299,8 +299,13 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
     }
 
     // ---- stages ---------------------------------------------------------------
-    plan->stages.resize(max_p);
-    std::vector<uint64_t> stage_max_m(max_p, 0), stage_cols(max_p, 0), stage_nq(max_p, 0);
+    // Stage s of every chain, split by row class of m (<= 16, <= 32, <= 64,
+    // larger) so that each launch gets a tile height that fits its ranks;
+    // launches of one stage are independent, stages run in order.
+    int const classes = precision == TTM_PREC_BF16X3 ? 1 : 4;
+    auto m_class = [&](uint64_t m) { return classes == 1 ? 0 : (m <= 16 ? 0 : m <= 32 ? 1 : m <= 64 ? 2 : 3); };
+    std::vector<Stage> groups(max_p * classes);
+    std::vector<uint64_t> stage_max_m(groups.size(), 0), stage_cols(groups.size(), 0), stage_nq(groups.size(), 0);
     size_t ws_off = ident_bytes;
     for (size_t t = 0; t < count; ++t) {
         ttm_chain_desc_t const &d = descs[t];
@@ -325,10 +330,11 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
             bool const a_al = (reinterpret_cast<uintptr_t>(P.a) & 15u) == 0;
             bool const c_al = (reinterpret_cast<uintptr_t>(P.c) & 15u) == 0;
             P.flags = ((a_al && P.nq % vec == 0) ? TILE_A_VEC : 0u) | ((c_al && ncols % vec == 0) ? TILE_C_VEC : 0u);
-            plan->stages[s].probs.push_back(P);
-            stage_max_m[s] = std::max(stage_max_m[s], P.m);
-            stage_cols[s] += ncols;
-            stage_nq[s] = std::max(stage_nq[s], P.nq);
+            size_t const g = (size_t)s * classes + m_class(P.m);
+            groups[g].probs.push_back(P);
+            stage_max_m[g] = std::max(stage_max_m[g], P.m);
+            stage_cols[g] += ncols;
+            stage_nq[g] = std::max(stage_nq[g], P.nq);
             cur[mode] = P.m;
             in = out;
         }
@@ -336,16 +342,18 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         tune_load_locked();
-        for (size_t s = 0; s < max_p; ++s) {
-            Stage &st = plan->stages[s];
+        for (size_t g = 0; g < groups.size(); ++g) {
+            Stage &st = groups[g];
+            if (st.probs.empty()) continue;
             std::ostringstream key;
-            key << (is_double ? "f64" : "f32") << "/p" << (int)precision << "/m" << log2_class(stage_max_m[s])
-                << "/nq" << log2_class(stage_nq[s]) << "/cols" << log2_class(stage_cols[s]) << "/n"
+            key << (is_double ? "f64" : "f32") << "/p" << (int)precision << "/m" << log2_class(stage_max_m[g])
+                << "/nq" << log2_class(stage_nq[g]) << "/cols" << log2_class(stage_cols[g]) << "/n"
                 << log2_class(st.probs.size());
             st.key = key.str();
-            st.cfg = default_config(stage_max_m[s], precision == TTM_PREC_BF16X3);
+            st.cfg = default_config(stage_max_m[g], precision == TTM_PREC_BF16X3);
             auto it = g_tune_cache.find(st.key);
             if (precision != TTM_PREC_BF16X3 && it != g_tune_cache.end()) st.cfg = group_config(it->second);
+            plan->stages.push_back(std::move(st));
         }
     }
     ttm_status_t const st = layout_tiles(plan);

@@ -32,6 +32,7 @@
 #include <cstdio>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace ttm_amd
@@ -220,31 +221,35 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     bool const full = n0 + BN <= ncols && nq % BK == 0 && (!TN || a_vec);
     bool const b_vec = nq % 4 == 0;
 
-    f32x4 ra[4];
-    f32x4 rb[4];
-    i32x4 rbs[BSPLIT ? P : 1][2];
+    // Registers of the next slice's global loads (one set; the index keeps
+    // the loaders generic over register sets).
+    using I0 = std::integral_constant<int, 0>;
+    f32x4 ra[1][4];
+    f32x4 rb[1][4];
+    i32x4 rbs[1][BSPLIT ? P : 1][2];
     auto load_a_vec = [&](const float *p) -> f32x4 {
         return stream ? __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(p))
                       : *reinterpret_cast<const f32x4 *>(p);
     };
-    auto load = [&](uint64_t l0) {
-        if (full) {
+    auto load = [&](uint64_t l0, auto full_tag, auto set_tag) {
+        constexpr int S = decltype(set_tag)::value;
+        if constexpr (decltype(full_tag)::value) {
             if constexpr (!TN) {
                 const float *pa = a + a_base[0] + l0 * w;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) ra[e] = load_a_vec(pa + e * w);
+                for (int e = 0; e < 4; ++e) ra[S][e] = load_a_vec(pa + e * w);
             } else {
 #pragma unroll
-                for (int e = 0; e < 4; ++e) ra[e] = load_a_vec(a + a_base[e >> 1] + l0 + 4 * (e & 1));
+                for (int e = 0; e < 4; ++e) ra[S][e] = load_a_vec(a + a_base[e >> 1] + l0 + 4 * (e & 1));
             }
         } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) ra[e] = zero4();
+            for (int e = 0; e < 4; ++e) ra[S][e] = zero4();
             if constexpr (!TN) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     uint64_t const l = l0 + mn_k4 + e;
-                    if (a_ok[0] && l < nq) ra[e] = load_a_vec(a + a_base[0] + (l0 + e) * w);
+                    if (a_ok[0] && l < nq) ra[S][e] = load_a_vec(a + a_base[0] + (l0 + e) * w);
                 }
             } else {
 #pragma unroll
@@ -253,11 +258,11 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                     uint64_t const l = l0 + 8 * k_c + 4 * (e & 1);
                     if (a_ok[s] && l < nq) {
                         if (a_vec) {
-                            ra[e] = load_a_vec(a + a_base[s] + l0 + 4 * (e & 1));
+                            ra[S][e] = load_a_vec(a + a_base[s] + l0 + 4 * (e & 1));
                         } else {
 #pragma unroll
                             for (int x = 0; x < 4; ++x)
-                                if (l + x < nq) ra[e][x] = a[a_base[s] + l0 + 4 * (e & 1) + x];
+                                if (l + x < nq) ra[S][e][x] = a[a_base[s] + l0 + 4 * (e & 1) + x];
                         }
                     }
                 }
@@ -269,65 +274,89 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
             for (int p = 0; p < P; ++p)
 #pragma unroll
                 for (int s = 0; s < 2; ++s)
-                    rbs[p][s] = *reinterpret_cast<const i32x4 *>(bs + p * plane + (j0 + k_row + 64 * s) * ldb + l0 +
-                                                                 8 * k_c);
+                    rbs[S][p][s] = *reinterpret_cast<const i32x4 *>(bs + p * plane + (j0 + k_row + 64 * s) * ldb + l0 +
+                                                                    8 * k_c);
         } else {
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                rb[e] = zero4();
+                rb[S][e] = zero4();
                 uint64_t const j = j0 + k_row + 64 * (e >> 1);
                 uint64_t const l = l0 + 8 * k_c + 4 * (e & 1);
                 if (j >= m) continue;
                 if (b_vec && l + 4 <= nq) {
-                    rb[e] = *reinterpret_cast<const f32x4 *>(b + j * nq + l);
+                    rb[S][e] = *reinterpret_cast<const f32x4 *>(b + j * nq + l);
                 } else {
 #pragma unroll
                     for (int x = 0; x < 4; ++x)
-                        if (l + x < nq) rb[e][x] = b[j * nq + l + x];
+                        if (l + x < nq) rb[S][e][x] = b[j * nq + l + x];
                 }
             }
         }
     };
 
-    // Convert and store: MN micro-block (r[e][x] = logical row mn_c4 + x,
-    // k mn_k4 + e) or K rows (r[2s], r[2s+1] = 8 k of physical row k_p + 64s).
-    auto store_mn = [&](f32x4 (&r)[4], op (*s)[BN * LP_LDK]) {
-#pragma unroll
-        for (int x = 0; x < 4; ++x) {
+    // Convert and store, in four pieces: piece t of the A tile is row x = t
+    // of the MN micro-block (r[e][x] = logical row mn_c4 + x, k mn_k4 + e),
+    // or, for K rows, half h = t of physical rows k_p + 64h (r[2h], r[2h+1] =
+    // its 8 k; pieces 2 and 3 empty); the B tile goes with piece 3.
+    auto store_piece = [&](int t, auto set_tag, op (*sa)[BN * LP_LDK], op (*sb)[BN * LP_LDK]) {
+        constexpr int S = decltype(set_tag)::value;
+        if constexpr (!TN) {
+            int const x = t;
             op2 lo[P], hi[P];
-            f32x2 const x01 = {r[0][x], r[1][x]};
-            f32x2 const x23 = {r[2][x], r[3][x]};
+            f32x2 const x01 = {ra[S][0][x], ra[S][1][x]};
+            f32x2 const x23 = {ra[S][2][x], ra[S][3][x]};
             L::split2(x01, lo);
             L::split2(x23, hi);
             int const off = lds_at(lds_row(mn_c4 + x), wave) + 4 * mn_half;
 #pragma unroll
             for (int p = 0; p < P; ++p)
-                *reinterpret_cast<v4 *>(&s[p][off]) = __builtin_shufflevector(lo[p], hi[p], 0, 1, 2, 3);
-        }
-    };
-    auto store_k = [&](f32x4 (&r)[4], op (*s)[BN * LP_LDK]) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
+                *reinterpret_cast<v4 *>(&sa[p][off]) = __builtin_shufflevector(lo[p], hi[p], 0, 1, 2, 3);
+        } else if (t < 2) {
+            int const h = t;
             op2 q[4][P];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                f32x2 const x = {r[2 * h + t / 2][2 * (t % 2)], r[2 * h + t / 2][2 * (t % 2) + 1]};
-                L::split2(x, q[t]);
+            for (int u = 0; u < 4; ++u) {
+                f32x2 const x = {ra[S][2 * h + u / 2][2 * (u % 2)], ra[S][2 * h + u / 2][2 * (u % 2) + 1]};
+                L::split2(x, q[u]);
             }
             int const off = lds_at(k_p + 64 * h, k_c);
 #pragma unroll
             for (int p = 0; p < P; ++p) {
                 v4 const lo = __builtin_shufflevector(q[0][p], q[1][p], 0, 1, 2, 3);
                 v4 const hi = __builtin_shufflevector(q[2][p], q[3][p], 0, 1, 2, 3);
-                *reinterpret_cast<v8 *>(&s[p][off]) = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+                *reinterpret_cast<v8 *>(&sa[p][off]) = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+            }
+        }
+        if (t == 3) {
+            if constexpr (BSPLIT) {
+#pragma unroll
+                for (int p = 0; p < P; ++p)
+#pragma unroll
+                    for (int s = 0; s < 2; ++s)
+                        *reinterpret_cast<i32x4 *>(&sb[p][lds_at(k_p + 64 * s, k_c)]) = rbs[S][p][s];
+            } else {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    op2 q[4][P];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        f32x2 const x = {rb[S][2 * h + u / 2][2 * (u % 2)], rb[S][2 * h + u / 2][2 * (u % 2) + 1]};
+                        L::split2(x, q[u]);
+                    }
+                    int const off = lds_at(k_p + 64 * h, k_c);
+#pragma unroll
+                    for (int p = 0; p < P; ++p) {
+                        v4 const lo = __builtin_shufflevector(q[0][p], q[1][p], 0, 1, 2, 3);
+                        v4 const hi = __builtin_shufflevector(q[2][p], q[3][p], 0, 1, 2, 3);
+                        *reinterpret_cast<v8 *>(&sb[p][off]) = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+                    }
+                }
             }
         }
     };
-    auto store_bs = [&]() {
+    auto store_all = [&](auto set_tag, op (*sa)[BN * LP_LDK], op (*sb)[BN * LP_LDK]) {
 #pragma unroll
-        for (int p = 0; p < P; ++p)
-#pragma unroll
-            for (int s = 0; s < 2; ++s) *reinterpret_cast<i32x4 *>(&sB[p][lds_at(k_p + 64 * s, k_c)]) = rbs[p][s];
+        for (int t = 0; t < 4; ++t) store_piece(t, set_tag, sa, sb);
     };
 
     f32x4 acc[4][4];
@@ -343,26 +372,18 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     int const arow = wn * 64 + lc; // + 16 * nt
     int const brow = wm * 64 + lc; // + 16 * jt
 
-    uint64_t const nslices = (nq + BK - 1) / BK;
-    load(0);
-    for (uint64_t s = 0; s < nslices; ++s) {
-        if constexpr (!TN) store_mn(ra, sA);
-        else store_k(ra, sA);
-        if constexpr (BSPLIT) store_bs();
-        else store_k(rb, sB);
-        __syncthreads();
-        if (s + 1 < nslices) load((s + 1) * BK);
+    // The products of one slice from the LDS tiles (sa, sb).
+    auto multiply = [&](op (*sa)[BN * LP_LDK], op (*sb)[BN * LP_LDK]) {
         v8 fa[P][4];
 #pragma unroll
         for (int p = 0; p < P; ++p)
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt)
-                fa[p][nt] = *reinterpret_cast<const v8 *>(&sA[p][lds_at(arow + 16 * nt, lg)]);
+            for (int nt = 0; nt < 4; ++nt) fa[p][nt] = *reinterpret_cast<const v8 *>(&sa[p][lds_at(arow + 16 * nt, lg)]);
 #pragma unroll
         for (int jt = 0; jt < 4; ++jt) {
             v8 fb[P];
 #pragma unroll
-            for (int p = 0; p < P; ++p) fb[p] = *reinterpret_cast<const v8 *>(&sB[p][lds_at(brow + 16 * jt, lg)]);
+            for (int p = 0; p < P; ++p) fb[p] = *reinterpret_cast<const v8 *>(&sb[p][lds_at(brow + 16 * jt, lg)]);
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
                 if constexpr (P == 3) {
@@ -375,8 +396,23 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                 acc[jt][nt] = L::mfma(fb[0], fa[0][nt], acc[jt][nt]);
             }
         }
-        __syncthreads();
-    }
+    };
+
+    uint64_t const nslices = (nq + BK - 1) / BK;
+    // one LDS buffer: store, barrier, prefetch the next slice, multiply,
+    // barrier (two workgroups per CU overlap each other's phases)
+    auto kloop = [&](auto full_tag) {
+        load(0, full_tag, I0());
+        for (uint64_t s = 0; s < nslices; ++s) {
+            store_all(I0(), sA, sB);
+            __syncthreads();
+            if (s + 1 < nslices) load((s + 1) * BK, full_tag, I0());
+            multiply(sA, sB);
+            __syncthreads();
+        }
+    };
+    if (full) kloop(std::true_type());
+    else kloop(std::false_type());
 
     // ---- epilogue ------------------------------------------------------------
     uint64_t const ncol0 = n0 + wn * 64 + 4 * lc; // this lane's 4 adjacent columns
