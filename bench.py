@@ -41,14 +41,17 @@ METRIC = "TTM TFLOP/s + HBM GB/s vs roofline; BF16x3 vs FP32 at equal FP64-oracl
 # WRITE_SIZE; KiB per dispatch, mean over the dispatches of the case), keyed by
 # kernel symbol and case.  Both counters are scaled by what they read for a
 # plain 1 GiB device copy (`bench.py --copy-calibration`): FETCH_SIZE reads
-# 524,300 KiB for 1,048,576 KiB fetched (scale 2.0), WRITE_SIZE 1,048,576 KiB
+# 524,305 KiB for 1,048,576 KiB fetched (scale 2.0), WRITE_SIZE 1,048,576 KiB
 # for 1,048,576 KiB written (scale 1.0).  Summaries and commands:
 # profiles/ttm_hbm_traffic_pmc.md.  A symbol or case without an entry
 # reports traffic null.
 PMC_FETCH_SCALE = 2.0
 PMC_WRITE_SCALE = 1.0
 PMC_TRAFFIC_KIB = {
-    "k_gemm<double, 4, 4, 2, 16, false, false>": {"f64-first-q2-m1024": (6791506.4, 4197019.4)},
+    "k_gemm<double, 4, 4, 2, 16, false, false>": {"f64-first-q2-m1024": (6269789.0, 4198931.7)},
+    "k_gemm<float, 4, 4, 2, 16, false, false>": {"f32-first-q2-m1024": (3087923.1, 2097152.0)},
+    "k_nn_stream<float, 1, 4, true>": {"f32-first-q2-m16": (1048894.0, 32768.0)},
+    "k_gemm<float, 1, 4, 1, 16, true, false>": {"f32-first-q1-m16": (1399285.3, 32930.8)},
 }
 
 
