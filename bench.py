@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--no-bf16x3", dest="bf16x3", action="store_false",
                     help="skip the BF16X3-vs-FP32 comparison on the compute-bound float32 products")
     ap.add_argument("--tucker-count", type=int, default=256,
-                    help="tensors of the batched multi-TTM (config 5) section (0 = skip)")
+                    help="tensors of the batched uneven Tucker-chain section (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
@@ -232,7 +232,7 @@ def bf16x3_vs_fp32(args, cases, A, C, stream):
 
 
 def tucker_batch(args):
-    """Config 5: 256 first-order float32 tensors with extents drawn from
+    """Tucker batch: 256 first-order float32 tensors with extents drawn from
     U{16..256}, each multiplied in all three modes by factors with orthonormal
     rows and ranks drawn from U{4..64} (QR of a Gaussian matrix); the whole
     batch as one plan, replayed as one hipGraph (FP32 and BF16X3)."""

@@ -129,7 +129,7 @@ def test_double_precision_variants_rejected(ttmlib):
 
 
 def test_plan_reference_configs(ttmlib):
-    """Kernel families the planner picks for the cfg-2 shapes (no GPU needed)."""
+    """Kernel families the planner picks for the A 1024x1024x512 sweep (no GPU needed)."""
     import ttm_amd
     na = [1024, 1024, 512]
     for pia in ([1, 2, 3], [3, 2, 1]):

@@ -140,8 +140,8 @@ def orthonormal_rows(rng, r, n):
 
 
 @pytest.mark.parametrize("prec", [None, "bf16x3"])
-def test_config5_reduced(gpu, prec):
-    """Tucker chains A x1 U1 x2 U2 x3 U3 over uneven first-order tensors (config 5 at reduced size)."""
+def test_tucker_batch_reduced(gpu, prec):
+    """Tucker chains A x1 U1 x2 U2 x3 U3 over 32 uneven first-order tensors (extents 16..64, ranks 4..16)."""
     import torch
     from ttm_amd.chain import ChainPlan
     rng = np.random.default_rng(0x77140005)
@@ -158,5 +158,5 @@ def test_config5_reduced(gpu, prec):
         refs.append((A, Us, einsum_chain(A.astype(np.float32), [U.astype(np.float32) for U in Us])))
     plan = ChainPlan(items, precision=prec)
     plan.run_graph()
-    check(items, refs, torch.float32, f"cfg5 {prec}", prec=prec)
+    check(items, refs, torch.float32, f"tucker {prec}", prec=prec)
     plan.close()

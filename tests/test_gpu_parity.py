@@ -105,7 +105,7 @@ def test_closed_form_each_family(gpu, family):
 
 # ----------------------------------------------------------------------------- random data
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_config0_vs_oracle(gpu, oracle, dtype):
+def test_64x48x32_q2_vs_oracle(gpu, oracle, dtype):
     """A 64x48x32 first-order, q = 2, B 40x48 (column- and row-major), uniform[-1,1]."""
     import torch
     for colmajor in (True, False):
@@ -124,7 +124,7 @@ def test_config0_vs_oracle(gpu, oracle, dtype):
             bound = 4 * 48 * rc.eps(dtype)
             e_ref = rc.normwise_error(c, c_ref, a, b)
             e_64 = rc.normwise_error(c, c64, a, b)
-            _say(f"cfg0 {np.dtype(dtype).name} colmajor={colmajor} kernel={fam}: vs ref {e_ref:.2e} "
+            _say(f"64x48x32 {np.dtype(dtype).name} colmajor={colmajor} kernel={fam}: vs ref {e_ref:.2e} "
                  f"vs fp64 {e_64:.2e} bound {bound:.2e}")
             assert e_ref <= bound and e_64 <= bound
 
@@ -163,7 +163,7 @@ def test_random_shapes_all_families_vs_fp64(gpu, oracle, dtype):
                     assert e <= bound, (na, pia, q, m, colmajor, fam, offset, e)
 
 
-def test_order5_config3_shapes_all_modes(gpu, oracle):
+def test_order5_random_layout_all_modes(gpu, oracle):
     """A 32x24x40x16x48 float32, seeded random layout, q = 1..5, B 64 x n_q: sampled entries
     against the float64 definition (the full oracle is too slow at this size)."""
     import torch
@@ -183,6 +183,27 @@ def test_order5_config3_shapes_all_modes(gpu, oracle):
         ttm_amd.ttm_raw(q, 5, A, na, wa, pia, Bt, [m, na[q - 1]], [2, 1], C, nc, wc)
         torch.cuda.synchronize()
         _check_sampled(torch, A, na, wa, Bt, C, nc, wc, q, np.float32, rng, f"order5 pi={pia} q={q}")
+        # the same product in BF16X3: normwise-relative error of sampled entries <= 1e-6
+        ttm_amd.ttm_raw(q, 5, A, na, wa, pia, Bt, [m, na[q - 1]], [2, 1], C, nc, wc, precision="bf16x3")
+        torch.cuda.synchronize()
+        rel = _sampled_normwise(torch, A, na, wa, Bt, C, nc, wc, q, rng)
+        _say(f"order5 pi={pia} q={q} bf16x3: sampled normwise-rel {rel:.3e}")
+        assert rel <= 1e-6, (pia, q, rel)
+
+
+def _sampled_normwise(torch, A, na, wa, Bt, C, nc, wc, q, rng, samples=4096):
+    """Normwise relative error of sampled entries of C against the float64 definition."""
+    p = len(na)
+    idx = [rng.integers(0, nc[r], size=samples) for r in range(p)]
+    offc = sum(idx[r] * wc[r] for r in range(p))
+    base_a = sum(idx[r] * wa[r] for r in range(p) if r != q - 1)
+    lq = np.arange(na[q - 1])
+    fib_idx = torch.from_numpy((base_a[:, None] + lq[None, :] * wa[q - 1]).astype(np.int64)).to(A.device)
+    fib = A[fib_idx].double().cpu().numpy()
+    brow = Bt.double().cpu().numpy()[idx[q - 1]]
+    expect = np.sum(fib * brow, axis=1)
+    got = C[torch.from_numpy(offc.astype(np.int64)).to(C.device)].double().cpu().numpy()
+    return float(np.linalg.norm(got - expect) / np.linalg.norm(expect))
 
 
 def _check_sampled(torch, A, na, wa, Bt, C, nc, wc, q, dtype, rng, label, samples=4096):
@@ -205,7 +226,7 @@ def _check_sampled(torch, A, na, wa, Bt, C, nc, wc, q, dtype, rng, label, sample
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_config2_full_size_sampled(gpu, dtype):
+def test_1024x1024x512_sweep_sampled(gpu, dtype):
     """A 1024x1024x512, first- and last-order, q = 1..3, m in {16, 1024} (the bench workload)."""
     import torch
     import ttm_amd
@@ -222,19 +243,19 @@ def test_config2_full_size_sampled(gpu, dtype):
                 nc[q - 1] = m
                 wc = rc.dense_strides(nc, pia)
                 C = torch.empty(int(np.prod(nc)), dtype=tdt, device=gpu)
-                _say(f"cfg2 {np.dtype(dtype).name} pi={pia} q={q} m={m} kernel="
+                _say(f"1024x1024x512 {np.dtype(dtype).name} pi={pia} q={q} m={m} kernel="
                      f"{ttm_amd.plan_kernel(q, na, pia, [m, na[q - 1]], [2, 1], is_double=dtype == np.float64)}")
                 ttm_amd.ttm_raw(q, 3, A, na, wa, pia, Bt, [m, na[q - 1]], [2, 1], C, nc, wc)
                 torch.cuda.synchronize()
                 _check_sampled(torch, A, na, wa, Bt, C, nc, wc, q, dtype, rng,
-                               f"cfg2 {np.dtype(dtype).name} pi={pia} q={q} m={m}", samples=1024)
+                               f"1024x1024x512 {np.dtype(dtype).name} pi={pia} q={q} m={m}", samples=1024)
                 del C, Bt
     del A
     torch.cuda.empty_cache()
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_config2_stream_kernels_sampled(gpu, dtype):
+def test_1024x1024x512_stream_kernels_sampled(gpu, dtype):
     """The streaming kernels forced on the bench extents (A 1024x1024x512, first- and
     last-order, q = 1..3): m = 16 with a row-major B and m = 40 with a column-major B."""
     import torch

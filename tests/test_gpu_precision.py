@@ -5,7 +5,7 @@ Each variant is checked twice:
   which it must match up to FP32 accumulation: |dC| <= 4 n_q 2^-24 sum|b||a|
   elementwise over the rounded operands;
 * against the FP64 oracle (brute force of the definition), reporting max and
-  normwise relative errors (the error table of the config-3 study), with
+  normwise relative errors (the error table on A 64x48x32), with
   BF16X3 held to the 1e-6 normwise target.
 """
 import sys
@@ -113,8 +113,8 @@ def test_variant_matches_its_emulation(gpu, prec):
                     check_vs_emulation(c3, a, na, pia, q, Bm, prec, f"{prec} {na} {pia} q={q} m={m} {dist} {kernel}")
 
 
-def test_config3_error_table(gpu, oracle):
-    """cfg-1 shapes (A 64x48x32, q=2, B 40x48), float32 inputs, U[-1,1] and log-uniform."""
+def test_precision_error_table_64x48x32(gpu, oracle):
+    """A 64x48x32 first-order, q=2, B 40x48, float32 inputs, U[-1,1] and log-uniform."""
     import torch
     na, pia, q, m = [64, 48, 32], [1, 2, 3], 2, 40
     wa = rc.dense_strides(na, pia)
@@ -133,7 +133,7 @@ def test_config3_error_table(gpu, oracle):
             c = run(torch, gpu, q, a, na, wa, pia, b, [m, 48], [1, 2], nc, wc, prec)
             mx, nw = errors_vs_fp64(c, c64)
             table[(dist, prec)] = (mx, nw)
-            _say(f"config-3 {dist:10s} {prec:8s} max-rel {mx:.3e}  normwise-rel {nw:.3e}")
+            _say(f"64x48x32 {dist:10s} {prec:8s} max-rel {mx:.3e}  normwise-rel {nw:.3e}")
     for dist in ("uniform", "loguniform"):
         fp32 = table[(dist, "default")][1]
         assert table[(dist, "bf16x3")][1] <= 1e-6, table[(dist, "bf16x3")]
@@ -160,7 +160,7 @@ def test_bf16x3_integer_data_is_exact(gpu):
 
 
 def test_bf16x3_full_size_sampled(gpu):
-    """Compute-bound cfg-2 shapes in BF16X3: sampled entries against the float64 definition."""
+    """Compute-bound A 1024x1024x512 products (m = 1024) in BF16X3: sampled entries against the float64 definition."""
     import torch
     import ttm_amd
     na = [1024, 1024, 512]
@@ -186,7 +186,7 @@ def test_bf16x3_full_size_sampled(gpu):
         expect = np.sum(fib * brow, axis=1)
         got = C[torch.from_numpy(offc.astype(np.int64)).to(gpu)].double().cpu().numpy()
         rel = np.linalg.norm(got - expect) / np.linalg.norm(expect)
-        _say(f"bf16x3 cfg2 pi={pia} q={q}: sampled normwise-rel {rel:.3e}")
+        _say(f"bf16x3 1024x1024x512 pi={pia} q={q}: sampled normwise-rel {rel:.3e}")
         assert rel <= 1e-6
         del C
     torch.cuda.empty_cache()

@@ -125,7 +125,7 @@ GroupConfig default_config(uint64_t max_m, bool bf16x3)
 
 bool config_allowed(const GroupConfig &cfg, bool is_double)
 {
-    (void)is_double;
+    if (is_double && cfg.bk > 16) return false; // slices of 32 are compiled for float32 only
     return cfg.nt * 16 * (4 / cfg.waves_m) >= 64;
 }
 

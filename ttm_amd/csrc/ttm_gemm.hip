@@ -144,9 +144,13 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     bool const stream = (flags & TILE_STREAM) != 0;
     // 16-byte loads of A: non-temporal when A is streamed once (measured on
     // MI355X: ~7.1 TB/s for non-temporal against ~6.2 TB/s for ordinary
-    // 16-byte read streams)
+    // 16-byte read streams) -- except in the TN/TT views, where a slice reads
+    // BK * sizeof(T) bytes of each row and consecutive slices share 128-byte
+    // lines: a non-temporal line is gone before its second half is read
+    // (FETCH_SIZE 1.33x the bytes of A).
     auto load_a = [&](const T *p) -> V {
-        return stream ? __builtin_nontemporal_load(reinterpret_cast<const V *>(p)) : *reinterpret_cast<const V *>(p);
+        return (stream && !TN) ? __builtin_nontemporal_load(reinterpret_cast<const V *>(p))
+                               : *reinterpret_cast<const V *>(p);
     };
 
     int const tid = threadIdx.x;
@@ -469,7 +473,10 @@ struct Config
 // Small m streams A once with wide column blocks; large m uses square blocks.
 Config pick_config(const Fold &f, bool is_double)
 {
-    (void)is_double;
+    // float32 rows read in the TN view: slices of 32 (128 bytes, one whole
+    // L2 line of each row per slice; with 16, the two halves of a line are
+    // read one slice apart and a third of the lines is fetched twice)
+    if (f.m <= 16 && f.w == 1 && !is_double) return {1, 4, 1, 32};
     if (f.m <= 16) return {1, 4, 1, 16};
     if (f.m <= 32) return {2, 4, 1, 16};
     if (f.m <= 64) return {4, 4, 1, 16};
@@ -492,6 +499,12 @@ void launch_cfg(const Fold &f, const T *a, const T *b, T *c, hipStream_t s)
 template <class T, bool TN, bool BCOL>
 void launch_layout(const Fold &f, const T *a, const T *b, T *c, hipStream_t s, Config cfg)
 {
+    if constexpr (sizeof(T) == 4 && TN) {
+        if (cfg.jt == 1 && cfg.bk == 32) {
+            launch_cfg<T, 1, 4, 1, 32, TN, BCOL>(f, a, b, c, s);
+            return;
+        }
+    }
     if (cfg.jt == 1) launch_cfg<T, 1, 4, 1, 16, TN, BCOL>(f, a, b, c, s);
     else if (cfg.jt == 2) launch_cfg<T, 2, 4, 1, 16, TN, BCOL>(f, a, b, c, s);
     else if (cfg.waves_m == 1) launch_cfg<T, 4, 4, 1, 16, TN, BCOL>(f, a, b, c, s);
@@ -553,8 +566,11 @@ namespace
 {
 // (jt, nt, waves_m, bk): rows 16/32/64 with 256 columns, 128 x 128, and 64 or 32
 // rows with 128 columns for short problems.
+// The last three (slices of 32) are float32 only: rows read in the TT view
+// then cover whole 128-byte lines per slice.
 constexpr GroupConfig k_group_configs[] = {
-    {1, 4, 1, 16}, {2, 4, 1, 16}, {4, 4, 1, 16}, {4, 4, 2, 16}, {2, 4, 2, 16}, {1, 4, 2, 16},
+    {1, 4, 1, 16}, {2, 4, 1, 16}, {4, 4, 1, 16}, {4, 4, 2, 16}, {2, 4, 2, 16},
+    {1, 4, 2, 16}, {1, 4, 1, 32}, {2, 4, 1, 32}, {4, 4, 1, 32},
 };
 constexpr int k_num_group_configs = sizeof(k_group_configs) / sizeof(k_group_configs[0]);
 
@@ -574,8 +590,17 @@ const void *grouped_kernel(int index)
     case 3: return grouped_fn<T, 4, 4, 2, 16>();
     case 4: return grouped_fn<T, 2, 4, 2, 16>();
     case 5: return grouped_fn<T, 1, 4, 2, 16>();
-    default: return nullptr;
+    default: break;
     }
+    if constexpr (sizeof(T) == 4) {
+        switch (index) {
+        case 6: return grouped_fn<T, 1, 4, 1, 32>();
+        case 7: return grouped_fn<T, 2, 4, 1, 32>();
+        case 8: return grouped_fn<T, 4, 4, 1, 32>();
+        default: break;
+        }
+    }
+    return nullptr;
 }
 
 int config_index(const GroupConfig &cfg)
@@ -605,6 +630,7 @@ int group_occupancy(const GroupConfig &cfg, bool is_double)
     int const idx = config_index(cfg);
     if (idx < 0) return 0;
     const void *fn = is_double ? grouped_kernel<double>(idx) : grouped_kernel<float>(idx);
+    if (!fn) return 0;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, 256, 0) != hipSuccess) {
         (void)hipGetLastError();
@@ -629,7 +655,16 @@ ttm_status_t launch_gemm_grouped(const GroupConfig &cfg, const GroupProblem *d_p
     case 3: hipLaunchKernelGGL((k_gemm_grouped<T, 4, 4, 2, 16>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total); break;
     case 4: hipLaunchKernelGGL((k_gemm_grouped<T, 2, 4, 2, 16>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total); break;
     case 5: hipLaunchKernelGGL((k_gemm_grouped<T, 1, 4, 2, 16>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total); break;
-    default: break;
+    default:
+        if constexpr (sizeof(T) == 4) {
+            if (idx == 6) hipLaunchKernelGGL((k_gemm_grouped<T, 1, 4, 1, 32>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
+            else if (idx == 7) hipLaunchKernelGGL((k_gemm_grouped<T, 2, 4, 1, 32>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
+            else if (idx == 8) hipLaunchKernelGGL((k_gemm_grouped<T, 4, 4, 1, 32>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
+        } else {
+            set_error(TTM_ERR_ARGUMENT, "grouped tile configuration not compiled for float64");
+            return TTM_ERR_ARGUMENT;
+        }
+        break;
     }
     hipError_t const e = hipGetLastError();
     if (e != hipSuccess) {
