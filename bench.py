@@ -51,7 +51,6 @@ PMC_TRAFFIC_KIB = {
     "k_gemm<double, 4, 4, 2, 16, false, false>": {"f64-first-q2-m1024": (6269789.0, 4198931.7)},
     "k_gemm<float, 4, 4, 2, 16, false, false>": {"f32-first-q2-m1024": (3087923.1, 2097152.0)},
     "k_nn_stream<float, 1, 4, true>": {"f32-first-q2-m16": (1048894.0, 32768.0)},
-    "k_gemm<float, 1, 4, 1, 16, true, false>": {"f32-first-q1-m16": (1399285.3, 32930.8)},
 }
 
 
@@ -69,6 +68,8 @@ def parse():
                     help="skip the BF16X3-vs-FP32 comparison on the compute-bound float32 products")
     ap.add_argument("--tucker-count", type=int, default=256,
                     help="tensors of the batched uneven Tucker-chain section (0 = skip)")
+    ap.add_argument("--no-host-path", dest="host_path", action="store_false",
+                    help="skip the host-pointer (PCIe-inclusive) drop-in measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
@@ -291,6 +292,31 @@ def tucker_batch(args):
     return out
 
 
+def host_path():
+    """The drop-in's synchronous host-pointer entry on pageable NumPy buffers
+    (the call stages A and B to the device and C back): one float32 product,
+    A 1024x1024x512 first-order, q = 2, m = 16, timed with the host clock
+    after one warm-up call.  Reported beside `value`, never as it."""
+    import ttm_amd
+    na, pia, q, m = list(EXTENTS), [1, 2, 3], 2, 16
+    rng = np.random.default_rng(5)
+    a = rng.random(int(np.prod(na)), dtype=np.float32)
+    b = rng.random(m * na[q - 1], dtype=np.float32)
+    nc = list(na)
+    nc[q - 1] = m
+    c = np.empty(int(np.prod(nc)), dtype=np.float32)
+    wa, wc = ttm_amd.dense_strides(na, pia), ttm_amd.dense_strides(nc, pia)
+    call = lambda: ttm_amd.ttm_raw(q, 3, a, na, wa, pia, b, [m, na[q - 1]], [2, 1], c, nc, wc)
+    call()
+    t0 = time.perf_counter()
+    call()
+    dt = time.perf_counter() - t0
+    _, bytes_ = work(na, q, m, 4)
+    return {"case": "f32-first-q2-m16, host (pageable) buffers", "ms": round(dt * 1e3, 2),
+            "gbps_algorithmic": round(bytes_ / dt / 1e9, 1),
+            "note": "includes host-to-device staging of A and B and the copy of C back over PCIe"}
+
+
 def cpu_baseline(args, budget_s):
     """The oracle's `combined` restatement over OpenMP + a CBLAS, on the
     same 24-case sweep with every extent divided by 4 (A 256 x 256 x 128)."""
@@ -406,6 +432,7 @@ def main():
         roofline["traffic_over_algorithmic"] = round(traffic / traffic_alg, 3)
 
     tucker = tucker_batch(args) if args.tucker_count > 0 else None
+    host = host_path() if args.host_path else None
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -434,6 +461,7 @@ def main():
         "cpu_baseline": cpu,
         "bf16x3_vs_fp32": split,
         "tucker_batch": tucker,
+        "host_path": host,
         "cases": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in c.items()
                    if k in ("label", "symbol", "ms", "ms_min", "tflops", "gbps", "bound", "frac")}
                   for c in cases],
