@@ -37,6 +37,12 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+// The same 16-byte vectors at element alignment: rows of A or B whose length
+// is not a multiple of the vector width start at element-aligned addresses;
+// dword-aligned multi-dword global loads are a supported access (one
+// global_load_dwordx4).
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
 
 template <class T>
 struct G;
@@ -45,6 +51,7 @@ struct G<float>
 {
     static constexpr int VEC = 4;
     using vec = f32x4;
+    using vec_u = f32x4u;
     using acc = f32x4;
     __device__ static constexpr int acc_row(int lr, int r) { return 4 * lr + r; }
     __device__ static f32x4 mfma(float a, float b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
@@ -54,6 +61,7 @@ struct G<double>
 {
     static constexpr int VEC = 2;
     using vec = f64x2;
+    using vec_u = f64x2u;
     using acc = f64x4;
     __device__ static constexpr int acc_row(int lr, int r) { return 4 * r + lr; }
     __device__ static f64x4 mfma(double a, double b, f64x4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
@@ -233,7 +241,12 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
                     if (a_vec) {
                         // nq % VEC == 0: a vector is wholly in or out of range
                         ra[e] = load_a(a + a_base[e] + l);
+                    } else if (l + VEC <= nq) {
+                        // element-aligned vector inside the row
+                        V const v = *reinterpret_cast<const typename Gt::vec_u *>(a + a_base[e] + l);
+                        ra[e] = v;
                     } else {
+                        // the row's last, partial vector
 #pragma unroll
                         for (int x = 0; x < VEC; ++x)
                             if (l + x < nq) ra[e][x] = a[a_base[e] + l + x];
@@ -249,8 +262,10 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
                 uint64_t const l = l0 + b_row[e];
                 uint64_t const j = j0 + b_col[e];
                 if (l < nq) {
-                    if (b_vec_ok && j + VEC <= m) {
-                        rb[e] = *reinterpret_cast<const V *>(b + j + l * m);
+                    if (j + VEC <= m) {
+                        // element-aligned vector inside column l (16-byte aligned when m % VEC == 0)
+                        V const v = *reinterpret_cast<const typename Gt::vec_u *>(b + j + l * m);
+                        rb[e] = v;
                     } else {
 #pragma unroll
                         for (int x = 0; x < VEC; ++x)
@@ -261,8 +276,10 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
                 uint64_t const j = j0 + b_col[e];
                 uint64_t const l = l0 + b_row[e];
                 if (j < m) {
-                    if (b_vec_ok && l + VEC <= nq) {
-                        rb[e] = *reinterpret_cast<const V *>(b + j * nq + l);
+                    if (l + VEC <= nq) {
+                        // element-aligned vector inside row j (16-byte aligned when nq % VEC == 0)
+                        V const v = *reinterpret_cast<const typename Gt::vec_u *>(b + j * nq + l);
+                        rb[e] = v;
                     } else {
 #pragma unroll
                         for (int x = 0; x < VEC; ++x)

@@ -42,6 +42,9 @@ namespace
 
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+// 16-byte vector at float alignment (rows whose length is not a multiple of 4;
+// dword-aligned multi-dword global loads are a supported access)
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -219,7 +222,6 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     }
     // every column of the tile in range and whole slices: unguarded 16-byte loads
     bool const full = n0 + BN <= ncols && nq % BK == 0 && (!TN || a_vec);
-    bool const b_vec = nq % 4 == 0;
 
     // Registers of the next slice's global loads (one set; the index keeps
     // the loaders generic over register sets).
@@ -259,6 +261,9 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                     if (a_ok[s] && l < nq) {
                         if (a_vec) {
                             ra[S][e] = load_a_vec(a + a_base[s] + l0 + 4 * (e & 1));
+                        } else if (l + 4 <= nq) {
+                            f32x4 const v = *reinterpret_cast<const f32x4u *>(a + a_base[s] + l0 + 4 * (e & 1));
+                            ra[S][e] = v;
                         } else {
 #pragma unroll
                             for (int x = 0; x < 4; ++x)
@@ -283,8 +288,9 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                 uint64_t const j = j0 + k_row + 64 * (e >> 1);
                 uint64_t const l = l0 + 8 * k_c + 4 * (e & 1);
                 if (j >= m) continue;
-                if (b_vec && l + 4 <= nq) {
-                    rb[S][e] = *reinterpret_cast<const f32x4 *>(b + j * nq + l);
+                if (l + 4 <= nq) {
+                    f32x4 const v = *reinterpret_cast<const f32x4u *>(b + j * nq + l); // 16-byte aligned when nq % 4 == 0
+                    rb[S][e] = v;
                 } else {
 #pragma unroll
                     for (int x = 0; x < 4; ++x)
