@@ -165,6 +165,21 @@ unsigned ttm_chain_stages(ttm_chain_plan_t plan);
 const char *ttm_chain_stage_kernel(ttm_chain_plan_t plan, unsigned s);
 void ttm_chain_plan_destroy(ttm_chain_plan_t plan);
 
+/* --- device memory for callers without a HIP toolchain ------------------ */
+/* No reference counterpart (the reference's tensor is host memory only,
+ * include/tlib/detail/tensor.h:59-121): these back the device-resident
+ * tlib::ttm::device_tensor of include/tlib/device.h, so that C++ callers keep
+ * tensors in HBM across calls without including HIP headers.  All are
+ * synchronous on the current device. */
+ttm_status_t ttm_device_alloc(size_t bytes, void **ptr);   /* *ptr = NULL when bytes == 0 */
+ttm_status_t ttm_device_free(void *ptr);                   /* NULL is a no-op */
+/* Copy `bytes` between any host / device pointers (direction from the
+ * pointers themselves). */
+ttm_status_t ttm_device_copy(void *dst, const void *src, size_t bytes);
+ttm_status_t ttm_device_zero(void *ptr, size_t bytes);
+/* Wait for all work submitted to the current device. */
+ttm_status_t ttm_device_synchronize(void);
+
 /* --- diagnostics -------------------------------------------------------- */
 /* Message of the last failing call on this thread ("" after a success). */
 const char *ttm_last_error(void);

@@ -229,6 +229,10 @@ def test_dropin_header_program_compiles(ttmlib):
     _compile_cpp(ROOT / "tests" / "cpp" / "interfaces.cpp", BUILD / "interfaces")
 
 
+def test_device_tensor_program_compiles(ttmlib):
+    _compile_cpp(ROOT / "tests" / "cpp" / "device_tensor.cpp", BUILD / "device_tensor")
+
+
 @pytest.mark.skipif(not REF_EXAMPLES.is_dir(), reason="reference examples not available")
 @pytest.mark.parametrize("example", ["interface1.cpp", "interface2.cpp", "interface3.cpp"])
 def test_reference_examples_compile_unchanged(ttmlib, example):

@@ -366,16 +366,18 @@ def test_host_entry_points_with_host_buffers(gpu, oracle):
         assert rc.normwise_error(c, c64, a, b) <= 4 * na[q - 1] * rc.eps(dtype)
 
 
-def test_cpp_dropin_program_runs(gpu):
-    """tests/cpp/interfaces.cpp: the three call forms + error texts, on the GPU."""
+@pytest.mark.parametrize("program", ["interfaces", "device_tensor"])
+def test_cpp_dropin_program_runs(gpu, program):
+    """tests/cpp/interfaces.cpp: the three call forms + error texts, on the GPU.
+    tests/cpp/device_tensor.cpp: HBM-resident tensors give the host-tensor results bit for bit."""
     import shutil
-    exe = ROOT / "tests" / "_build" / "interfaces"
+    exe = ROOT / "tests" / "_build" / program
     if not exe.exists():
         gxx = shutil.which("g++")
         if not gxx:
             pytest.skip("no g++ to build the program")
         exe.parent.mkdir(exist_ok=True)
-        subprocess.run([gxx, "-O1", "-std=c++17", f"-I{ROOT / 'include'}", str(ROOT / "tests" / "cpp" / "interfaces.cpp"),
+        subprocess.run([gxx, "-O1", "-std=c++17", f"-I{ROOT / 'include'}", str(ROOT / "tests" / "cpp" / f"{program}.cpp"),
                         "-o", str(exe), f"-L{ROOT / 'ttm_amd' / 'lib'}", "-lttm_amd",
                         f"-Wl,-rpath,{ROOT / 'ttm_amd' / 'lib'}"], check=True)
     res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)

@@ -80,6 +80,11 @@ SIGNATURES = [
      [ctypes.c_uint, ctypes.c_uint, ctypes.c_int, _size_p, _size_p, _size_p, _size_p, ctypes.c_int]),
     ("ttm_version", ctypes.c_char_p, []),
     ("ttm_force_kernel", ctypes.c_int, [ctypes.c_char_p]),
+    ("ttm_device_alloc", ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    ("ttm_device_free", ctypes.c_int, [_vp]),
+    ("ttm_device_copy", ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    ("ttm_device_zero", ctypes.c_int, [_vp, ctypes.c_size_t]),
+    ("ttm_device_synchronize", ctypes.c_int, []),
 ]
 
 _lib = None

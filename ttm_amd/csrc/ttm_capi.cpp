@@ -223,6 +223,73 @@ const char *ttm_plan_kernel_symbol(unsigned q, unsigned p, int is_double, const 
 
 const char *ttm_version(void) { return "ttm_amd 0.1.0 (gfx950)"; }
 
+namespace
+{
+ttm_status_t hip_status(hipError_t e, ttm_status_t on_error)
+{
+    if (e == hipSuccess) {
+        clear_error();
+        return TTM_SUCCESS;
+    }
+    (void)hipGetLastError();
+    set_error(on_error, hipGetErrorString(e));
+    return on_error;
+}
+} // namespace
+
+ttm_status_t ttm_device_alloc(size_t bytes, void **ptr)
+{
+    if (!ptr) {
+        set_error(TTM_ERR_ARGUMENT, "output pointer is null");
+        return TTM_ERR_ARGUMENT;
+    }
+    *ptr = nullptr;
+    if (bytes == 0) {
+        clear_error();
+        return TTM_SUCCESS;
+    }
+    ttm_status_t const st = hip_status(hipMalloc(ptr, bytes), TTM_ERR_ALLOC);
+    if (st != TTM_SUCCESS) *ptr = nullptr;
+    return st;
+}
+
+ttm_status_t ttm_device_free(void *ptr)
+{
+    if (!ptr) {
+        clear_error();
+        return TTM_SUCCESS;
+    }
+    return hip_status(hipFree(ptr), TTM_ERR_HIP);
+}
+
+ttm_status_t ttm_device_copy(void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) {
+        clear_error();
+        return TTM_SUCCESS;
+    }
+    if (!dst || !src) {
+        set_error(TTM_ERR_ARGUMENT, "copy with a null pointer");
+        return TTM_ERR_ARGUMENT;
+    }
+    return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyDefault), TTM_ERR_HIP);
+}
+
+ttm_status_t ttm_device_zero(void *ptr, size_t bytes)
+{
+    if (bytes == 0) {
+        clear_error();
+        return TTM_SUCCESS;
+    }
+    if (!ptr) {
+        set_error(TTM_ERR_ARGUMENT, "zero with a null pointer");
+        return TTM_ERR_ARGUMENT;
+    }
+    return hip_status(hipMemset(ptr, 0, bytes), TTM_ERR_HIP);
+}
+
+ttm_status_t ttm_device_synchronize(void) { return hip_status(hipDeviceSynchronize(), TTM_ERR_HIP); }
+
 // Testing hook (not part of the public header's contract): force a kernel
 // family by name ("" restores the planner's choice).  Returns 0 when the
 // name is known.
