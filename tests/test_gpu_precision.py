@@ -101,7 +101,9 @@ def test_variant_matches_its_emulation(gpu, prec):
     rng = np.random.default_rng(11)
     shapes = [([64, 48, 32], [1, 2, 3], 2, 40), ([16, 24, 20], [3, 2, 1], 3, 64), ([40, 12, 8], [1, 2, 3], 1, 33),
               ([8, 36, 10], [2, 1, 3], 2, 130), ([20, 8, 5], [1, 3, 2], 2, 7), ([13, 7, 9], [1, 2, 3], 2, 5),
-              ([256, 4, 6], [3, 1, 2], 1, 17)]
+              ([256, 4, 6], [3, 1, 2], 1, 17),
+              # more than 128 rows of B: several row blocks (ragged rows, columns and slices)
+              ([12, 40, 24], [1, 2, 3], 2, 300), ([36, 20, 16], [1, 2, 3], 1, 260), ([64, 10, 14], [1, 2, 3], 1, 512)]
     for na, pia, q, m in shapes:
         for colmajor in (True, False):
             for dist in ("uniform", "loguniform"):
