@@ -50,7 +50,7 @@ PMC_WRITE_SCALE = 1.0
 PMC_TRAFFIC_KIB = {
     "k_gemm<double, 4, 4, 2, 16, false, false>": {"f64-first-q2-m1024": (6269789.0, 4198931.7)},
     "k_gemm<float, 4, 4, 2, 16, false, false>": {"f32-first-q2-m1024": (3087923.1, 2097152.0)},
-    "k_nn_stream<float, 1, 4, true>": {"f32-first-q2-m16": (1048894.0, 32768.0)},
+    "k_nn_stream<float, 1, 6, true>": {"f32-first-q2-m16": (1048894.0, 32768.0)},
 }
 
 

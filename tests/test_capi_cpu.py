@@ -144,7 +144,7 @@ def test_plan_reference_configs(ttmlib):
                     if m == 16 and qh > 1:
                         # NN view (w % vec == 0), one row block: the streaming kernel
                         assert k == "nn_stream", (pia, q, m, k)
-                        assert sym.startswith("k_nn_stream<" + ("double" if dbl else "float") + ", 1, 4, ")
+                        assert sym.startswith("k_nn_stream<" + ("double" if dbl else "float") + ", 1, 6, ")
                         continue
                     assert k == "gemm", (pia, q, m, k)
                     assert sym.startswith("k_gemm<" + ("double" if dbl else "float"))

@@ -365,8 +365,11 @@ __global__ __launch_bounds__(256) void k_tn_stream(const T *__restrict__ a, cons
 int pick_jt(uint64_t m) { return m <= 16 ? 1 : (m <= 32 ? 2 : 4); }
 
 // Pipeline depth (groups of A loads in flight per wave) of the stream
-// kernels; the register cost of a group grows with JT.
-constexpr int stream_depth(int jt, bool tn) { return jt == 1 ? (tn ? 3 : 4) : (jt == 2 ? (tn ? 2 : 3) : 2); }
+// kernels; the register cost of a group grows with JT.  NN, JT = 1, measured
+// on A 1024x1024x512 at m = 16 (fraction of 8 TB/s, float / double): depth 4
+// 0.69 / 0.66 where q is the slowest mode (rows of A 2-4 MB apart), depth 6
+// 0.76 / 0.72, depth 7 0.77 / 0.59 and depth 8 0.62 / 0.60 (occupancy falls).
+constexpr int stream_depth(int jt, bool tn) { return jt == 1 ? (tn ? 3 : 6) : (jt == 2 ? (tn ? 2 : 3) : 2); }
 
 // 16-byte vectors of B rows: row-major B with nq % VEC == 0.
 bool stream_bvec(const Fold &f, int vec) { return !f.b_colmajor && f.nq % (uint64_t)vec == 0; }
