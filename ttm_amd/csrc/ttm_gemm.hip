@@ -152,13 +152,15 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     bool const stream = (flags & TILE_STREAM) != 0;
     // 16-byte loads of A: non-temporal when A is streamed once (measured on
     // MI355X: ~7.1 TB/s for non-temporal against ~6.2 TB/s for ordinary
-    // 16-byte read streams) -- except in the TN/TT views, where a slice reads
-    // BK * sizeof(T) bytes of each row and consecutive slices share 128-byte
-    // lines: a non-temporal line is gone before its second half is read
-    // (FETCH_SIZE 1.33x the bytes of A).
+    // 16-byte read streams).  In the TN/TT views a slice reads BK * sizeof(T)
+    // bytes of each row: only when that is a whole number of 128-byte lines
+    // and rows start on a line does a slice consume its lines completely;
+    // otherwise consecutive slices share lines and a non-temporal line is gone
+    // before its second half is read (FETCH_SIZE 1.33x the bytes of A).
+    constexpr bool TN_WHOLE_LINES = BK * sizeof(T) % 128 == 0;
+    bool const nt_a = stream && (!TN || (TN_WHOLE_LINES && nq * sizeof(T) % 128 == 0));
     auto load_a = [&](const T *p) -> V {
-        return (stream && !TN) ? __builtin_nontemporal_load(reinterpret_cast<const V *>(p))
-                               : *reinterpret_cast<const V *>(p);
+        return nt_a ? __builtin_nontemporal_load(reinterpret_cast<const V *>(p)) : *reinterpret_cast<const V *>(p);
     };
 
     int const tid = threadIdx.x;
@@ -308,6 +310,14 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
         }
     };
 
+    // TN views store each 16-byte A vector (VEC consecutive l of one column n)
+    // as VEC scalars into rows l of the At tile; column n of row l sits at
+    // n ^ tn_swz(l).  The XOR moves whole groups of NT columns (the lanes'
+    // fragment reads stay one vector read) and spreads the 8 vectors of a row
+    // segment over distinct banks: the float stores of every compiled TN shape
+    // are conflict-free (32- and 64-bank models; 4-way for BK = 32 without it),
+    // the double stores 2-way as before, the fragment reads unchanged.
+    auto tn_swz = [](int l) { return NT * (((l >> 3) * 3) & 3); };
     auto store_tiles = [&](int buf) {
 #pragma unroll
         for (int e = 0; e < A_PER; ++e) {
@@ -316,7 +326,8 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
                 *reinterpret_cast<V *>(&sA[buf][a_row[e] * LDA + a_col[e]]) = ra[e];
             } else {
 #pragma unroll
-                for (int x = 0; x < VEC; ++x) sA[buf][(a_row[e] + x) * LDA + a_col[e]] = ra[e][x];
+                for (int x = 0; x < VEC; ++x)
+                    sA[buf][(a_row[e] + x) * LDA + (a_col[e] ^ tn_swz(a_row[e] + x))] = ra[e][x];
             }
         }
 #pragma unroll
@@ -359,7 +370,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
                 int const kr = kk * 4 + lr;
                 T fa[JT], fb[NT];
                 lds_read<T, JT>(&sB[buf][kr * LDB + b_off], fa);
-                lds_read<T, NT>(&sA[buf][kr * LDA + a_off], fb);
+                lds_read<T, NT>(&sA[buf][kr * LDA + (TN ? (a_off ^ tn_swz(kr)) : a_off)], fb);
 #pragma unroll
                 for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
@@ -494,6 +505,9 @@ Config pick_config(const Fold &f, bool is_double)
     // L2 line of each row per slice; with 16, the two halves of a line are
     // read one slice apart and a third of the lines is fetched twice)
     if (f.m <= 16 && f.w == 1 && !is_double) return {1, 4, 1, 32};
+    // double rows in the TN view: 128-column blocks (half the LDS, twice the
+    // workgroups per CU): 0.64-0.68 -> 0.67-0.69 of 8 TB/s at m = 16
+    if (f.m <= 16 && f.w == 1) return {1, 2, 1, 16};
     if (f.m <= 16) return {1, 4, 1, 16};
     if (f.m <= 32) return {2, 4, 1, 16};
     if (f.m <= 64) return {4, 4, 1, 16};
@@ -519,6 +533,12 @@ void launch_layout(const Fold &f, const T *a, const T *b, T *c, hipStream_t s, C
     if constexpr (sizeof(T) == 4 && TN) {
         if (cfg.jt == 1 && cfg.bk == 32) {
             launch_cfg<T, 1, 4, 1, 32, TN, BCOL>(f, a, b, c, s);
+            return;
+        }
+    }
+    if constexpr (sizeof(T) == 8 && TN) {
+        if (cfg.nt == 2) {
+            launch_cfg<T, 1, 2, 1, 16, TN, BCOL>(f, a, b, c, s);
             return;
         }
     }
