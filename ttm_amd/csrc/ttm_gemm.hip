@@ -94,6 +94,50 @@ __device__ __forceinline__ void lds_read(const T *p, T (&out)[N])
     }
 }
 
+// Global memory accesses of the tiles.  The grouped kernels take A, B and C
+// from problem descriptors in memory, so the compiler cannot tell that they
+// point to global memory and would emit FLAT instructions, which also count
+// against the LDS counter: every LDS wait before a barrier then waits for
+// all loads in flight.  The explicit address space keeps them global loads.
+template <class V>
+__device__ __forceinline__ V ldg(const void *p)
+{
+    return *(const __attribute__((address_space(1))) V *)p;
+}
+template <class V>
+__device__ __forceinline__ V ldg_nt(const void *p)
+{
+    return __builtin_nontemporal_load((const __attribute__((address_space(1))) V *)p);
+}
+template <class V>
+__device__ __forceinline__ void stg(void *p, V v)
+{
+    *(__attribute__((address_space(1))) V *)p = v;
+}
+template <class V>
+__device__ __forceinline__ void stg_nt(void *p, V v)
+{
+    __builtin_nontemporal_store(v, (__attribute__((address_space(1))) V *)p);
+}
+
+// Elements d .. d + VEC - 1 of v moved to 0 .. VEC - 1, the first nvalid
+// kept and the rest zeroed (element x + d < VEC whenever x < nvalid).
+// Selects only: no branch, no partial register write.
+template <class V>
+__device__ __forceinline__ V shift_mask(V v, int d, int nvalid)
+{
+    constexpr int N = (int)(sizeof(V) / sizeof(v[0]));
+    V out;
+#pragma unroll
+    for (int x = 0; x < N; ++x) {
+        auto r = v[x];
+#pragma unroll
+        for (int y = 1; x + y < N; ++y) r = d == y ? v[x + y] : r;
+        out[x] = x < nvalid ? r : decltype(r)(0);
+    }
+    return out;
+}
+
 // Workgroup id -> (row block, column block).  Consecutive workgroup ids are
 // dispatched round-robin over the 8 XCDs; regroup them so that the row
 // blocks sharing one column block run on the same XCD (shared L2 for that
@@ -160,7 +204,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     constexpr bool TN_WHOLE_LINES = BK * sizeof(T) % 128 == 0;
     bool const nt_a = stream && (!TN || (TN_WHOLE_LINES && nq * sizeof(T) % 128 == 0));
     auto load_a = [&](const T *p) -> V {
-        return nt_a ? __builtin_nontemporal_load(reinterpret_cast<const V *>(p)) : *reinterpret_cast<const V *>(p);
+        return nt_a ? ldg_nt<V>(p) : ldg<V>(p);
     };
 
     int const tid = threadIdx.x;
@@ -220,7 +264,17 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     }
     bool const b_vec_ok = BCOL ? (m % VEC == 0) : (nq % VEC == 0);
 
-    V ra[A_PER], rb[B_PER];
+    // Register sets of slices in flight: two for tiles of at most 32 rows per
+    // wave (little matrix work per slice to hide a load behind: the loads of
+    // slice s + 2 are issued while slice s is multiplied) whose slice is at
+    // most five vectors per thread, or the 16-row TN tile of one product; one
+    // otherwise (larger sets would spill in the grouped kernels).
+    constexpr int PF = (JT <= 2 && (A_PER + B_PER <= 5 || (VIEW == VIEW_TN && JT == 1))) ? 2 : 1;
+    V ra[PF][A_PER], rb[PF][B_PER];
+    // Guarded slices: shift d and count nvalid of each vector (d | nvalid << 8),
+    // applied when the slice is stored so that no load is waited for early.
+    int ra_dm[PF][A_PER], rb_dm[PF][B_PER];
+    constexpr int DM_WHOLE = VEC << 8; // no shift, all VEC elements valid
 
     // Interior tile (every row and column in range, whole slices, 16-byte
     // vectors throughout): unguarded loads, all in flight together.
@@ -230,35 +284,52 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     for (int e = 0; e < A_PER; ++e) a_fix[e] = TN ? a_base[e] + a_row[e] : a_base[e] + (uint64_t)a_row[e] * w;
     uint64_t const a_step = TN ? 1 : w; // A offset per contraction index
 
-    auto load_tiles = [&](uint64_t l0) {
+    auto load_tiles = [&](uint64_t l0, auto set) {
+        constexpr int S = decltype(set)::value;
 #pragma unroll
         for (int e = 0; e < A_PER; ++e) {
-            ra[e] = vzero<V>();
+            ra[S][e] = vzero<V>();
+            ra_dm[S][e] = DM_WHOLE;
             if constexpr (!TN) {
                 uint64_t const l = l0 + a_row[e];
-                if (a_ok[e] && l < nq) ra[e] = load_a(a + a_base[e] + l * w);
+                if (a_ok[e] && l < nq) ra[S][e] = load_a(a + a_base[e] + l * w);
+            } else if (nq >= (uint64_t)VEC) {
+                // One in-bounds vector per thread and no divergent branch (a
+                // per-element fill of a vector register whose load is still in
+                // flight makes the compiler wait for every load): a vector
+                // that would run past the row end is read as the row's last
+                // VEC elements and shifted into place; elements past the row
+                // end, and rows past the last column (read as row 0), are zero.
+                uint64_t const l = l0 + a_row[e];
+                uint64_t const last = nq - VEC;
+                uint64_t const lc = l < last ? l : last;
+                V const v = ldg<typename Gt::vec_u>(a + a_base[e] + lc);
+                int const nvalid = (!a_ok[e] || l >= nq) ? 0 : (int)(nq - l < (uint64_t)VEC ? nq - l : VEC);
+                ra[S][e] = v;
+                ra_dm[S][e] = (int)(l - lc) | nvalid << 8;
             } else {
                 uint64_t const l = l0 + a_row[e];
                 if (a_ok[e] && l < nq) {
                     if (a_vec) {
                         // nq % VEC == 0: a vector is wholly in or out of range
-                        ra[e] = load_a(a + a_base[e] + l);
+                        ra[S][e] = load_a(a + a_base[e] + l);
                     } else if (l + VEC <= nq) {
                         // element-aligned vector inside the row
-                        V const v = *reinterpret_cast<const typename Gt::vec_u *>(a + a_base[e] + l);
-                        ra[e] = v;
+                        V const v = ldg<typename Gt::vec_u>(a + a_base[e] + l);
+                        ra[S][e] = v;
                     } else {
                         // the row's last, partial vector
 #pragma unroll
                         for (int x = 0; x < VEC; ++x)
-                            if (l + x < nq) ra[e][x] = a[a_base[e] + l + x];
+                            if (l + x < nq) ra[S][e][x] = ldg<T>(a + a_base[e] + l + x);
                     }
                 }
             }
         }
 #pragma unroll
         for (int e = 0; e < B_PER; ++e) {
-            rb[e] = vzero<V>();
+            rb[S][e] = vzero<V>();
+            rb_dm[S][e] = DM_WHOLE;
             if (!b_in[e]) continue;
             if constexpr (BCOL) {
                 uint64_t const l = l0 + b_row[e];
@@ -266,26 +337,38 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
                 if (l < nq) {
                     if (j + VEC <= m) {
                         // element-aligned vector inside column l (16-byte aligned when m % VEC == 0)
-                        V const v = *reinterpret_cast<const typename Gt::vec_u *>(b + j + l * m);
-                        rb[e] = v;
+                        V const v = ldg<typename Gt::vec_u>(b + j + l * m);
+                        rb[S][e] = v;
                     } else {
 #pragma unroll
                         for (int x = 0; x < VEC; ++x)
-                            if (j + x < m) rb[e][x] = b[j + x + l * m];
+                            if (j + x < m) rb[S][e][x] = ldg<T>(b + j + x + l * m);
                     }
                 }
+            } else if (nq >= (uint64_t)VEC) {
+                // as the TN rows of A; rows past m read row m - 1 (their
+                // products are never stored)
+                uint64_t const j = j0 + b_col[e];
+                uint64_t const jc = j < m ? j : m - 1;
+                uint64_t const l = l0 + b_row[e];
+                uint64_t const last = nq - VEC;
+                uint64_t const lc = l < last ? l : last;
+                V const v = ldg<typename Gt::vec_u>(b + jc * nq + lc);
+                int const nvalid = l >= nq ? 0 : (int)(nq - l < (uint64_t)VEC ? nq - l : VEC);
+                rb[S][e] = v;
+                rb_dm[S][e] = (int)(l - lc) | nvalid << 8;
             } else {
                 uint64_t const j = j0 + b_col[e];
                 uint64_t const l = l0 + b_row[e];
                 if (j < m) {
                     if (l + VEC <= nq) {
                         // element-aligned vector inside row j (16-byte aligned when nq % VEC == 0)
-                        V const v = *reinterpret_cast<const typename Gt::vec_u *>(b + j * nq + l);
-                        rb[e] = v;
+                        V const v = ldg<typename Gt::vec_u>(b + j * nq + l);
+                        rb[S][e] = v;
                     } else {
 #pragma unroll
                         for (int x = 0; x < VEC; ++x)
-                            if (l + x < nq) rb[e][x] = b[j * nq + l + x];
+                            if (l + x < nq) rb[S][e][x] = ldg<T>(b + j * nq + l + x);
                     }
                 }
             }
@@ -298,15 +381,16 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     // vectors past B_VECS (BM * BK < 256 * VEC) re-read the tile's first
     // vector, B(j0, l0), which is in range, and are not stored.
     static_assert(A_VECS % 256 == 0, "every thread loads whole A vectors");
-    auto load_full = [&](uint64_t l0) {
+    auto load_full = [&](uint64_t l0, auto set) {
+        constexpr int S = decltype(set)::value;
 #pragma unroll
-        for (int e = 0; e < A_PER; ++e) ra[e] = load_a(a + a_fix[e] + l0 * a_step);
+        for (int e = 0; e < A_PER; ++e) ra[S][e] = load_a(a + a_fix[e] + l0 * a_step);
 #pragma unroll
         for (int e = 0; e < B_PER; ++e) {
             uint64_t const jj = b_in[e] ? (uint64_t)b_col[e] : 0;
             uint64_t const ll = b_in[e] ? (uint64_t)b_row[e] : 0;
-            if constexpr (BCOL) rb[e] = *reinterpret_cast<const V *>(b + j0 + jj + (l0 + ll) * m);
-            else rb[e] = *reinterpret_cast<const V *>(b + (j0 + jj) * nq + l0 + ll);
+            if constexpr (BCOL) rb[S][e] = ldg<V>(b + j0 + jj + (l0 + ll) * m);
+            else rb[S][e] = ldg<V>(b + (j0 + jj) * nq + l0 + ll);
         }
     };
 
@@ -318,26 +402,32 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     // are conflict-free (32- and 64-bank models; 4-way for BK = 32 without it),
     // the double stores 2-way as before, the fragment reads unchanged.
     auto tn_swz = [](int l) { return NT * (((l >> 3) * 3) & 3); };
-    auto store_tiles = [&](int buf) {
+    // full_tag false: the slice came from load_tiles (shift and mask the TN
+    // rows of A and the rows of a row-major B here).
+    auto store_tiles = [&](int buf, auto set, auto full_tag) {
+        constexpr int S = decltype(set)::value;
+        constexpr bool GUARDED = !decltype(full_tag)::value;
 #pragma unroll
         for (int e = 0; e < A_PER; ++e) {
             if (tid + 256 * e >= A_VECS) continue;
             if constexpr (!TN) {
-                *reinterpret_cast<V *>(&sA[buf][a_row[e] * LDA + a_col[e]]) = ra[e];
+                *reinterpret_cast<V *>(&sA[buf][a_row[e] * LDA + a_col[e]]) = ra[S][e];
             } else {
+                V const v = GUARDED ? shift_mask(ra[S][e], ra_dm[S][e] & 0xff, ra_dm[S][e] >> 8) : ra[S][e];
 #pragma unroll
                 for (int x = 0; x < VEC; ++x)
-                    sA[buf][(a_row[e] + x) * LDA + (a_col[e] ^ tn_swz(a_row[e] + x))] = ra[e][x];
+                    sA[buf][(a_row[e] + x) * LDA + (a_col[e] ^ tn_swz(a_row[e] + x))] = v[x];
             }
         }
 #pragma unroll
         for (int e = 0; e < B_PER; ++e) {
             if (!b_in[e]) continue;
             if constexpr (BCOL) {
-                *reinterpret_cast<V *>(&sB[buf][b_row[e] * LDB + b_col[e]]) = rb[e];
+                *reinterpret_cast<V *>(&sB[buf][b_row[e] * LDB + b_col[e]]) = rb[S][e];
             } else {
+                V const v = GUARDED ? shift_mask(rb[S][e], rb_dm[S][e] & 0xff, rb_dm[S][e] >> 8) : rb[S][e];
 #pragma unroll
-                for (int x = 0; x < VEC; ++x) sB[buf][(b_row[e] + x) * LDB + b_col[e]] = rb[e][x];
+                for (int x = 0; x < VEC; ++x) sB[buf][(b_row[e] + x) * LDB + b_col[e]] = v[x];
             }
         }
     };
@@ -352,32 +442,60 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     int const b_off = wm * 16 * JT + JT * lc; // this lane's rows within the B tile
 
     uint64_t const nslices = (nq + BK - 1) / BK;
+    // The products of one slice from LDS buffer buf.
+    auto multiply = [&](int buf) {
+#pragma unroll
+        for (int kk = 0; kk < BK / 4; ++kk) {
+            int const kr = kk * 4 + lr;
+            T fa[JT], fb[NT];
+            lds_read<T, JT>(&sB[buf][kr * LDB + b_off], fa);
+            lds_read<T, NT>(&sA[buf][kr * LDA + (TN ? (a_off ^ tn_swz(kr)) : a_off)], fb);
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) acc[jt][nt] = Gt::mfma(fa[jt], fb[nt], acc[jt][nt]);
+        }
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, PF - 1>;
     auto kloop = [&](auto full_tag) {
         constexpr bool FULL = decltype(full_tag)::value;
-        auto load = [&](uint64_t l0) {
-            if constexpr (FULL) load_full(l0);
-            else load_tiles(l0);
+        auto load = [&](uint64_t l0, auto set) {
+            if constexpr (FULL) load_full(l0, set);
+            else load_tiles(l0, set);
         };
-        load(0);
-        store_tiles(0);
-        __syncthreads();
-        for (uint64_t s = 0; s < nslices; ++s) {
-            int const buf = (int)(s & 1);
-            bool const more = s + 1 < nslices;
-            if (more) load((s + 1) * BK);
-#pragma unroll
-            for (int kk = 0; kk < BK / 4; ++kk) {
-                int const kr = kk * 4 + lr;
-                T fa[JT], fb[NT];
-                lds_read<T, JT>(&sB[buf][kr * LDB + b_off], fa);
-                lds_read<T, NT>(&sA[buf][kr * LDA + (TN ? (a_off ^ tn_swz(kr)) : a_off)], fb);
-#pragma unroll
-                for (int jt = 0; jt < JT; ++jt)
-#pragma unroll
-                    for (int nt = 0; nt < NT; ++nt) acc[jt][nt] = Gt::mfma(fa[jt], fb[nt], acc[jt][nt]);
-            }
-            if (more) store_tiles(buf ^ 1);
+        if constexpr (PF == 1) {
+            load(0, I0());
+            store_tiles(0, I0(), full_tag);
             __syncthreads();
+            for (uint64_t s = 0; s < nslices; ++s) {
+                int const buf = (int)(s & 1);
+                bool const more = s + 1 < nslices;
+                if (more) load((s + 1) * BK, I0());
+                multiply(buf);
+                if (more) store_tiles(buf ^ 1, I0(), full_tag);
+                __syncthreads();
+            }
+        } else {
+            // Slice s sits in LDS buffer s & 1 and arrives in register set
+            // s & 1.  A set is reloaded only after its slice was stored, a
+            // buffer only after the barrier that ended its slice's products.
+            load(0, I0());
+            if (nslices > 1) load(BK, I1());
+            store_tiles(0, I0(), full_tag);
+            __syncthreads();
+            for (uint64_t s = 0; s < nslices; s += 2) {
+                if (s + 2 < nslices) load((s + 2) * BK, I0());
+                multiply(0);
+                if (s + 1 < nslices) store_tiles(1, I1(), full_tag);
+                __syncthreads();
+                if (s + 1 < nslices) {
+                    if (s + 3 < nslices) load((s + 3) * BK, I1());
+                    multiply(1);
+                    if (s + 2 < nslices) store_tiles(0, I0(), full_tag);
+                    __syncthreads();
+                }
+            }
         }
     };
     if (full) kloop(std::true_type());
@@ -412,13 +530,13 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
                         V o;
 #pragma unroll
                         for (int x = 0; x < VEC; ++x) o[x] = acc[jt][g * VEC + x][r];
-                        if (stream) __builtin_nontemporal_store(o, reinterpret_cast<V *>(cgroup[g] + j * w));
-                        else *reinterpret_cast<V *>(cgroup[g] + j * w) = o;
+                        if (stream) stg_nt<V>(cgroup[g] + j * w, o);
+                        else stg<V>(cgroup[g] + j * w, o);
                     } else {
                         // TT with ncols % VEC != 0: the group may run past the end
 #pragma unroll
                         for (int x = 0; x < VEC; ++x)
-                            if (ncol0 + g * VEC + x < ncols) cgroup[g][j * w + x] = acc[jt][g * VEC + x][r];
+                            if (ncol0 + g * VEC + x < ncols) stg<T>(cgroup[g] + j * w + x, acc[jt][g * VEC + x][r]);
                     }
                 }
             }
@@ -439,15 +557,15 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
                             V o;
 #pragma unroll
                             for (int x = 0; x < VEC; ++x) o[x] = acc[jt + x][nt][r];
-                            if (stream) __builtin_nontemporal_store(o, reinterpret_cast<V *>(crow + jr + jt));
-                            else *reinterpret_cast<V *>(crow + jr + jt) = o;
+                            if (stream) stg_nt<V>(crow + jr + jt, o);
+                            else stg<V>(crow + jr + jt, o);
                         }
                         continue;
                     }
                 }
 #pragma unroll
                 for (int jt = 0; jt < JT; ++jt)
-                    if (jr + jt < m) crow[jr + jt] = acc[jt][nt][r];
+                    if (jr + jt < m) stg<T>(crow + jr + jt, acc[jt][nt][r]);
             }
         }
     }
@@ -480,13 +598,33 @@ __global__ __launch_bounds__(256, 2) void k_gemm_grouped(const GroupProblem *__r
     using S = TileShape<T, JT, NT, WAVES_M, BK>;
     __shared__ __attribute__((aligned(16))) T sA[2][BK * S::LDA];
     __shared__ __attribute__((aligned(16))) T sB[2][BK * S::LDB];
-    for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
-        GroupProblem const &P = probs[tile_prob[t]];
-        unsigned const local = t - P.tile_begin;
-        unsigned const jb = local % P.nj, nb = local / P.nj;
+    // Tiles are short (a stage contracts 16..256 indices), so the descriptor
+    // of the next tile is read while this one runs and the problem id of the
+    // tile after it one round earlier: no tile starts behind two dependent
+    // descriptor loads.
+    unsigned t = blockIdx.x;
+    unsigned const step = gridDim.x;
+    if (t >= total) return;
+    GroupProblem cur = probs[tile_prob[t]];
+    unsigned pid_next = t + step < total ? tile_prob[t + step] : 0u;
+    for (;;) {
+        unsigned const tn = t + step;
+        bool const has_next = tn < total;
+        GroupProblem nxt = cur;
+        unsigned pid_after = 0;
+        if (has_next) {
+            nxt = probs[pid_next];
+            if (tn + step < total) pid_after = tile_prob[tn + step];
+        }
+        unsigned const local = t - cur.tile_begin;
+        unsigned const jb = local % cur.nj, nb = local / cur.nj;
         gemm_tile<T, JT, NT, WAVES_M, BK, VIEW_TT, false>(
-            static_cast<const T *>(P.a), static_cast<const T *>(P.b), static_cast<T *>(P.c), P.ncols, P.nq,
-            P.ncols, P.m, (uint64_t)jb * S::BM, (uint64_t)nb * S::BN, P.flags, sA, sB);
+            static_cast<const T *>(cur.a), static_cast<const T *>(cur.b), static_cast<T *>(cur.c), cur.ncols,
+            cur.nq, cur.ncols, cur.m, (uint64_t)jb * S::BM, (uint64_t)nb * S::BN, cur.flags, sA, sB);
+        if (!has_next) break;
+        t = tn;
+        cur = nxt;
+        pid_next = pid_after;
     }
 }
 
