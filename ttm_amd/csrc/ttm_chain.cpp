@@ -137,7 +137,16 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
         unsigned const bm = (unsigned)st.cfg.bm(), bn = (unsigned)st.cfg.bn();
         st.tile_prob.clear();
         unsigned begin = 0;
-        for (size_t i = 0; i < st.probs.size(); ++i) {
+        // Workgroup b takes tiles b, b + grid, b + 2 grid, ...; a tile's cost
+        // grows with its contraction length (16..256 indices in a Tucker
+        // batch).  Listing the problems longest contraction first gives every
+        // round of grid tiles nearly equal costs and leaves the cheapest tiles
+        // for the last, partial round, so no workgroup finishes far behind.
+        std::vector<size_t> order(st.probs.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t x, size_t y) { return st.probs[x].nq > st.probs[y].nq; });
+        for (size_t i : order) {
             GroupProblem &P = st.probs[i];
             unsigned const nj = (unsigned)((P.m + bm - 1) / bm);
             uint64_t const nn = (P.ncols + bn - 1) / bn;

@@ -741,11 +741,12 @@ namespace
 {
 // (jt, nt, waves_m, bk): rows 16/32/64 with 256 columns, 128 x 128, and 64 or 32
 // rows with 128 columns for short problems.
-// The last three (slices of 32) are float32 only: rows read in the TT view
-// then cover whole 128-byte lines per slice.
+// The last five (slices of 32) are float32 only: rows read in the TT view
+// then cover whole 128-byte lines per slice, and short rows (16..256
+// indices) take half as many slices, each with twice the matrix work.
 constexpr GroupConfig k_group_configs[] = {
-    {1, 4, 1, 16}, {2, 4, 1, 16}, {4, 4, 1, 16}, {4, 4, 2, 16}, {2, 4, 2, 16},
-    {1, 4, 2, 16}, {1, 4, 1, 32}, {2, 4, 1, 32}, {4, 4, 1, 32},
+    {1, 4, 1, 16}, {2, 4, 1, 16}, {4, 4, 1, 16}, {4, 4, 2, 16}, {2, 4, 2, 16}, {1, 4, 2, 16},
+    {1, 4, 1, 32}, {2, 4, 1, 32}, {4, 4, 1, 32}, {1, 4, 2, 32}, {2, 4, 2, 32},
 };
 constexpr int k_num_group_configs = sizeof(k_group_configs) / sizeof(k_group_configs[0]);
 
@@ -772,6 +773,8 @@ const void *grouped_kernel(int index)
         case 6: return grouped_fn<T, 1, 4, 1, 32>();
         case 7: return grouped_fn<T, 2, 4, 1, 32>();
         case 8: return grouped_fn<T, 4, 4, 1, 32>();
+        case 9: return grouped_fn<T, 1, 4, 2, 32>();
+        case 10: return grouped_fn<T, 2, 4, 2, 32>();
         default: break;
         }
     }
@@ -835,6 +838,8 @@ ttm_status_t launch_gemm_grouped(const GroupConfig &cfg, const GroupProblem *d_p
             if (idx == 6) hipLaunchKernelGGL((k_gemm_grouped<T, 1, 4, 1, 32>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
             else if (idx == 7) hipLaunchKernelGGL((k_gemm_grouped<T, 2, 4, 1, 32>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
             else if (idx == 8) hipLaunchKernelGGL((k_gemm_grouped<T, 4, 4, 1, 32>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
+            else if (idx == 9) hipLaunchKernelGGL((k_gemm_grouped<T, 1, 4, 2, 32>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
+            else if (idx == 10) hipLaunchKernelGGL((k_gemm_grouped<T, 2, 4, 2, 32>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
         } else {
             set_error(TTM_ERR_ARGUMENT, "grouped tile configuration not compiled for float64");
             return TTM_ERR_ARGUMENT;
