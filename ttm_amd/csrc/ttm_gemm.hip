@@ -741,12 +741,13 @@ namespace
 {
 // (jt, nt, waves_m, bk): rows 16/32/64 with 256 columns, 128 x 128, and 64 or 32
 // rows with 128 columns for short problems.
-// The last five (slices of 32) are float32 only: rows read in the TT view
+// Indices 6..10 (slices of 32) are float32 only: rows read in the TT view
 // then cover whole 128-byte lines per slice, and short rows (16..256
 // indices) take half as many slices, each with twice the matrix work.
 constexpr GroupConfig k_group_configs[] = {
     {1, 4, 1, 16}, {2, 4, 1, 16}, {4, 4, 1, 16}, {4, 4, 2, 16}, {2, 4, 2, 16}, {1, 4, 2, 16},
     {1, 4, 1, 32}, {2, 4, 1, 32}, {4, 4, 1, 32}, {1, 4, 2, 32}, {2, 4, 2, 32},
+    {1, 4, 4, 16}, // 64 x 64: four times the tiles of 64 x 256 for the small later stages
 };
 constexpr int k_num_group_configs = sizeof(k_group_configs) / sizeof(k_group_configs[0]);
 
@@ -766,6 +767,7 @@ const void *grouped_kernel(int index)
     case 3: return grouped_fn<T, 4, 4, 2, 16>();
     case 4: return grouped_fn<T, 2, 4, 2, 16>();
     case 5: return grouped_fn<T, 1, 4, 2, 16>();
+    case 11: return grouped_fn<T, 1, 4, 4, 16>();
     default: break;
     }
     if constexpr (sizeof(T) == 4) {
@@ -833,6 +835,7 @@ ttm_status_t launch_gemm_grouped(const GroupConfig &cfg, const GroupProblem *d_p
     case 3: hipLaunchKernelGGL((k_gemm_grouped<T, 4, 4, 2, 16>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total); break;
     case 4: hipLaunchKernelGGL((k_gemm_grouped<T, 2, 4, 2, 16>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total); break;
     case 5: hipLaunchKernelGGL((k_gemm_grouped<T, 1, 4, 2, 16>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total); break;
+    case 11: hipLaunchKernelGGL((k_gemm_grouped<T, 1, 4, 4, 16>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total); break;
     default:
         if constexpr (sizeof(T) == 4) {
             if (idx == 6) hipLaunchKernelGGL((k_gemm_grouped<T, 1, 4, 1, 32>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
