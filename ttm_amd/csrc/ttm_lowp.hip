@@ -116,6 +116,27 @@ struct Lp<LP_BF16X3>
     }
 };
 
+// Global memory accesses of the tiles.  The grouped kernel takes A, B and C
+// from problem descriptors in memory, so the compiler cannot tell that they
+// point to global memory and would emit FLAT instructions, which also count
+// against the LDS wait counter.  The explicit address space keeps them
+// global loads and stores.
+template <class V>
+__device__ __forceinline__ V ldg(const void *p)
+{
+    return *(const __attribute__((address_space(1))) V *)p;
+}
+template <class V>
+__device__ __forceinline__ V ldg_nt(const void *p)
+{
+    return __builtin_nontemporal_load((const __attribute__((address_space(1))) V *)p);
+}
+template <class V>
+__device__ __forceinline__ void stg(void *p, V v)
+{
+    *(__attribute__((address_space(1))) V *)p = v;
+}
+
 __device__ __forceinline__ f32x4 zero4()
 {
     f32x4 z = {0.f, 0.f, 0.f, 0.f};
@@ -230,8 +251,7 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     f32x4 rb[1][4];
     i32x4 rbs[1][BSPLIT ? P : 1][2];
     auto load_a_vec = [&](const float *p) -> f32x4 {
-        return stream ? __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(p))
-                      : *reinterpret_cast<const f32x4 *>(p);
+        return stream ? ldg_nt<f32x4>(p) : ldg<f32x4>(p);
     };
     auto load = [&](uint64_t l0, auto full_tag, auto set_tag) {
         constexpr int S = decltype(set_tag)::value;
@@ -262,12 +282,12 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                         if (a_vec) {
                             ra[S][e] = load_a_vec(a + a_base[s] + l0 + 4 * (e & 1));
                         } else if (l + 4 <= nq) {
-                            f32x4 const v = *reinterpret_cast<const f32x4u *>(a + a_base[s] + l0 + 4 * (e & 1));
+                            f32x4 const v = ldg<f32x4u>(a + a_base[s] + l0 + 4 * (e & 1));
                             ra[S][e] = v;
                         } else {
 #pragma unroll
                             for (int x = 0; x < 4; ++x)
-                                if (l + x < nq) ra[S][e][x] = a[a_base[s] + l0 + 4 * (e & 1) + x];
+                                if (l + x < nq) ra[S][e][x] = ldg<float>(a + a_base[s] + l0 + 4 * (e & 1) + x);
                         }
                     }
                 }
@@ -279,7 +299,7 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
             for (int p = 0; p < P; ++p)
 #pragma unroll
                 for (int s = 0; s < 2; ++s)
-                    rbs[S][p][s] = *reinterpret_cast<const i32x4 *>(bs + p * plane + (j0 + k_row + 64 * s) * ldb + l0 +
+                    rbs[S][p][s] = ldg<i32x4>(bs + p * plane + (j0 + k_row + 64 * s) * ldb + l0 +
                                                                     8 * k_c);
         } else {
 #pragma unroll
@@ -289,12 +309,12 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                 uint64_t const l = l0 + 8 * k_c + 4 * (e & 1);
                 if (j >= m) continue;
                 if (l + 4 <= nq) {
-                    f32x4 const v = *reinterpret_cast<const f32x4u *>(b + j * nq + l); // 16-byte aligned when nq % 4 == 0
+                    f32x4 const v = ldg<f32x4u>(b + j * nq + l); // 16-byte aligned when nq % 4 == 0
                     rb[S][e] = v;
                 } else {
 #pragma unroll
                     for (int x = 0; x < 4; ++x)
-                        if (l + x < nq) rb[S][e][x] = b[j * nq + l + x];
+                        if (l + x < nq) rb[S][e][x] = ldg<float>(b + j * nq + l + x);
                 }
             }
         }
@@ -434,12 +454,12 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                 if (j >= m) continue;
                 if (c_vec) {
                     f32x4 o = {acc[jt][0][r], acc[jt][1][r], acc[jt][2][r], acc[jt][3][r]};
-                    *reinterpret_cast<f32x4 *>(cb + j * w) = o;
+                    stg<f32x4>(cb + j * w, o);
                 } else {
                     // TT with ncols % 4 != 0: the 4 columns may run past the end
 #pragma unroll
                     for (int nt = 0; nt < 4; ++nt)
-                        if (ncol0 + nt < ncols) cb[j * w + nt] = acc[jt][nt][r];
+                        if (ncol0 + nt < ncols) stg<float>(cb + j * w + nt, acc[jt][nt][r]);
                 }
             }
     } else {
@@ -453,11 +473,11 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                 uint64_t const jr = j0 + wm * 64 + 4 * (4 * lg + r);
                 if ((m & 3) == 0 && jr + 4 <= m) {
                     f32x4 o = {acc[0][nt][r], acc[1][nt][r], acc[2][nt][r], acc[3][nt][r]};
-                    *reinterpret_cast<f32x4 *>(crow + jr) = o;
+                    stg<f32x4>(crow + jr, o);
                 } else {
 #pragma unroll
                     for (int jt = 0; jt < 4; ++jt)
-                        if (jr + jt < m) crow[jr + jt] = acc[jt][nt][r];
+                        if (jr + jt < m) stg<float>(crow + jr + jt, acc[jt][nt][r]);
                 }
             }
         }
