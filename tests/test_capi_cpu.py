@@ -128,6 +128,32 @@ def test_double_precision_variants_rejected(ttmlib):
     assert e.value.status == ttm_amd.Status.PRECISION
 
 
+def test_device_memory_entry_points_without_device_work(ttmlib):
+    """The paths of ttm_device_* that touch no device: empty sizes, null pointers."""
+    import ctypes
+    import ttm_amd
+    L = ttmlib.lib()
+    p = ctypes.c_void_p(12345)
+    assert L.ttm_device_alloc(0, ctypes.byref(p)) == 0 and not p.value  # nothing allocated, NULL returned
+    assert L.ttm_device_alloc(16, None) == ttm_amd.Status.ARGUMENT
+    assert b"null" in L.ttm_last_error()
+    assert L.ttm_device_free(None) == 0
+    assert L.ttm_device_copy(None, None, 0) == 0
+    assert L.ttm_device_copy(None, ctypes.c_void_p(8), 4) == ttm_amd.Status.ARGUMENT
+    assert L.ttm_device_zero(None, 0) == 0
+    assert L.ttm_device_zero(None, 4) == ttm_amd.Status.ARGUMENT
+    assert L.ttm_last_error() != b""
+
+
+def test_tucker_rank_checks_before_any_device_work():
+    import torch
+    from ttm_amd.tucker import factors_of
+    A = torch.zeros(4, 5, 6, dtype=torch.float64)
+    for bad in ([0, 2, 2], [2, 6, 2], [2, 2, 7]):
+        with pytest.raises(ValueError):
+            factors_of(A, bad)
+
+
 def test_plan_reference_configs(ttmlib):
     """Kernel families the planner picks for the A 1024x1024x512 sweep (no GPU needed)."""
     import ttm_amd
