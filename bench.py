@@ -169,6 +169,9 @@ def gpu_bench(args, cases):
         c["ms_min"] = float(np.min(ms))
         t = c["ms"] * 1e-3
         c["tflops"] = c["flops"] / t / 1e12
+        # the reference's count, m * (N_A / n_q) * (2 n_q - 1) (example/measure.cpp:70-73)
+        nq = EXTENTS[c["q"] - 1]
+        c["tflops_reference_count"] = c["m"] * (c["flops"] / (2.0 * c["m"] * nq)) * (2 * nq - 1) / t / 1e12
         c["gbps"] = c["bytes"] / t / 1e9
         t_hbm = c["bytes"] / PEAK_HBM
         t_cmp = c["flops"] / PEAK_FLOPS[c["dtype"]]
@@ -225,7 +228,9 @@ def bf16x3_vs_fp32(args, cases, A, C, stream):
                      "bf16x3_ms": round(res["bf16x3"][0], 4),
                      "speedup": round(res["default"][0] / res["bf16x3"][0], 3),
                      "fp32_normwise_err": res["default"][1], "bf16x3_normwise_err": res["bf16x3"][1],
-                     "bf16x3_fp32equiv_tflops": round(c["flops"] / res["bf16x3"][0] / 1e9, 2)})
+                     "bf16x3_fp32equiv_tflops": round(c["flops"] / res["bf16x3"][0] / 1e9, 2),
+                     # six bf16 products per FP32-equivalent product
+                     "bf16x3_mfma_issued_tflops": round(6 * c["flops"] / res["bf16x3"][0] / 1e9, 2)})
     if not rows:
         return None
     sp = [r["speedup"] for r in rows]
@@ -357,20 +362,29 @@ def cpu_baseline(args, budget_s):
         return branches
 
     branches = sweep()  # warm-up
-    t0 = time.perf_counter()
-    n = 0
+    # each timed sweep follows an untimed 32 MiB scrub of the caches, as the
+    # reference's measurement loop does (example/measure.cpp:85-100)
+    scrub = np.empty(8 << 20, dtype=np.float32)
+    times = []
+    t_start = time.perf_counter()
     while True:
+        scrub.fill(float(len(times)))
+        t0 = time.perf_counter()
         sweep()
-        n += 1
-        if time.perf_counter() - t0 >= budget_s:
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_start >= budget_s:
             break
-    dt = (time.perf_counter() - t0) / n
+    dt = float(np.mean(times))
     return {
         "value": flops_sweep / dt / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
         "sample": (f"the same 24-product sweep (f32+f64, first/last-order, q=1..3, m in {{16,1024}}) on A "
                    f"256x256x128 (1/64 of the volume), oracle restatement of tlib::ttm `combined` "
-                   f"(OpenMP x {threads} threads + {os.path.basename(blas)}), mean of {n} sweeps of {dt:.2f} s"),
+                   f"(OpenMP x {threads} threads + {os.path.basename(blas)}), mean of {len(times)} sweeps of "
+                   f"{dt:.2f} s, each after a 32 MiB cache scrub"),
+        "median_value": flops_sweep / float(np.median(times)) / 1e12,
         "blas": os.path.basename(blas), "branches": sorted(set(branches)),
+        "host_cores_reference_count": orc.cores(), "hardware_concurrency": os.cpu_count(),
+        "cpus_available": avail,
     }
 
 
@@ -466,7 +480,8 @@ def main():
         "tucker_batch": tucker,
         "host_path": host,
         "cases": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in c.items()
-                   if k in ("label", "symbol", "ms", "ms_min", "tflops", "gbps", "bound", "frac")}
+                   if k in ("label", "symbol", "ms", "ms_min", "tflops", "tflops_reference_count", "gbps", "bound",
+                           "frac")}
                   for c in cases],
     }
     print(json.dumps(line), flush=True)
