@@ -1,4 +1,4 @@
-"""CPU checks of the reduced-precision emulation used to judge the GPU variants."""
+"""CPU checks of the reduced-precision emulation that the GPU variants are compared with."""
 import numpy as np
 
 import lowp_emul as le

@@ -765,7 +765,7 @@ __global__ __launch_bounds__(256) void k_lowp_generic(const float *__restrict__ 
 
 // Stream-ordered device workspace (the pre-converted B operand): one pool per
 // device that keeps its memory between calls, so a steady stream of products
-// allocates nothing from the driver after the first.
+// allocates no new device memory after the first.
 hipMemPool_t workspace_pool()
 {
     int dev = 0;
