@@ -116,7 +116,8 @@ ttm_status_t hip_check(hipError_t e)
 // column blocks (the stages stream their input once).
 GroupConfig default_config(uint64_t max_m, bool bf16x3)
 {
-    if (bf16x3) return GroupConfig{4, 4, 2, 32}; // the reduced-precision kernel has one shape
+    // the reduced-precision kernel has two shapes: 64 or 128 rows by 128 columns
+    if (bf16x3) return max_m <= 64 ? GroupConfig{2, 4, 2, 32} : GroupConfig{4, 4, 2, 32};
     if (max_m <= 16) return group_config(0);
     if (max_m <= 32) return group_config(1);
     if (max_m <= 64) return group_config(2);
@@ -159,7 +160,8 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
             begin += (unsigned)tiles;
         }
         st.total = begin;
-        int occ = plan->prec == TTM_PREC_BF16X3 ? lowp_group_occupancy() : group_occupancy(st.cfg, plan->is_double);
+        int occ = plan->prec == TTM_PREC_BF16X3 ? lowp_group_occupancy(st.cfg.bm())
+                                                : group_occupancy(st.cfg, plan->is_double);
         if (occ <= 0) occ = 1;
         uint64_t const resident = (uint64_t)occ * (uint64_t)std::max(plan->num_cu, 1);
         st.grid = (unsigned)std::min<uint64_t>(st.total, resident);
@@ -200,7 +202,8 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
 ttm_status_t launch_stage(ttm_chain_plan_t plan, const Stage &st, hipStream_t s)
 {
     if (st.total == 0) return TTM_SUCCESS;
-    if (plan->prec == TTM_PREC_BF16X3) return launch_lowp_grouped(st.d_probs, st.d_tile_prob, st.total, st.grid, s);
+    if (plan->prec == TTM_PREC_BF16X3)
+        return launch_lowp_grouped(st.cfg.bm(), st.d_probs, st.d_tile_prob, st.total, st.grid, s);
     if (plan->is_double)
         return launch_gemm_grouped<double>(st.cfg, st.d_probs, st.d_tile_prob, st.total, st.grid, s);
     return launch_gemm_grouped<float>(st.cfg, st.d_probs, st.d_tile_prob, st.total, st.grid, s);
@@ -311,8 +314,11 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
     // Stage s of every chain, split by row class of m (<= 16, <= 32, <= 64,
     // larger) so that each launch gets a tile height that fits its ranks;
     // launches of one stage are independent, stages run in order.
-    int const classes = precision == TTM_PREC_BF16X3 ? 1 : 4;
-    auto m_class = [&](uint64_t m) { return classes == 1 ? 0 : (m <= 16 ? 0 : m <= 32 ? 1 : m <= 64 ? 2 : 3); };
+    int const classes = precision == TTM_PREC_BF16X3 ? 2 : 4;
+    auto m_class = [&](uint64_t m) {
+        if (classes == 2) return m <= 64 ? 0 : 1; // the two reduced-precision tile heights
+        return m <= 16 ? 0 : m <= 32 ? 1 : m <= 64 ? 2 : 3;
+    };
     std::vector<Stage> groups(max_p * classes);
     std::vector<uint64_t> stage_max_m(groups.size(), 0), stage_cols(groups.size(), 0), stage_nq(groups.size(), 0);
     size_t ws_off = ident_bytes;
@@ -476,7 +482,7 @@ unsigned ttm_chain_stages(ttm_chain_plan_t plan) { return plan ? (unsigned)plan-
 const char *ttm_chain_stage_kernel(ttm_chain_plan_t plan, unsigned s)
 {
     if (!plan || s >= plan->stages.size()) return "";
-    plan->symbol = plan->prec == TTM_PREC_BF16X3 ? lowp_group_symbol()
+    plan->symbol = plan->prec == TTM_PREC_BF16X3 ? lowp_group_symbol(plan->stages[s].cfg.bm())
                                                  : group_symbol(plan->stages[s].cfg, plan->is_double);
     return plan->symbol.c_str();
 }

@@ -186,14 +186,21 @@ __device__ __forceinline__ int lds_at(int P, int c) { return P * LP_LDK + 8 * (c
 // planes of `op` at bs + p * plane, row j at j * ldb, zero padded to
 // multiples of 32 columns and 128 rows (k_lowp_split_b); otherwise B is the
 // row-major float32 m x nq matrix b, converted here.
-template <int MODE, int VIEW, bool BSPLIT>
+template <int MODE, int VIEW, bool BSPLIT, int BM = LP_BM>
 __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const float *__restrict__ b,
                                           const typename Lp<MODE>::op *__restrict__ bs, uint64_t ldb, uint64_t plane,
                                           float *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w, uint64_t m,
                                           uint64_t j0, uint64_t n0, unsigned flags,
                                           typename Lp<MODE>::op (*sA)[LP_BN * LP_LDK],
-                                          typename Lp<MODE>::op (*sB)[LP_BM * LP_LDK])
+                                          typename Lp<MODE>::op (*sB)[BM * LP_LDK])
 {
+    // BM = 128: wave wm owns the 64-row group wm (MFMA tiles t = 0..3);
+    // BM = 64 (TT products of at most 64 rows): both waves along m share the
+    // one 64-row group, wave wm taking tiles t = 2 wm, 2 wm + 1.  The LDS row
+    // order is the same (tile t of group h reads physical rows 64h + 16t + lc).
+    static_assert(BM == 128 || (BM == 64 && VIEW == LPV_TT && !BSPLIT), "64-row blocks: TT products only");
+    constexpr int BH = BM / 64;  // 64-row groups of the block
+    constexpr int JTL = BM / 32; // MFMA row tiles per wave
     using L = Lp<MODE>;
     using op = typename L::op;
     using op2 = typename L::v2;
@@ -298,12 +305,12 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
 #pragma unroll
             for (int p = 0; p < P; ++p)
 #pragma unroll
-                for (int s = 0; s < 2; ++s)
+                for (int s = 0; s < BH; ++s)
                     rbs[S][p][s] = ldg<i32x4>(bs + p * plane + (j0 + k_row + 64 * s) * ldb + l0 +
                                                                     8 * k_c);
         } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
+            for (int e = 0; e < 2 * BH; ++e) {
                 rb[S][e] = zero4();
                 uint64_t const j = j0 + k_row + 64 * (e >> 1);
                 uint64_t const l = l0 + 8 * k_c + 4 * (e & 1);
@@ -324,7 +331,7 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     // of the MN micro-block (r[e][x] = logical row mn_c4 + x, k mn_k4 + e),
     // or, for K rows, half h = t of physical rows k_p + 64h (r[2h], r[2h+1] =
     // its 8 k; pieces 2 and 3 empty); the B tile goes with piece 3.
-    auto store_piece = [&](int t, auto set_tag, op (*sa)[BN * LP_LDK], op (*sb)[BN * LP_LDK]) {
+    auto store_piece = [&](int t, auto set_tag, op (*sa)[BN * LP_LDK], op (*sb)[BM * LP_LDK]) {
         constexpr int S = decltype(set_tag)::value;
         if constexpr (!TN) {
             int const x = t;
@@ -358,11 +365,11 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
 #pragma unroll
                 for (int p = 0; p < P; ++p)
 #pragma unroll
-                    for (int s = 0; s < 2; ++s)
+                    for (int s = 0; s < BH; ++s)
                         *reinterpret_cast<i32x4 *>(&sb[p][lds_at(k_p + 64 * s, k_c)]) = rbs[S][p][s];
             } else {
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
+                for (int h = 0; h < BH; ++h) {
                     op2 q[4][P];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
@@ -380,14 +387,14 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
             }
         }
     };
-    auto store_all = [&](auto set_tag, op (*sa)[BN * LP_LDK], op (*sb)[BN * LP_LDK]) {
+    auto store_all = [&](auto set_tag, op (*sa)[BN * LP_LDK], op (*sb)[BM * LP_LDK]) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) store_piece(t, set_tag, sa, sb);
     };
 
-    f32x4 acc[4][4];
+    f32x4 acc[JTL][4];
 #pragma unroll
-    for (int jt = 0; jt < 4; ++jt)
+    for (int jt = 0; jt < JTL; ++jt)
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) acc[jt][nt] = zero4();
 
@@ -396,17 +403,17 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     // row (column) are adjacent and leave with one 16-byte store.  Physical
     // rows: tile t, lane lc -> 64*half + 16*t + lc.
     int const arow = wn * 64 + lc; // + 16 * nt
-    int const brow = wm * 64 + lc; // + 16 * jt
+    int const brow = wm * (BM / 2) + lc; // + 16 * jt
 
     // The products of one slice from the LDS tiles (sa, sb).
-    auto multiply = [&](op (*sa)[BN * LP_LDK], op (*sb)[BN * LP_LDK]) {
+    auto multiply = [&](op (*sa)[BN * LP_LDK], op (*sb)[BM * LP_LDK]) {
         v8 fa[P][4];
 #pragma unroll
         for (int p = 0; p < P; ++p)
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) fa[p][nt] = *reinterpret_cast<const v8 *>(&sa[p][lds_at(arow + 16 * nt, lg)]);
 #pragma unroll
-        for (int jt = 0; jt < 4; ++jt) {
+        for (int jt = 0; jt < JTL; ++jt) {
             v8 fb[P];
 #pragma unroll
             for (int p = 0; p < P; ++p) fb[p] = *reinterpret_cast<const v8 *>(&sb[p][lds_at(brow + 16 * jt, lg)]);
@@ -447,10 +454,11 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
         uint64_t const k = ncol0 / w;
         float *cb = c + k * m * w + (ncol0 - k * w);
 #pragma unroll
-        for (int jt = 0; jt < 4; ++jt)
+        for (int jt = 0; jt < JTL; ++jt)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                uint64_t const j = j0 + wm * 64 + 4 * (4 * lg + r) + jt;
+                // logical row 64h + 4x + t of MFMA tile t, acc row x = 4 lg + r
+                uint64_t const j = j0 + (BM == 128 ? 64 * wm + 4 * (4 * lg + r) + jt : 4 * (4 * lg + r) + 2 * wm + jt);
                 if (j >= m) continue;
                 if (c_vec) {
                     f32x4 o = {acc[jt][0][r], acc[jt][1][r], acc[jt][2][r], acc[jt][3][r]};
@@ -535,21 +543,22 @@ __global__ __launch_bounds__(256, 2) void k_lowp(const float *__restrict__ a, co
                                                 TILE_A_VEC | TILE_C_VEC | (nj == 1 ? TILE_STREAM : 0u), sA, sB);
 }
 
-// Many TT products (stages of multi-TTM chains) in one persistent launch.
-template <int MODE>
+// Many TT products (stages of multi-TTM chains) in one persistent launch;
+// BM rows (128, or 64 for products of at most 64 rows) by 128 columns.
+template <int MODE, int BM>
 __global__ __launch_bounds__(256, 2) void k_lowp_grouped(const GroupProblem *__restrict__ probs,
                                                          const unsigned *__restrict__ tile_prob, unsigned total)
 {
     using op = typename Lp<MODE>::op;
     __shared__ __attribute__((aligned(16))) op sA[Lp<MODE>::PARTS][LP_BN * LP_LDK];
-    __shared__ __attribute__((aligned(16))) op sB[Lp<MODE>::PARTS][LP_BM * LP_LDK];
+    __shared__ __attribute__((aligned(16))) op sB[Lp<MODE>::PARTS][BM * LP_LDK];
     for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
         GroupProblem const &P = probs[tile_prob[t]];
         unsigned const local = t - P.tile_begin;
         unsigned const jb = local % P.nj, nb = local / P.nj;
-        lowp_tile<MODE, LPV_TT, false>(static_cast<const float *>(P.a), static_cast<const float *>(P.b), nullptr, 0, 0,
-                                       static_cast<float *>(P.c), P.ncols, P.nq, P.ncols, P.m, (uint64_t)jb * LP_BM,
-                                       (uint64_t)nb * LP_BN, P.flags, sA, sB);
+        lowp_tile<MODE, LPV_TT, false, BM>(static_cast<const float *>(P.a), static_cast<const float *>(P.b), nullptr,
+                                           0, 0, static_cast<float *>(P.c), P.ncols, P.nq, P.ncols, P.m,
+                                           (uint64_t)jb * BM, (uint64_t)nb * LP_BN, P.flags, sA, sB);
     }
 }
 
@@ -912,23 +921,27 @@ ttm_status_t launch_lowp(const Fold &f, const float *a, const float *b, float *c
 namespace ttm_amd
 {
 
-int lowp_group_occupancy()
+int lowp_group_occupancy(int bm)
 {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, reinterpret_cast<const void *>(&k_lowp_grouped<LP_BF16X3>), 256, 0) != hipSuccess) {
+    const void *fn = bm == 64 ? reinterpret_cast<const void *>(&k_lowp_grouped<LP_BF16X3, 64>)
+                              : reinterpret_cast<const void *>(&k_lowp_grouped<LP_BF16X3, 128>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, 256, 0) != hipSuccess) {
         (void)hipGetLastError();
         return 0;
     }
     return blocks;
 }
 
-std::string lowp_group_symbol() { return "k_lowp_grouped<2>"; }
+std::string lowp_group_symbol(int bm) { return bm == 64 ? "k_lowp_grouped<2, 64>" : "k_lowp_grouped<2, 128>"; }
 
-ttm_status_t launch_lowp_grouped(const GroupProblem *d_probs, const unsigned *d_tile_prob, unsigned total,
+ttm_status_t launch_lowp_grouped(int bm, const GroupProblem *d_probs, const unsigned *d_tile_prob, unsigned total,
                                  unsigned grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_lowp_grouped<LP_BF16X3>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
+    if (bm == 64)
+        hipLaunchKernelGGL((k_lowp_grouped<LP_BF16X3, 64>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
+    else
+        hipLaunchKernelGGL((k_lowp_grouped<LP_BF16X3, 128>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
     hipError_t const e = hipGetLastError();
     if (e != hipSuccess) {
         set_error(TTM_ERR_HIP, hipGetErrorString(e));

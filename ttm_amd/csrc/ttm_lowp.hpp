@@ -24,10 +24,11 @@ ttm_status_t launch_lowp(const Fold &f, const float *a, const float *b, float *c
                          hipStream_t s, bool generic);
 
 // Grouped BF16X3 launches of TT products (stages of multi-TTM chains) in
-// 128 x 128 tiles; tile lists as for launch_gemm_grouped.
-int lowp_group_occupancy();
-std::string lowp_group_symbol();
-ttm_status_t launch_lowp_grouped(const GroupProblem *d_probs, const unsigned *d_tile_prob, unsigned total,
+// bm x 128 tiles, bm = 128 or 64 (products of at most 64 rows); tile lists as
+// for launch_gemm_grouped.
+int lowp_group_occupancy(int bm);
+std::string lowp_group_symbol(int bm);
+ttm_status_t launch_lowp_grouped(int bm, const GroupProblem *d_probs, const unsigned *d_tile_prob, unsigned total,
                                  unsigned grid, hipStream_t s);
 
 } // namespace ttm_amd
