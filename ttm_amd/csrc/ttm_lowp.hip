@@ -195,10 +195,10 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                                           typename Lp<MODE>::op (*sB)[BM * LP_LDK])
 {
     // BM = 128: wave wm owns the 64-row group wm (MFMA tiles t = 0..3);
-    // BM = 64 (TT products of at most 64 rows): both waves along m share the
-    // one 64-row group, wave wm taking tiles t = 2 wm, 2 wm + 1.  The LDS row
+    // BM = 64 (products of at most 64 rows): both waves along m share the one
+    // 64-row group, wave wm taking tiles t = 2 wm, 2 wm + 1.  The LDS row
     // order is the same (tile t of group h reads physical rows 64h + 16t + lc).
-    static_assert(BM == 128 || (BM == 64 && VIEW == LPV_TT && !BSPLIT), "64-row blocks: TT products only");
+    static_assert(BM == 128 || BM == 64, "64- or 128-row blocks");
     constexpr int BH = BM / 64;  // 64-row groups of the block
     constexpr int JTL = BM / 32; // MFMA row tiles per wave
     using L = Lp<MODE>;
@@ -470,6 +470,22 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                         if (ncol0 + nt < ncols) stg<float>(cb + j * w + nt, acc[jt][nt][r]);
                 }
             }
+    } else if constexpr (BM == 64) {
+        // C[n][j]: the lane's two tiles are the adjacent rows 2 wm, 2 wm + 1 of
+        // each group of four
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            uint64_t const k = ncol0 + nt;
+            if (k >= ncols) continue;
+            float *crow = c + k * m;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int jt = 0; jt < JTL; ++jt) {
+                    uint64_t const j = j0 + 4 * (4 * lg + r) + 2 * wm + jt;
+                    if (j < m) stg<float>(crow + j, acc[jt][nt][r]);
+                }
+        }
     } else {
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) {
@@ -527,20 +543,21 @@ __global__ __launch_bounds__(256) void k_lowp_split_b(const float *__restrict__ 
     }
 }
 
-// One product: a grid of 128 x 128 blocks over the pre-converted B.
-template <int MODE, bool TN>
+// One product: a grid of BM x 128 blocks over the pre-converted B (BM = 64
+// for products of at most 64 rows, else 128).
+template <int MODE, bool TN, int BM>
 __global__ __launch_bounds__(256, 2) void k_lowp(const float *__restrict__ a, const typename Lp<MODE>::op *__restrict__ bs,
                                                  uint64_t ldb, uint64_t plane, float *__restrict__ c, uint64_t ncols,
                                                  uint64_t nq, uint64_t w, uint64_t m, unsigned nj, unsigned total)
 {
     using op = typename Lp<MODE>::op;
     __shared__ __attribute__((aligned(16))) op sA[Lp<MODE>::PARTS][LP_BN * LP_LDK];
-    __shared__ __attribute__((aligned(16))) op sB[Lp<MODE>::PARTS][LP_BM * LP_LDK];
+    __shared__ __attribute__((aligned(16))) op sB[Lp<MODE>::PARTS][BM * LP_LDK];
     unsigned jb, nb;
     lp_block_coords(nj, total, jb, nb);
-    lowp_tile<MODE, TN ? LPV_TN : LPV_NN, true>(a, nullptr, bs, ldb, plane, c, ncols, nq, w, m, (uint64_t)jb * LP_BM,
-                                                (uint64_t)nb * LP_BN,
-                                                TILE_A_VEC | TILE_C_VEC | (nj == 1 ? TILE_STREAM : 0u), sA, sB);
+    lowp_tile<MODE, TN ? LPV_TN : LPV_NN, true, BM>(a, nullptr, bs, ldb, plane, c, ncols, nq, w, m, (uint64_t)jb * BM,
+                                                    (uint64_t)nb * LP_BN,
+                                                    TILE_A_VEC | TILE_C_VEC | (nj == 1 ? TILE_STREAM : 0u), sA, sB);
 }
 
 // Many TT products (stages of multi-TTM chains) in one persistent launch;
@@ -801,19 +818,23 @@ hipMemPool_t workspace_pool()
     return pools[dev];
 }
 
+// Block height of the single-product tiles.
+int lp_block_rows(const Fold &f) { return f.m <= 64 ? 64 : 128; }
+
 // Convert B once into padded planes (ldb = nq rounded up to 32, rows = m
-// rounded up to 128), then run the blocks; the workspace is released in
-// stream order after the product.
+// rounded up to the block height), then run the blocks; the workspace is
+// released in stream order after the product.
 template <int MODE, bool TN>
 ttm_status_t launch_lp(const Fold &f, const float *a, const float *b, float *c, hipStream_t s)
 {
     using op = typename Lp<MODE>::op;
     constexpr int P = Lp<MODE>::PARTS;
+    uint64_t const bm = (uint64_t)lp_block_rows(f);
     uint64_t const ncols = TN ? f.outer : f.outer * f.w;
-    unsigned const nj = (unsigned)((f.m + 127) / 128);
+    unsigned const nj = (unsigned)((f.m + bm - 1) / bm);
     unsigned const total = nj * (unsigned)((ncols + 127) / 128);
     uint64_t const ldb = (f.nq + 31) / 32 * 32;
-    uint64_t const rows = (uint64_t)nj * 128;
+    uint64_t const rows = (uint64_t)nj * bm;
     uint64_t const plane = rows * ldb;
     uint64_t const split_blocks = (rows * (ldb / 8) + 255) / 256;
     if (split_blocks >= (1ull << 31)) {
@@ -830,8 +851,12 @@ ttm_status_t launch_lp(const Fold &f, const float *a, const float *b, float *c, 
     op *const bs = static_cast<op *>(ws);
     hipLaunchKernelGGL((k_lowp_split_b<MODE>), dim3((unsigned)split_blocks), dim3(256), 0, s, b, bs, f.m, f.nq,
                        f.b_stride_j(), f.b_stride_l(), ldb, rows, plane);
-    hipLaunchKernelGGL((k_lowp<MODE, TN>), dim3(total), dim3(256), 0, s, a, bs, ldb, plane, c, ncols, f.nq, f.w, f.m,
-                       nj, total);
+    if (bm == 64)
+        hipLaunchKernelGGL((k_lowp<MODE, TN, 64>), dim3(total), dim3(256), 0, s, a, bs, ldb, plane, c, ncols, f.nq,
+                           f.w, f.m, nj, total);
+    else
+        hipLaunchKernelGGL((k_lowp<MODE, TN, 128>), dim3(total), dim3(256), 0, s, a, bs, ldb, plane, c, ncols, f.nq,
+                           f.w, f.m, nj, total);
     hipError_t const e = hipGetLastError();
     (void)hipFreeAsync(ws, s);
     if (e != hipSuccess) {
@@ -878,7 +903,7 @@ std::string lowp_symbol(const Fold &f, ttm_precision_t prec, bool generic)
         std::snprintf(buf, sizeof buf, "k_fp8<%s, %s>", tn, bc);
     } else {
         int const mode = prec == TTM_PREC_BF16 ? LP_BF16 : (prec == TTM_PREC_FP16 ? LP_FP16 : LP_BF16X3);
-        std::snprintf(buf, sizeof buf, "k_lowp<%d, %s>", mode, tn);
+        std::snprintf(buf, sizeof buf, "k_lowp<%d, %s, %d>", mode, tn, lp_block_rows(f));
     }
     return buf;
 }
