@@ -366,6 +366,35 @@ def test_host_entry_points_with_host_buffers(gpu, oracle):
         assert rc.normwise_error(c, c64, a, b) <= 4 * na[q - 1] * rc.eps(dtype)
 
 
+def test_host_entry_staging_of_large_buffers(gpu):
+    """Host buffers of 16 MiB and more move through the pinned chunk pipeline
+    (64 MiB chunks, ragged last chunk) both ways: the host entry must give the
+    device entry's result bit for bit.  A is 150 MiB (three chunks, the last
+    22 MiB), C is 176 MiB, B stays below the staging threshold."""
+    import torch
+    import ttm_amd
+    na, pia, q, m = [256, 384, 400], [1, 2, 3], 1, 300
+    rng = np.random.default_rng(12)
+    a = rng.uniform(-1, 1, size=int(np.prod(na))).astype(np.float32)
+    b = rng.uniform(-1, 1, size=m * na[q - 1]).astype(np.float32)
+    nc = list(na)
+    nc[q - 1] = m
+    wa, wc = ttm_amd.dense_strides(na, pia), ttm_amd.dense_strides(nc, pia)
+    c = np.full(int(np.prod(nc)), np.nan, dtype=np.float32)
+    ttm_amd.ttm_raw(q, 3, a, na, wa, pia, b, [m, na[q - 1]], [1, 2], c, nc, wc)
+    ad, bd = torch.from_numpy(a).to(gpu), torch.from_numpy(b).to(gpu)
+    cd = torch.empty(c.size, dtype=torch.float32, device=gpu)
+    ttm_amd.ttm_raw(q, 3, ad, na, wa, pia, bd, [m, na[q - 1]], [1, 2], cd, nc, wc)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(c, cd.cpu().numpy())
+    # A's chunks land whole: one column of the product checked in float64
+    i3 = 399
+    ref = (b.reshape(na[0], m).T.astype(np.float64) @
+           a.reshape(na[2], na[1], na[0])[i3].T.astype(np.float64))
+    got = c.reshape(nc[2], nc[1], nc[0])[i3].T
+    assert np.max(np.abs(got - ref)) <= 1e-3
+
+
 @pytest.mark.parametrize("program", ["interfaces", "device_tensor"])
 def test_cpp_dropin_program_runs(gpu, program):
     """tests/cpp/interfaces.cpp: the three call forms + error texts, on the GPU.
