@@ -172,6 +172,14 @@ def test_plan_reference_configs(ttmlib):
                         assert k == "nn_stream", (pia, q, m, k)
                         assert sym.startswith("k_nn_stream<" + ("double" if dbl else "float") + ", 1, 6, ")
                         continue
+                    if m == 16:
+                        # TN view (q fastest) with all of B in LDS: 8 waves, or 3 for 513-piece B rows
+                        nq = na[q - 1]
+                        spb = {(False, 1024): "8, 2, 257", (False, 512): "8, 2, 129",
+                               (True, 1024): "3, 2, 513", (True, 512): "8, 2, 257"}[(dbl, nq)]
+                        assert k == "tn_stream", (pia, q, m, k)
+                        assert sym == "k_tn_lds<" + ("double" if dbl else "float") + ", " + spb + ">", sym
+                        continue
                     assert k == "gemm", (pia, q, m, k)
                     assert sym.startswith("k_gemm<" + ("double" if dbl else "float"))
                     assert sym.endswith(("true, true>" if qh == 1 else "false, true>")), sym

@@ -293,6 +293,35 @@ def test_1024x1024x512_stream_kernels_sampled(gpu, dtype):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_tn_lds_edges_vs_oracle(gpu, oracle, dtype):
+    """The TN streaming kernel with all of B in LDS (q fastest, m <= 16): ragged
+    512-byte row segments, row counts that are not a multiple of 16, fewer
+    tiles than waves, an odd number of segments per wave, m from 1 to 16, both
+    B layouts and every B capacity (129, 257 and, for float64, 513 pieces per
+    row), against the float64 brute force."""
+    import torch
+    import ttm_amd
+    rng = np.random.default_rng(77)
+    cases = [([4, 3, 5], 4), ([132, 7, 3], 16), ([516, 5, 3], 13), ([1024, 33, 2], 16), ([1020, 17, 1], 1),
+             ([8, 1, 1], 3), ([260, 70, 9], 16), ([384, 100, 4], 7), ([132, 300, 70], 16), ([128, 512, 64], 12)]
+    q, pia = 1, [1, 2, 3]  # q is the fastest mode: the TN view
+    for na, m in cases:
+        for colmajor in (False, True):
+            a, wa, b, nb, pib, nc, wc = random_case(rng, na, pia, q, m, colmajor, dtype)
+            ttm_amd.force_kernel("tn_stream")
+            try:
+                sym = ttm_amd.plan_kernel_symbol(q, na, pia, nb, pib, is_double=dtype == np.float64)
+            finally:
+                ttm_amd.force_kernel(None)
+            assert sym.startswith("k_tn_lds<" + ("double" if dtype == np.float64 else "float")), sym
+            c = run_gpu(torch, gpu, q, a, na, wa, pia, b, nb, pib, nc, wc, kernel="tn_stream")
+            c64 = oracle.ttm_brute_f64(q, a, na, wa, b, m, colmajor, nc, wc)
+            err = rc.normwise_error(c, c64, a, b)
+            _say(f"tn_lds na={na} m={m} col={colmajor} {sym}: err {err:.2e}")
+            assert err <= 4 * na[q - 1] * rc.eps(dtype), (na, m, colmajor)
+
+
 # ----------------------------------------------------------------------------- python bindings
 def test_ttmpy_reference_vectors_numpy_and_torch(gpu):
     """ttmpy/tests/test.py: exact equality with numpy.einsum, for ttm and all three ttms orders."""

@@ -358,6 +358,169 @@ __global__ __launch_bounds__(256) void k_tn_stream(const T *__restrict__ a, cons
         }
     }
 }
+// tn_lds: w == 1, m <= 16, nq % VEC == 0, all of B in LDS.
+// C[k][j] = sum_l A[k*nq + l] * B(j, l).  Persistent: wave g of G takes the
+// 16-row tiles g, g + G, ...; a tile's rows are walked in segments of 512
+// bytes (32 pieces of 16 bytes).  A segment is read as eight wave loads of
+// 2 rows x 512 B (lanes 0-31 row t, lanes 32-63 row t + 8, lane piece l & 31):
+// whole 128-byte lines per load, the pattern that reads fastest here.  The
+// wave writes the segment to its own 8 KB LDS buffer (row r's piece p at
+// r*32 + ((p + r) & 31): each load's half is one contiguous 512 B store, and
+// the 16 rows a read spans fall in 16 different bank groups) and reads it back
+// by row: step s gives lane (lc, lr) piece 4s + lr of row lc, i.e. indices
+// l = VEC*(32*seg + 4s + lr) + e, and MFMA (s, e) puts them in k-slot lr.  B is
+// staged once per workgroup as rows of `spb` pieces (spb = 1 mod 16: the 16
+// rows of a read fall in 16 different bank groups), zero past m and nq.  Only
+// the wave that wrote a buffer reads it, and LDS operations of one wave
+// complete in issue order, so the buffer needs no barrier.
+template <class T, int WAVES, int D, int SPB_MAX>
+__global__ __launch_bounds__(WAVES * 64) void k_tn_lds(const T *__restrict__ a, const T *__restrict__ b,
+                                                       T *__restrict__ c, uint64_t outer, uint64_t nq, uint64_t m,
+                                                       uint64_t bsj, uint64_t bsl, uint32_t spb)
+{
+    using Tr = Traits<T>;
+    constexpr int VEC = Tr::VEC;
+    using V = typename Tr::vec;
+    using Acc = typename Tr::acc;
+    constexpr int SEGP = 32; // pieces per segment
+
+    __shared__ V bs[16 * SPB_MAX];
+    __shared__ V as[WAVES][16 * SEGP];
+
+    int const lane = threadIdx.x & 63;
+    int const wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int const lc = lane & 15;
+    int const lr = lane >> 4;
+
+    // B rows 0..15, spb pieces each (zero past m and past nq)
+    {
+        T *bsl_ = reinterpret_cast<T *>(bs);
+        uint32_t const rowlen = spb * VEC;
+        for (uint32_t idx = threadIdx.x; idx < 16u * rowlen; idx += WAVES * 64) {
+            uint32_t const j = idx / rowlen;
+            uint32_t const l = idx - j * rowlen;
+            bsl_[idx] = (j < m && l < nq) ? b[j * bsj + l * bsl] : T(0);
+        }
+    }
+    __syncthreads();
+
+    uint64_t const nvec = nq / VEC; // pieces per row
+    uint64_t const cpt = (nvec + SEGP - 1) / SEGP; // segments per row
+    uint64_t const ntiles = (outer + 15) / 16;
+    uint64_t const nw = (uint64_t)gridDim.x * WAVES;
+    uint64_t const wid = (uint64_t)blockIdx.x * WAVES + wave;
+    uint64_t const mytiles = wid < ntiles ? (ntiles - 1 - wid) / nw + 1 : 0;
+    uint64_t const nch = mytiles * cpt;
+    if (nch == 0) return;
+
+    int const hrow = lane >> 5; // load: rows t (lanes 0-31) and t + 8 (lanes 32-63)
+    int const piece = lane & 31;
+
+    // Issue cursor.  Every step issues loads (the loop then has one shape and
+    // its waits count only older segments); past the last segment of the wave
+    // the cursor stays on it, so the extra loads repeat in-bounds addresses.
+    V av[D][8];
+    bool pv[D]; // the lane's piece is inside the row (zeroed when stored to LDS otherwise)
+    bool rg[D]; // the slot holds a row's last, partial segment (uniform)
+    bool const ragged = (nvec % SEGP) != 0;
+    uint64_t i_tile = wid, i_seg = 0, i_count = 0;
+    auto issue = [&](int slot) {
+        uint64_t const p = i_seg * SEGP + piece;
+        pv[slot] = p < nvec;
+        rg[slot] = ragged && i_seg + 1 == cpt;
+        uint64_t const pc = pv[slot] ? p : 0; // pieces past the row end load piece 0
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            uint64_t row = i_tile * 16 + t + 8 * hrow;
+            row = row < outer ? row : outer - 1; // rows past the end read the last row; not stored
+            av[slot][t] = ld_nt(reinterpret_cast<V const *>(a + row * nq) + pc);
+        }
+        if (++i_count < nch) {
+            if (++i_seg == cpt) {
+                i_seg = 0;
+                i_tile += nw;
+            }
+        }
+    };
+
+    Acc acc[2] = {splat_zero<Acc>(), splat_zero<Acc>()}; // two chains: even and odd steps
+    V *const buf = as[wave];
+    uint64_t c_tile = wid, c_seg = 0; // compute cursor
+    auto compute = [&](int slot) {
+        if (rg[slot]) {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                int const r = t + 8 * hrow;
+                buf[r * SEGP + ((piece + r) & (SEGP - 1))] = pv[slot] ? av[slot][t] : splat_zero<V>();
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                int const r = t + 8 * hrow;
+                buf[r * SEGP + ((piece + r) & (SEGP - 1))] = av[slot][t];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        V const *brow = bs + (uint32_t)lc * spb + (uint32_t)c_seg * SEGP;
+#pragma unroll
+        for (int s = 0; s < SEGP / 4; ++s) {
+            int const q = 4 * s + lr;
+            V const x = buf[lc * SEGP + ((q + lc) & (SEGP - 1))];
+            V const y = brow[q];
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) acc[s & 1] = mfma4(y[e], x[e], acc[s & 1]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (++c_seg == cpt) {
+            uint64_t const row = c_tile * 16 + lc;
+            Acc const sum = acc[0] + acc[1];
+            if (row < outer) {
+                T *cp = c + row * m;
+                if constexpr (Tr::acc_row(1, 0) == 4) {
+                    // f32: the lane's four values are rows 4*lr .. 4*lr + 3 of the product tile
+                    uint64_t const jb = (uint64_t)lr * 4;
+                    if ((m & 3) == 0) {
+                        if (jb < m) __builtin_nontemporal_store(sum, reinterpret_cast<f32x4 *>(cp + jb));
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            if (jb + r < m) cp[jb + r] = sum[r];
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        uint64_t const j = (uint64_t)Tr::acc_row(lr, r);
+                        if (j < m) cp[j] = sum[r];
+                    }
+                }
+            }
+            acc[0] = acc[1] = splat_zero<Acc>();
+            c_seg = 0;
+            c_tile += nw;
+        }
+    };
+
+    // segment ch sits in slot ch % D
+#pragma unroll
+    for (int d = 0; d < D - 1; ++d) issue(d);
+    uint64_t const nfull = nch / D * D;
+    for (uint64_t ch = 0; ch < nfull; ch += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            issue((d + D - 1) % D);
+            compute(d);
+        }
+    }
+    // the last nch - nfull (< D) segments were issued by the loop or the prologue
+#pragma unroll
+    for (int d = 0; d < D - 1; ++d)
+        if ((uint64_t)d < nch - nfull) compute(d);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -374,12 +537,79 @@ constexpr int stream_depth(int jt, bool tn) { return jt == 1 ? (tn ? 3 : 6) : (j
 // 16-byte vectors of B rows: row-major B with nq % VEC == 0.
 bool stream_bvec(const Fold &f, int vec) { return !f.b_colmajor && f.nq % (uint64_t)vec == 0; }
 
+// tn_lds instances: B rows of at most SPB_MAX pieces (nq <= VEC * (SPB_MAX - 1)),
+// eight waves per workgroup while B and the eight 8 KB wave buffers fit the
+// 160 KB of LDS, three waves for the 16 rows x 513 pieces of B (131 KB).
+// m = 16 on A 1024x1024x512 (q fastest, n_q = 1024 / 512): float32 with 8
+// waves 6.0 / 5.7 TB/s, with 4 waves 5.6 / 5.6 TB/s, pipeline depth 3 or 4 no
+// better than 2 (the LDS-staged tile: 5.7 / 5.4 TB/s); float64 with 3 / 8
+// waves 5.7 / 6.2 TB/s (the tile: 5.6 / 5.4 TB/s).
+constexpr int kTnLdsDepth = 2;
+uint32_t tn_lds_spb(const Fold &f, int vec) { return (uint32_t)(((f.nq / vec + 31) / 32) * 32 + 1); }
+int tn_lds_spb_max(const Fold &f, bool is_double)
+{
+    uint32_t const spb = tn_lds_spb(f, is_double ? 2 : 4);
+    if (spb <= 129) return 129;
+    if (spb <= 257) return 257;
+    return is_double && spb <= 513 ? 513 : 0;
+}
+constexpr int tn_lds_waves(int spb_max) { return spb_max <= 257 ? 8 : 3; }
+bool tn_lds_ok(const Fold &f, bool is_double) { return f.m <= 16 && tn_lds_spb_max(f, is_double) > 0; }
+
+int device_cus()
+{
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        return 256;
+    }
+    if (cus[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) {
+            (void)hipGetLastError();
+            n = 256;
+        }
+        cus[dev] = n;
+    }
+    return cus[dev];
+}
+
+template <class T, int SPB_MAX>
+void launch_tn_lds(const Fold &f, const T *a, const T *b, T *c, hipStream_t s)
+{
+    constexpr int W = tn_lds_waves(SPB_MAX);
+    auto const fn = k_tn_lds<T, W, kTnLdsDepth, SPB_MAX>;
+    static int const occ = [&] {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void *>(fn), W * 64, 0) !=
+                hipSuccess ||
+            blocks <= 0) {
+            (void)hipGetLastError();
+            blocks = 1;
+        }
+        return blocks;
+    }();
+    uint64_t const ntiles = (f.outer + 15) / 16;
+    uint64_t const want = (ntiles + W - 1) / W;
+    uint64_t const full = (uint64_t)device_cus() * occ;
+    unsigned const grid = (unsigned)(want < full ? want : full);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(W * 64), 0, s, a, b, c, f.outer, f.nq, f.m, f.b_stride_j(),
+                       f.b_stride_l(), tn_lds_spb(f, Traits<T>::VEC));
+}
+
 template <class T, int JT, bool BROW>
 void launch_stream(const Fold &f, const T *a, const T *b, T *c, hipStream_t s, bool tn)
 {
     constexpr int VEC = Traits<T>::VEC;
     uint64_t const bsj = f.b_stride_j(), bsl = f.b_stride_l();
-    if (tn) {
+    constexpr bool is_double = std::is_same_v<T, double>;
+    if (tn && JT == 1 && tn_lds_ok(f, is_double)) {
+        int const smax = tn_lds_spb_max(f, is_double);
+        if (smax == 129) launch_tn_lds<T, 129>(f, a, b, c, s);
+        else if (smax == 257) launch_tn_lds<T, 257>(f, a, b, c, s);
+        else if constexpr (is_double) launch_tn_lds<T, 513>(f, a, b, c, s);
+    } else if (tn) {
         constexpr int D = stream_depth(JT, true);
         dim3 grid((unsigned)((f.outer + 63) / 64));
         hipLaunchKernelGGL((k_tn_stream<T, JT, D, BROW>), grid, dim3(256), 0, s, a, b, c, f.outer, f.nq, f.m, bsj,
@@ -449,6 +679,11 @@ std::string kernel_symbol(Kernel k, const Fold &f, bool is_double, ttm_precision
     case Kernel::tn_stream: {
         bool const tn = k == Kernel::tn_stream;
         int const j = pick_jt(f.m);
+        if (tn && tn_lds_ok(f, is_double)) {
+            int const smax = tn_lds_spb_max(f, is_double);
+            return "k_tn_lds<" + t + ", " + std::to_string(tn_lds_waves(smax)) + ", " + std::to_string(kTnLdsDepth) +
+                   ", " + std::to_string(smax) + ">";
+        }
         return std::string(tn ? "k_tn_stream<" : "k_nn_stream<") + t + ", " + jt + ", " +
                std::to_string(stream_depth(j, tn)) + ", " + (stream_bvec(f, is_double ? 2 : 4) ? "true" : "false") +
                ">";
@@ -489,6 +724,10 @@ Kernel choose_kernel(const Fold &f, bool is_double, ttm_precision_t prec, bool a
     // m <= 16 in the NN view: the register-streaming kernel reads A at
     // 6.4-6.8 TB/s on the bench shapes, the LDS-staged tile at 5.3-5.8
     if (nn_ok && f.m <= 16) return Kernel::nn_stream;
+    // m <= 16 in the TN view with all of B in LDS: tn_lds reads A at
+    // 5.7-6.1 TB/s (float32) and 5.7-6.2 TB/s (float64) on the bench shapes,
+    // the LDS-staged tile at 5.3-5.7 and 5.4-5.6
+    if (tn_ok && tn_lds_ok(f, is_double)) return Kernel::tn_stream;
     if (gemm_ok) return Kernel::gemm;
     if (tn_ok) return Kernel::tn_stream;
     if (nn_ok) return Kernel::nn_stream;
