@@ -185,8 +185,8 @@ def gpu_bench(args, cases):
 
 
 def bf16x3_vs_fp32(args, cases, A, C, stream):
-    """The compute-bound float32 products again in BF16X3: time, and the
-    normwise relative error of both arithmetics against the float64
+    """The compute-bound float32 products again in BF16X3 and FP16X2: time,
+    and the normwise relative error of each arithmetic against the float64
     definition on 4096 sampled entries of C."""
     import torch
     import ttm_amd
@@ -203,7 +203,7 @@ def bf16x3_vs_fp32(args, cases, A, C, stream):
                             precision=prec)
 
         res = {}
-        for prec in ("default", "bf16x3"):
+        for prec in ("default", "bf16x3", "fp16x2"):
             for _ in range(2):
                 run(prec)
             ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
@@ -229,11 +229,20 @@ def bf16x3_vs_fp32(args, cases, A, C, stream):
                      "fp32_normwise_err": res["default"][1], "bf16x3_normwise_err": res["bf16x3"][1],
                      "bf16x3_fp32equiv_tflops": round(c["flops"] / res["bf16x3"][0] / 1e9, 2),
                      # six bf16 products per FP32-equivalent product
-                     "bf16x3_mfma_issued_tflops": round(6 * c["flops"] / res["bf16x3"][0] / 1e9, 2)})
+                     "bf16x3_mfma_issued_tflops": round(6 * c["flops"] / res["bf16x3"][0] / 1e9, 2),
+                     "fp16x2_ms": round(res["fp16x2"][0], 4),
+                     "fp16x2_speedup": round(res["default"][0] / res["fp16x2"][0], 3),
+                     "fp16x2_normwise_err": res["fp16x2"][1],
+                     "fp16x2_fp32equiv_tflops": round(c["flops"] / res["fp16x2"][0] / 1e9, 2),
+                     # three fp16 products per FP32-equivalent product (plus the scale pass over A)
+                     "fp16x2_mfma_issued_tflops": round(3 * c["flops"] / res["fp16x2"][0] / 1e9, 2)})
     if not rows:
         return None
     sp = [r["speedup"] for r in rows]
+    sp2 = [r["fp16x2_speedup"] for r in rows]
     return {"speedup_geomean": round(float(np.exp(np.mean(np.log(sp)))), 3),
+            "fp16x2_speedup_geomean": round(float(np.exp(np.mean(np.log(sp2)))), 3),
+            "max_fp16x2_normwise_err": max(r["fp16x2_normwise_err"] for r in rows),
             "max_bf16x3_normwise_err": max(r["bf16x3_normwise_err"] for r in rows),
             "max_fp32_normwise_err": max(r["fp32_normwise_err"] for r in rows),
             "error_reference": "float64 definition on 4096 sampled entries", "cases": rows}

@@ -76,7 +76,9 @@ typedef enum ttm_precision_t {
     TTM_PREC_BF16X3 = 1,    /* three-term bfloat16 split, 6 products */
     TTM_PREC_BF16 = 2,      /* inputs rounded to bfloat16 */
     TTM_PREC_FP16 = 3,      /* inputs rounded to binary16 */
-    TTM_PREC_FP8_E4M3 = 4   /* OCP e4m3fn with per-slice power-of-two scaling */
+    TTM_PREC_FP8_E4M3 = 4,  /* OCP e4m3fn with per-slice power-of-two scaling */
+    TTM_PREC_FP16X2 = 5     /* two-term binary16 split of A and B, each scaled by one power
+                               of two (from its largest magnitude), 3 products */
 } ttm_precision_t;
 
 /* --- device pointers, stream ordered ---------------------------------- */

@@ -5,6 +5,10 @@ Written from the format definitions, independent of the GPU code:
 * binary16: numpy's float16 conversion (round-to-nearest-even);
 * BF16X3: x = x0 + x1 + x2 with x_k = bf16(x - x0 - .. - x_{k-1}) and the six
   products of total order <= 2 (Henry, Tang & Heinecke, ARITH 2019);
+* FP16X2: each operand scaled by one power of two 2^(15 - E), where f * 2^E
+  (f in [0.5, 1)) is its largest magnitude (all-zero or non-finite: 2^0),
+  then x = x0 + x1 with x0 = fp16(x), x1 = fp16(x - x0), and the three
+  products x0 y0 + x0 y1 + x1 y0, scaled back by the two powers of two;
 * OCP FP8 e4m3fn: 1 sign, 4 exponent (bias 7), 3 mantissa bits, no
   infinities, max 448, subnormal step 2^-9; round-to-nearest-even; with one
   power-of-two scale 2^e per block of 32 consecutive contraction indices,
@@ -36,6 +40,22 @@ def bf16x3_parts(x):
     r2 = (r1.astype(np.float64) - p1).astype(np.float32)
     p2 = bf16(r2)
     return p0, p1, p2
+
+
+def fp16x2_exponent(x):
+    """15 - E for the largest magnitude f * 2^E of x (0 if all zero or not finite)."""
+    amax = float(np.max(np.abs(np.asarray(x, dtype=np.float64)))) if np.size(x) else 0.0
+    if not (amax > 0.0) or not np.isfinite(amax):
+        return 0
+    return 15 - int(np.frexp(amax)[1])
+
+
+def fp16x2_parts(x, e):
+    """Two binary16 parts of x * 2^e (x float32, the scaling exact)."""
+    xs = (np.asarray(x, dtype=np.float64) * 2.0 ** e).astype(np.float32)
+    p0 = fp16(xs)
+    r1 = (xs.astype(np.float64) - p0).astype(np.float32)  # exact in float32
+    return p0, fp16(r1)
 
 
 def e4m3fn(v):
@@ -100,6 +120,14 @@ def emulate(prec, A3, Bm):
         for p, q in ((0, 0), (0, 1), (1, 0), (0, 2), (1, 1), (2, 0)):
             total = total + np.einsum("jl,kli->kji", b[p], a[q])
         return total
+    if prec == "fp16x2":
+        ea, eb = fp16x2_exponent(A3), fp16x2_exponent(Bm)
+        a = fp16x2_parts(A3, ea)
+        b = fp16x2_parts(Bm, eb)
+        total = 0
+        for p, q in ((0, 0), (0, 1), (1, 0)):
+            total = total + np.einsum("jl,kli->kji", b[p], a[q])
+        return total * 2.0 ** -(ea + eb)
     if prec == "fp8":
         return np.einsum("jl,kli->kji", fp8_block_quantize(Bm, axis=1), fp8_block_quantize(A3, axis=1))
     raise ValueError(prec)

@@ -6,7 +6,7 @@ Each variant is checked twice:
   elementwise over the rounded operands;
 * against the FP64 oracle (brute force of the definition), reporting max and
   normwise relative errors (the error table on A 64x48x32), with
-  BF16X3 held to the 1e-6 normwise target.
+  BF16X3 and FP16X2 held to the 1e-6 normwise target.
 """
 import sys
 
@@ -18,7 +18,7 @@ import refcases as rc
 
 pytestmark = pytest.mark.gpu
 
-PRECS = ["bf16x3", "bf16", "fp16", "fp8"]
+PRECS = ["bf16x3", "bf16", "fp16", "fp8", "fp16x2"]
 
 
 def _say(*a):
@@ -69,6 +69,7 @@ def check_vs_emulation(c, a, na, pia, q, Bm, prec, label):
     ref = le.emulate(prec, A3, Bm)                       # (outer, m, w)
     mag = {"bf16": lambda x: np.abs(le.bf16(x)), "fp16": lambda x: np.abs(le.fp16(x)),
            "bf16x3": lambda x: np.abs(x.astype(np.float64)) * 1.001,
+           "fp16x2": lambda x: np.abs(x.astype(np.float64)) * 1.001,
            "fp8": lambda x: np.abs(x.astype(np.float64)) * 1.2}[prec]
     # FP32 accumulation bound; the scaled FP8 instruction sums 128 products per
     # step inside the matrix core with undocumented intermediate rounding, so
@@ -140,13 +141,17 @@ def test_precision_error_table_64x48x32(gpu, oracle):
         fp32 = table[(dist, "default")][1]
         assert table[(dist, "bf16x3")][1] <= 1e-6, table[(dist, "bf16x3")]
         assert table[(dist, "bf16x3")][1] <= 4 * fp32 + 1e-9
+        assert table[(dist, "fp16x2")][1] <= 1e-6, table[(dist, "fp16x2")]
         assert 1e-5 < table[(dist, "bf16")][1] < 2e-2
         assert 1e-6 < table[(dist, "fp16")][1] < 3e-3
         assert 1e-3 < table[(dist, "fp8")][1] < 0.15
 
 
-def test_bf16x3_integer_data_is_exact(gpu):
-    """Integers below 2^24 split exactly into three bfloat16 parts: the closed form holds."""
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x2"])
+def test_split_variants_integer_data_is_exact(gpu, prec):
+    """Integers below 2^24 split exactly into three bfloat16 parts, and the
+    scaled integers of these cases into two binary16 parts: the closed form
+    holds."""
     import torch
     for na, pia, q, colmajor in [([4, 8, 12], [1, 2, 3], 2, True), ([16, 12, 8], [3, 2, 1], 3, False),
                                  ([8, 8, 8], [2, 1, 3], 1, True)]:
@@ -157,12 +162,14 @@ def test_bf16x3_integer_data_is_exact(gpu):
         nc[q - 1] = m
         wc = rc.dense_strides(nc, pia)
         for kernel in (None, "lowp_generic"):
-            c = run(torch, gpu, q, a.astype(np.float32), na, wa, pia, b, [m, na[q - 1]], pib, nc, wc, "bf16x3", kernel)
+            c = run(torch, gpu, q, a.astype(np.float32), na, wa, pia, b, [m, na[q - 1]], pib, nc, wc, prec, kernel)
             assert np.array_equal(rc.as_nd(c, nc, wc), rc.closed_form(q, na, pia, m, "ones")), (na, pia, q, kernel)
 
 
-def test_bf16x3_full_size_sampled(gpu):
-    """Compute-bound A 1024x1024x512 products (m = 1024) in BF16X3: sampled entries against the float64 definition."""
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x2"])
+def test_split_variants_full_size_sampled(gpu, prec):
+    """Compute-bound A 1024x1024x512 products (m = 1024) in BF16X3 and FP16X2:
+    sampled entries against the float64 definition."""
     import torch
     import ttm_amd
     na = [1024, 1024, 512]
@@ -176,7 +183,7 @@ def test_bf16x3_full_size_sampled(gpu):
         nc[q - 1] = m
         wc = rc.dense_strides(nc, pia)
         C = torch.empty(int(np.prod(nc)), dtype=torch.float32, device=gpu)
-        ttm_amd.ttm_raw(q, 3, A, na, wa, pia, Bt, [m, na[q - 1]], [2, 1], C, nc, wc, precision="bf16x3")
+        ttm_amd.ttm_raw(q, 3, A, na, wa, pia, Bt, [m, na[q - 1]], [2, 1], C, nc, wc, precision=prec)
         torch.cuda.synchronize()
         p = 3
         idx = [rng.integers(0, nc[r], size=2048) for r in range(p)]
@@ -188,7 +195,7 @@ def test_bf16x3_full_size_sampled(gpu):
         expect = np.sum(fib * brow, axis=1)
         got = C[torch.from_numpy(offc.astype(np.int64)).to(gpu)].double().cpu().numpy()
         rel = np.linalg.norm(got - expect) / np.linalg.norm(expect)
-        _say(f"bf16x3 1024x1024x512 pi={pia} q={q}: sampled normwise-rel {rel:.3e}")
+        _say(f"{prec} 1024x1024x512 pi={pia} q={q}: sampled normwise-rel {rel:.3e}")
         assert rel <= 1e-6
         del C
     torch.cuda.empty_cache()

@@ -21,6 +21,47 @@ def test_bf16x3_split_is_exact_to_24_bits():
     assert (np.abs(p0 + p1 - x) / np.abs(x)).max() > 2.0 ** -20
 
 
+def test_fp16x2_split_and_scale():
+    """The scale puts the largest magnitude in [2^14, 2^15); two binary16 parts
+    then hold 22 significant bits, so dropping x1 y1 leaves about 2^-22."""
+    assert le.fp16x2_exponent(np.array([0.75, -0.1])) == 15 and le.fp16x2_exponent(np.array([1.0])) == 14
+    assert le.fp16x2_exponent(np.array([3.0e38])) == 15 - 128 and le.fp16x2_exponent(np.zeros(3)) == 0
+    assert le.fp16x2_exponent(np.array([1.0, np.inf])) == 0
+    rng = np.random.default_rng(3)
+    for mag in (1e-30, 1e-3, 1.0, 1e5, 1e30):
+        x = (rng.uniform(-1, 1, 10000) * mag).astype(np.float32)
+        e = le.fp16x2_exponent(x)
+        assert 2.0 ** 14 <= np.abs(x).max() * 2.0 ** e < 2.0 ** 15
+        p0, p1 = le.fp16x2_parts(x, e)
+        back = (p0 + p1) * 2.0 ** -e
+        big = np.abs(x) > np.abs(x).max() * 2.0 ** -8        # x1 above the binary16 subnormal range
+        rel = np.abs(back - x.astype(np.float64))[big] / np.abs(x.astype(np.float64))[big]
+        assert rel.max() <= 2.0 ** -22
+        # every element: 2^-22 relative, or half the binary16 subnormal step
+        # (2^-25 scaled back) where the second part is subnormal
+        assert np.all(np.abs(back - x.astype(np.float64)) <= 2.0 ** -22 * np.abs(x.astype(np.float64))
+                      + 2.0 ** -25 * 2.0 ** -e)
+        # one part alone is only binary16 accurate: the second part matters
+        assert (np.abs(p0 * 2.0 ** -e - x)[big] / np.abs(x)[big]).max() > 2.0 ** -13
+
+
+def test_fp16x2_product_error_is_fp32_level():
+    """FP16X2 against the exact float64 product: normwise error below 1e-6 on
+    uniform and wide-exponent data (the three dropped-term and rounding terms
+    are each below 2^-22 relative)."""
+    rng = np.random.default_rng(4)
+    for wide in (False, True):
+        A3 = rng.uniform(-1, 1, (4, 256, 6))
+        B = rng.uniform(-1, 1, (9, 256))
+        if wide:
+            A3 = A3 * np.exp2(rng.integers(-30, 30, A3.shape))
+        A3, B = A3.astype(np.float32), B.astype(np.float32)
+        exact = np.einsum("jl,kli->kji", B.astype(np.float64), A3.astype(np.float64))
+        got = le.emulate("fp16x2", A3, B)
+        err = np.linalg.norm(got - exact) / (np.linalg.norm(A3.astype(np.float64)) * np.linalg.norm(B.astype(np.float64)))
+        assert err < 1e-6, (wide, err)
+
+
 def test_e4m3fn_known_answers():
     # values read back from the gfx950 conversion instruction (in-bounds probe):
     # 1.0 -> 0x38, 448 -> 0x7e, -1.5 -> 0xbc, 0.001 -> 0x01, 0.0625 -> 0x18, 3e-3 -> 0x02
@@ -48,5 +89,5 @@ def test_emulation_matches_float64_for_exact_inputs():
     A3 = rng.integers(-8, 9, (3, 40, 5)).astype(np.float32)
     B = rng.integers(-8, 9, (6, 40)).astype(np.float32)
     exact = np.einsum("jl,kli->kji", B.astype(np.float64), A3.astype(np.float64))
-    for prec in ("bf16", "fp16", "bf16x3", "fp8"):
+    for prec in ("bf16", "fp16", "bf16x3", "fp8", "fp16x2"):
         np.testing.assert_array_equal(le.emulate(prec, A3, B), exact, err_msg=prec)

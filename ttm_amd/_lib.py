@@ -51,12 +51,13 @@ class Precision(enum.IntEnum):
     BF16 = 2
     FP16 = 3
     FP8_E4M3 = 4
+    FP16X2 = 5
 
 
 PRECISION_NAMES = {
     "default": Precision.DEFAULT, "fp32": Precision.DEFAULT, "fp64": Precision.DEFAULT,
     "bf16x3": Precision.BF16X3, "bf16": Precision.BF16, "fp16": Precision.FP16,
-    "fp8": Precision.FP8_E4M3, "fp8_e4m3": Precision.FP8_E4M3,
+    "fp8": Precision.FP8_E4M3, "fp8_e4m3": Precision.FP8_E4M3, "fp16x2": Precision.FP16X2,
 }
 
 # (name, restype, argtypes) for every symbol of include/ttm_amd.h

@@ -706,7 +706,7 @@ Kernel kernel_from_name(const char *name)
 Kernel choose_kernel(const Fold &f, bool is_double, ttm_precision_t prec, bool aligned, Kernel force)
 {
     if (prec != TTM_PREC_DEFAULT) {
-        if (is_double || prec > TTM_PREC_FP8_E4M3) return Kernel::none;
+        if (is_double || prec > TTM_PREC_FP16X2) return Kernel::none;
         bool const lp_ok = lowp_supports(f, aligned);
         if (force == Kernel::generic || force == Kernel::lowp_generic || !lp_ok) return Kernel::lowp_generic;
         return Kernel::lowp;

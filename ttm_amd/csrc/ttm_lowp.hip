@@ -58,7 +58,8 @@ enum LpMode : int
 {
     LP_BF16 = 0,
     LP_FP16 = 1,
-    LP_BF16X3 = 2
+    LP_BF16X3 = 2,
+    LP_FP16X2 = 3
 };
 
 template <int MODE>
@@ -115,6 +116,47 @@ struct Lp<LP_BF16X3>
         p[2] = __builtin_convertvector(r2, bf16x2);
     }
 };
+
+template <>
+struct Lp<LP_FP16X2>
+{
+    using op = _Float16;
+    using v2 = f16x2;
+    using v4 = f16x4;
+    using v8 = f16x8;
+    static constexpr int PARTS = 2;
+    __device__ static f32x4 mfma(v8 a, v8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+    // x (already scaled into the binary16 range) = x0 + x1 + O(2^-22 |x|);
+    // x - float(x0) is exact in float32.
+    __device__ static void split(float x, op (&p)[2])
+    {
+        p[0] = (_Float16)x;
+        p[1] = (_Float16)(x - (float)p[0]);
+    }
+    __device__ static void split2(f32x2 x, v2 (&p)[2])
+    {
+        p[0] = __builtin_convertvector(x, f16x2);
+        p[1] = __builtin_convertvector(x - __builtin_convertvector(p[0], f32x2), f16x2);
+    }
+};
+
+// FP16X2 scaling: every element of A is multiplied by 2^ea and every element
+// of B by 2^eb before the split, with e = 15 - E for the operand's largest
+// magnitude f * 2^E (f in [0.5, 1)), so the scaled operand lies below 2^15
+// (binary16 reaches 65504); the product is multiplied by 2^-(ea + eb).  An
+// all-zero or non-finite maximum gives e = 0.
+__device__ __forceinline__ int fp16x2_exponent(float amax)
+{
+    if (!(amax > 0.f) || !__builtin_isfinite(amax)) return 0;
+    int e = 0;
+    (void)frexpf(amax, &e);
+    return 15 - e;
+}
+__device__ __forceinline__ f32x2 scale2(f32x2 x, int e)
+{
+    f32x2 r = {ldexpf(x[0], e), ldexpf(x[1], e)};
+    return r;
+}
 
 // Global memory accesses of the tiles.  The grouped kernel takes A, B and C
 // from problem descriptors in memory, so the compiler cannot tell that they
@@ -192,7 +234,7 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                                           float *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w, uint64_t m,
                                           uint64_t j0, uint64_t n0, unsigned flags,
                                           typename Lp<MODE>::op (*sA)[LP_BN * LP_LDK],
-                                          typename Lp<MODE>::op (*sB)[BM * LP_LDK])
+                                          typename Lp<MODE>::op (*sB)[BM * LP_LDK], int ea = 0, int ce = 0)
 {
     // BM = 128: wave wm owns the 64-row group wm (MFMA tiles t = 0..3);
     // BM = 64 (products of at most 64 rows): both waves along m share the one
@@ -336,8 +378,12 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
         if constexpr (!TN) {
             int const x = t;
             op2 lo[P], hi[P];
-            f32x2 const x01 = {ra[S][0][x], ra[S][1][x]};
-            f32x2 const x23 = {ra[S][2][x], ra[S][3][x]};
+            f32x2 x01 = {ra[S][0][x], ra[S][1][x]};
+            f32x2 x23 = {ra[S][2][x], ra[S][3][x]};
+            if constexpr (MODE == LP_FP16X2) {
+                x01 = scale2(x01, ea);
+                x23 = scale2(x23, ea);
+            }
             L::split2(x01, lo);
             L::split2(x23, hi);
             int const off = lds_at(lds_row(mn_c4 + x), wave) + 4 * mn_half;
@@ -349,7 +395,8 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
             op2 q[4][P];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                f32x2 const x = {ra[S][2 * h + u / 2][2 * (u % 2)], ra[S][2 * h + u / 2][2 * (u % 2) + 1]};
+                f32x2 x = {ra[S][2 * h + u / 2][2 * (u % 2)], ra[S][2 * h + u / 2][2 * (u % 2) + 1]};
+                if constexpr (MODE == LP_FP16X2) x = scale2(x, ea);
                 L::split2(x, q[u]);
             }
             int const off = lds_at(k_p + 64 * h, k_c);
@@ -425,6 +472,9 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                     acc[jt][nt] = L::mfma(fb[2], fa[0][nt], acc[jt][nt]);
                     acc[jt][nt] = L::mfma(fb[0], fa[1][nt], acc[jt][nt]);
                     acc[jt][nt] = L::mfma(fb[1], fa[0][nt], acc[jt][nt]);
+                } else if constexpr (P == 2) {
+                    acc[jt][nt] = L::mfma(fb[0], fa[1][nt], acc[jt][nt]);
+                    acc[jt][nt] = L::mfma(fb[1], fa[0][nt], acc[jt][nt]);
                 }
                 acc[jt][nt] = L::mfma(fb[0], fa[0][nt], acc[jt][nt]);
             }
@@ -446,6 +496,14 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     };
     if (full) kloop(std::true_type());
     else kloop(std::false_type());
+    if constexpr (MODE == LP_FP16X2) {
+#pragma unroll
+        for (int jt = 0; jt < JTL; ++jt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[jt][nt][r] = ldexpf(acc[jt][nt][r], -ce);
+    }
 
     // ---- epilogue ------------------------------------------------------------
     uint64_t const ncol0 = n0 + wn * 64 + 4 * lc; // this lane's 4 adjacent columns
@@ -514,7 +572,7 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
 template <int MODE>
 __global__ __launch_bounds__(256) void k_lowp_split_b(const float *__restrict__ b, typename Lp<MODE>::op *__restrict__ bs,
                                                       uint64_t m, uint64_t nq, uint64_t bsj, uint64_t bsl, uint64_t ldb,
-                                                      uint64_t rows, uint64_t plane)
+                                                      uint64_t rows, uint64_t plane, const int *__restrict__ exps)
 {
     using L = Lp<MODE>;
     using v2 = typename L::v2;
@@ -533,6 +591,7 @@ __global__ __launch_bounds__(256) void k_lowp_split_b(const float *__restrict__ 
             uint64_t const l = l0 + 2 * t + e;
             x[e] = (j < m && l < nq) ? b[j * bsj + l * bsl] : 0.f;
         }
+        if constexpr (MODE == LP_FP16X2) x = scale2(x, exps[1]);
         L::split2(x, q[t]);
     }
 #pragma unroll
@@ -544,11 +603,15 @@ __global__ __launch_bounds__(256) void k_lowp_split_b(const float *__restrict__ 
 }
 
 // One product: a grid of BM x 128 blocks over the pre-converted B (BM = 64
-// for products of at most 64 rows, else 128).
+// for products of at most 64 rows, else 128).  FP16X2 (two LDS planes, 32 KB
+// per block) runs three blocks per CU: its register budget drops to 168 (a
+// few values spill), which measured 3.85-4.54 ms against 4.22-4.85 ms with
+// two blocks on the 1024-row float32 products of A 1024x1024x512.
 template <int MODE, bool TN, int BM>
-__global__ __launch_bounds__(256, 2) void k_lowp(const float *__restrict__ a, const typename Lp<MODE>::op *__restrict__ bs,
+__global__ __launch_bounds__(256, MODE == LP_FP16X2 ? 3 : 2) void k_lowp(const float *__restrict__ a, const typename Lp<MODE>::op *__restrict__ bs,
                                                  uint64_t ldb, uint64_t plane, float *__restrict__ c, uint64_t ncols,
-                                                 uint64_t nq, uint64_t w, uint64_t m, unsigned nj, unsigned total)
+                                                 uint64_t nq, uint64_t w, uint64_t m, unsigned nj, unsigned total,
+                                                 const int *__restrict__ exps)
 {
     using op = typename Lp<MODE>::op;
     __shared__ __attribute__((aligned(16))) op sA[Lp<MODE>::PARTS][LP_BN * LP_LDK];
@@ -557,7 +620,69 @@ __global__ __launch_bounds__(256, 2) void k_lowp(const float *__restrict__ a, co
     lp_block_coords(nj, total, jb, nb);
     lowp_tile<MODE, TN ? LPV_TN : LPV_NN, true, BM>(a, nullptr, bs, ldb, plane, c, ncols, nq, w, m, (uint64_t)jb * BM,
                                                     (uint64_t)nb * LP_BN,
-                                                    TILE_A_VEC | TILE_C_VEC | (nj == 1 ? TILE_STREAM : 0u), sA, sB);
+                                                    TILE_A_VEC | TILE_C_VEC | (nj == 1 ? TILE_STREAM : 0u), sA, sB,
+                                                    MODE == LP_FP16X2 ? exps[0] : 0,
+                                                    MODE == LP_FP16X2 ? exps[0] + exps[1] : 0);
+}
+
+// FP16X2 scales: the largest magnitude of A (blocks [0, blocks_a)) and of B
+// (the remaining blocks), one partial maximum per block, then one block
+// turns both into exponents (no atomics: the result does not depend on the
+// order of the blocks).  NaNs are skipped by fmaxf; they stay NaN in the
+// product through the conversion.
+__device__ __forceinline__ float block_max(float v, float *red)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    v = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    __syncthreads();
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_absmax_partial(const float *__restrict__ a, uint64_t na,
+                                                        const float *__restrict__ b, uint64_t nb, unsigned blocks_a,
+                                                        float *__restrict__ part)
+{
+    __shared__ float red[4];
+    bool const is_a = blockIdx.x < blocks_a;
+    const float *x = is_a ? a : b;
+    uint64_t const n = is_a ? na : nb;
+    uint64_t const blk = is_a ? blockIdx.x : blockIdx.x - blocks_a;
+    uint64_t const nblk = is_a ? blocks_a : gridDim.x - blocks_a;
+    uint64_t const stride = nblk * 256;
+    float mx = 0.f;
+    uint64_t vec_end = 0;
+    if ((reinterpret_cast<uintptr_t>(x) & 15u) == 0) {
+        uint64_t const nv = n / 4;
+        for (uint64_t i = blk * 256 + threadIdx.x; i < nv; i += stride) {
+            f32x4 const v = ldg_nt<f32x4>(x + 4 * i);
+            mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))));
+        }
+        vec_end = nv * 4;
+    }
+    for (uint64_t i = vec_end + blk * 256 + threadIdx.x; i < n; i += stride) mx = fmaxf(mx, fabsf(x[i]));
+    mx = block_max(mx, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = mx;
+}
+
+__global__ __launch_bounds__(256) void k_absmax_final(const float *__restrict__ part, unsigned blocks_a,
+                                                      unsigned blocks, int *__restrict__ exps)
+{
+    __shared__ float red[4];
+    float ma = 0.f, mb = 0.f;
+    for (unsigned i = threadIdx.x; i < blocks; i += 256) {
+        if (i < blocks_a) ma = fmaxf(ma, part[i]);
+        else mb = fmaxf(mb, part[i]);
+    }
+    ma = block_max(ma, red);
+    mb = block_max(mb, red);
+    if (threadIdx.x == 0) {
+        exps[0] = fp16x2_exponent(ma);
+        exps[1] = fp16x2_exponent(mb);
+    }
 }
 
 // Many TT products (stages of multi-TTM chains) in one persistent launch;
@@ -744,7 +869,8 @@ __device__ __forceinline__ float fp8_value(float x, int e)
 
 __global__ __launch_bounds__(256) void k_lowp_generic(const float *__restrict__ a, const float *__restrict__ b,
                                                       float *__restrict__ c, uint64_t outer, uint64_t nq, uint64_t w,
-                                                      uint64_t m, uint64_t bsj, uint64_t bsl, int prec)
+                                                      uint64_t m, uint64_t bsj, uint64_t bsl, int prec,
+                                                      const int *__restrict__ exps)
 {
     uint64_t const total = outer * m * w;
     uint64_t const step = (uint64_t)gridDim.x * blockDim.x;
@@ -774,6 +900,14 @@ __global__ __launch_bounds__(256) void k_lowp_generic(const float *__restrict__ 
                 case TTM_PREC_BF16: s = fmaf((float)(__bf16)y, (float)(__bf16)x, s); break;
                 case TTM_PREC_FP16: s = fmaf((float)(_Float16)y, (float)(_Float16)x, s); break;
                 case TTM_PREC_FP8_E4M3: s = fmaf(fp8_value(y, eb), fp8_value(x, ea), s); break;
+                case TTM_PREC_FP16X2: {
+                    _Float16 px[2], py[2];
+                    Lp<LP_FP16X2>::split(ldexpf(x, exps[0]), px);
+                    Lp<LP_FP16X2>::split(ldexpf(y, exps[1]), py);
+                    float t = (float)py[0] * (float)px[1] + (float)py[1] * (float)px[0];
+                    s += t + (float)py[0] * (float)px[0];
+                    break;
+                }
                 default: {
                     __bf16 px[3], py[3];
                     Lp<LP_BF16X3>::split(x, px);
@@ -785,7 +919,7 @@ __global__ __launch_bounds__(256) void k_lowp_generic(const float *__restrict__ 
                 }
             }
         }
-        c[idx] = s;
+        c[idx] = prec == TTM_PREC_FP16X2 ? ldexpf(s, -(exps[0] + exps[1])) : s;
     }
 }
 
@@ -824,11 +958,46 @@ int lp_block_rows(const Fold &f) { return f.m <= 64 ? 64 : 128; }
 // Convert B once into padded planes (ldb = nq rounded up to 32, rows = m
 // rounded up to the block height), then run the blocks; the workspace is
 // released in stream order after the product.
+// FP16X2 scale pass: grid sizes and the workspace it needs (partial maxima,
+// then the two exponents).
+struct ScalePass
+{
+    unsigned blocks_a = 0, blocks = 0;
+    size_t bytes() const { return (size_t)blocks * sizeof(float) + 2 * sizeof(int); }
+};
+ScalePass scale_pass(const Fold &f)
+{
+    auto nblk = [](uint64_t n, uint64_t cap) {
+        uint64_t const b = (n / 4 + 255) / 256;
+        return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
+    };
+    ScalePass sp;
+    sp.blocks_a = nblk(f.numel_a(), 2048);
+    sp.blocks = sp.blocks_a + nblk(f.numel_b(), 64);
+    return sp;
+}
+// Launch the scale pass into ws (partials first, exponents after them);
+// returns the exponents' address.
+int *launch_scale_pass(const Fold &f, const float *a, const float *b, const ScalePass &sp, void *ws, hipStream_t s)
+{
+    float *const part = static_cast<float *>(ws);
+    int *const exps = reinterpret_cast<int *>(part + sp.blocks);
+    hipLaunchKernelGGL(k_absmax_partial, dim3(sp.blocks), dim3(256), 0, s, a, f.numel_a(), b, f.numel_b(),
+                       sp.blocks_a, part);
+    hipLaunchKernelGGL(k_absmax_final, dim3(1), dim3(256), 0, s, part, sp.blocks_a, sp.blocks, exps);
+    return exps;
+}
+
+// Convert B once into padded planes (ldb = nq rounded up to 32, rows = m
+// rounded up to the block height), then run the blocks; the workspace is
+// released in stream order after the product.  FP16X2 first finds the two
+// scale exponents (kept after the planes in the same workspace).
 template <int MODE, bool TN>
 ttm_status_t launch_lp(const Fold &f, const float *a, const float *b, float *c, hipStream_t s)
 {
     using op = typename Lp<MODE>::op;
     constexpr int P = Lp<MODE>::PARTS;
+    constexpr bool SCALED = MODE == LP_FP16X2;
     uint64_t const bm = (uint64_t)lp_block_rows(f);
     uint64_t const ncols = TN ? f.outer : f.outer * f.w;
     unsigned const nj = (unsigned)((f.m + bm - 1) / bm);
@@ -841,22 +1010,26 @@ ttm_status_t launch_lp(const Fold &f, const float *a, const float *b, float *c, 
         set_error(TTM_ERR_ARGUMENT, "B too large for the reduced-precision kernels");
         return TTM_ERR_ARGUMENT;
     }
+    size_t const planes_bytes = (P * plane * sizeof(op) + 255) / 256 * 256;
+    ScalePass const sp = SCALED ? scale_pass(f) : ScalePass();
     hipMemPool_t const pool = workspace_pool();
     void *ws = nullptr;
-    if (!pool || hipMallocFromPoolAsync(&ws, P * plane * sizeof(op), pool, s) != hipSuccess) {
+    if (!pool || hipMallocFromPoolAsync(&ws, planes_bytes + (SCALED ? sp.bytes() : 0), pool, s) != hipSuccess) {
         (void)hipGetLastError();
         set_error(TTM_ERR_ALLOC, "workspace for the converted B operand");
         return TTM_ERR_ALLOC;
     }
     op *const bs = static_cast<op *>(ws);
+    int const *exps = nullptr;
+    if constexpr (SCALED) exps = launch_scale_pass(f, a, b, sp, static_cast<char *>(ws) + planes_bytes, s);
     hipLaunchKernelGGL((k_lowp_split_b<MODE>), dim3((unsigned)split_blocks), dim3(256), 0, s, b, bs, f.m, f.nq,
-                       f.b_stride_j(), f.b_stride_l(), ldb, rows, plane);
+                       f.b_stride_j(), f.b_stride_l(), ldb, rows, plane, exps);
     if (bm == 64)
         hipLaunchKernelGGL((k_lowp<MODE, TN, 64>), dim3(total), dim3(256), 0, s, a, bs, ldb, plane, c, ncols, f.nq,
-                           f.w, f.m, nj, total);
+                           f.w, f.m, nj, total, exps);
     else
         hipLaunchKernelGGL((k_lowp<MODE, TN, 128>), dim3(total), dim3(256), 0, s, a, bs, ldb, plane, c, ncols, f.nq,
-                           f.w, f.m, nj, total);
+                           f.w, f.m, nj, total, exps);
     hipError_t const e = hipGetLastError();
     (void)hipFreeAsync(ws, s);
     if (e != hipSuccess) {
@@ -902,7 +1075,10 @@ std::string lowp_symbol(const Fold &f, ttm_precision_t prec, bool generic)
     if (prec == TTM_PREC_FP8_E4M3) {
         std::snprintf(buf, sizeof buf, "k_fp8<%s, %s>", tn, bc);
     } else {
-        int const mode = prec == TTM_PREC_BF16 ? LP_BF16 : (prec == TTM_PREC_FP16 ? LP_FP16 : LP_BF16X3);
+        int const mode = prec == TTM_PREC_BF16 ? LP_BF16
+                         : prec == TTM_PREC_FP16 ? LP_FP16
+                         : prec == TTM_PREC_FP16X2 ? LP_FP16X2
+                                                   : LP_BF16X3;
         std::snprintf(buf, sizeof buf, "k_lowp<%d, %s, %d>", mode, tn, lp_block_rows(f));
     }
     return buf;
@@ -915,8 +1091,21 @@ ttm_status_t launch_lowp(const Fold &f, const float *a, const float *b, float *c
         uint64_t const total = f.numel_c();
         uint64_t blocks = (total + 255) / 256;
         if (blocks > (1u << 20)) blocks = 1u << 20;
+        void *ws = nullptr;
+        int const *exps = nullptr;
+        if (prec == TTM_PREC_FP16X2) {
+            ScalePass const sp = scale_pass(f);
+            hipMemPool_t const pool = workspace_pool();
+            if (!pool || hipMallocFromPoolAsync(&ws, sp.bytes(), pool, s) != hipSuccess) {
+                (void)hipGetLastError();
+                set_error(TTM_ERR_ALLOC, "workspace for the FP16X2 scales");
+                return TTM_ERR_ALLOC;
+            }
+            exps = launch_scale_pass(f, a, b, sp, ws, s);
+        }
         hipLaunchKernelGGL(k_lowp_generic, dim3((unsigned)blocks), dim3(256), 0, s, a, b, c, f.outer, f.nq, f.w,
-                           f.m, f.b_stride_j(), f.b_stride_l(), (int)prec);
+                           f.m, f.b_stride_j(), f.b_stride_l(), (int)prec, exps);
+        if (ws) (void)hipFreeAsync(ws, s);
     } else if (prec == TTM_PREC_FP8_E4M3) {
         bool const tn = f.w == 1;
         if (tn) {
@@ -930,6 +1119,8 @@ ttm_status_t launch_lowp(const Fold &f, const float *a, const float *b, float *c
         return launch_lp_mode<LP_BF16>(f, a, b, c, s);
     } else if (prec == TTM_PREC_FP16) {
         return launch_lp_mode<LP_FP16>(f, a, b, c, s);
+    } else if (prec == TTM_PREC_FP16X2) {
+        return launch_lp_mode<LP_FP16X2>(f, a, b, c, s);
     } else {
         return launch_lp_mode<LP_BF16X3>(f, a, b, c, s);
     }
