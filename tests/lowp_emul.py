@@ -6,7 +6,7 @@ Written from the format definitions, independent of the GPU code:
 * BF16X3: x = x0 + x1 + x2 with x_k = bf16(x - x0 - .. - x_{k-1}) and the six
   products of total order <= 2 (Henry, Tang & Heinecke, ARITH 2019);
 * FP16X2: each operand scaled by one power of two 2^(15 - E), where f * 2^E
-  (f in [0.5, 1)) is its largest magnitude (all-zero or non-finite: 2^0),
+  (f in [0.5, 1)) is its largest finite magnitude (none non-zero: 2^0),
   then x = x0 + x1 with x0 = fp16(x), x1 = fp16(x - x0), and the three
   products x0 y0 + x0 y1 + x1 y0, scaled back by the two powers of two;
 * OCP FP8 e4m3fn: 1 sign, 4 exponent (bias 7), 3 mantissa bits, no
@@ -43,9 +43,11 @@ def bf16x3_parts(x):
 
 
 def fp16x2_exponent(x):
-    """15 - E for the largest magnitude f * 2^E of x (0 if all zero or not finite)."""
-    amax = float(np.max(np.abs(np.asarray(x, dtype=np.float64)))) if np.size(x) else 0.0
-    if not (amax > 0.0) or not np.isfinite(amax):
+    """15 - E for the largest finite magnitude f * 2^E of x (0 if none is non-zero)."""
+    a = np.abs(np.asarray(x, dtype=np.float64)).reshape(-1)
+    a = a[np.isfinite(a)]
+    amax = float(a.max()) if a.size else 0.0
+    if not (amax > 0.0):
         return 0
     return 15 - int(np.frexp(amax)[1])
 

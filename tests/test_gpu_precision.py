@@ -147,6 +147,33 @@ def test_precision_error_table_64x48x32(gpu, oracle):
         assert 1e-3 < table[(dist, "fp8")][1] < 0.15
 
 
+def test_fp16x2_infinity_sets_no_scale(gpu):
+    """Data beyond binary16's range (|x| up to 1e6) with one infinity in A: the
+    scale comes from the finite values, so every entry of C that does not
+    involve the infinity matches the emulation, and those that do are not
+    finite (a scale taken from the infinity would overflow every value)."""
+    import torch
+    rng = np.random.default_rng(21)
+    na, pia, q, m = [64, 48, 32], [1, 2, 3], 2, 40
+    for kernel in (None, "lowp_generic"):
+        a, wa, Bm, b, nb, pib, nc, wc = case(rng, na, pia, q, m, False, "uniform")
+        a = (a * 1.0e6).astype(np.float32)
+        Bm = (Bm * 1.0e6).astype(np.float32)
+        b = Bm.reshape(-1).copy()
+        a[1234] = np.inf
+        c = run(torch, gpu, q, a, na, wa, pia, b, nb, pib, nc, wc, "fp16x2", kernel)
+        c3 = folded(c, nc, pia, q).astype(np.float64)
+        A3 = folded(a, na, pia, q)
+        with np.errstate(invalid="ignore"):
+            ref = le.emulate("fp16x2", A3, Bm)
+        touched = np.einsum("jl,kli->kji", np.ones(Bm.shape), np.isinf(A3).astype(np.float64)) > 0
+        assert touched.sum() == m and not np.isfinite(c3[touched]).any(), kernel
+        A3f = np.where(np.isinf(A3), 0, A3).astype(np.float64)
+        bound = 4 * na[q - 1] * 2.0 ** -24 * np.einsum("jl,kli->kji", np.abs(Bm.astype(np.float64)), np.abs(A3f)) * 1.001
+        err = np.abs(c3 - ref)[~touched]
+        assert np.all(err <= bound[~touched]), (kernel, float(np.max(err / bound[~touched])))
+
+
 @pytest.mark.parametrize("prec", ["bf16x3", "fp16x2"])
 def test_split_variants_integer_data_is_exact(gpu, prec):
     """Integers below 2^24 split exactly into three bfloat16 parts, and the

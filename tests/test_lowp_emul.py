@@ -26,7 +26,7 @@ def test_fp16x2_split_and_scale():
     then hold 22 significant bits, so dropping x1 y1 leaves about 2^-22."""
     assert le.fp16x2_exponent(np.array([0.75, -0.1])) == 15 and le.fp16x2_exponent(np.array([1.0])) == 14
     assert le.fp16x2_exponent(np.array([3.0e38])) == 15 - 128 and le.fp16x2_exponent(np.zeros(3)) == 0
-    assert le.fp16x2_exponent(np.array([1.0, np.inf])) == 0
+    assert le.fp16x2_exponent(np.array([1.0, np.inf, np.nan])) == 14
     rng = np.random.default_rng(3)
     for mag in (1e-30, 1e-3, 1.0, 1e5, 1e30):
         x = (rng.uniform(-1, 1, 10000) * mag).astype(np.float32)

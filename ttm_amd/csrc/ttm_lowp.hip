@@ -142,9 +142,9 @@ struct Lp<LP_FP16X2>
 
 // FP16X2 scaling: every element of A is multiplied by 2^ea and every element
 // of B by 2^eb before the split, with e = 15 - E for the operand's largest
-// magnitude f * 2^E (f in [0.5, 1)), so the scaled operand lies below 2^15
-// (binary16 reaches 65504); the product is multiplied by 2^-(ea + eb).  An
-// all-zero or non-finite maximum gives e = 0.
+// finite magnitude f * 2^E (f in [0.5, 1)), so the scaled operand lies below
+// 2^15 (binary16 reaches 65504); the product is multiplied by 2^-(ea + eb).
+// An operand with no finite non-zero value gives e = 0.
 __device__ __forceinline__ int fp16x2_exponent(float amax)
 {
     if (!(amax > 0.f) || !__builtin_isfinite(amax)) return 0;
@@ -628,8 +628,9 @@ __global__ __launch_bounds__(256, MODE == LP_FP16X2 ? 3 : 2) void k_lowp(const f
 // FP16X2 scales: the largest magnitude of A (blocks [0, blocks_a)) and of B
 // (the remaining blocks), one partial maximum per block, then one block
 // turns both into exponents (no atomics: the result does not depend on the
-// order of the blocks).  NaNs are skipped by fmaxf; they stay NaN in the
-// product through the conversion.
+// order of the blocks).  Infinities and NaNs set no scale (an infinity in
+// one place must not flush every finite value); they stay what they are in
+// the product through the conversion.
 __device__ __forceinline__ float block_max(float v, float *red)
 {
 #pragma unroll
@@ -659,11 +660,18 @@ __global__ __launch_bounds__(256) void k_absmax_partial(const float *__restrict_
         uint64_t const nv = n / 4;
         for (uint64_t i = blk * 256 + threadIdx.x; i < nv; i += stride) {
             f32x4 const v = ldg_nt<f32x4>(x + 4 * i);
-            mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))));
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float const t = fabsf(v[e]);
+                mx = fmaxf(mx, t < __builtin_inff() ? t : 0.f);
+            }
         }
         vec_end = nv * 4;
     }
-    for (uint64_t i = vec_end + blk * 256 + threadIdx.x; i < n; i += stride) mx = fmaxf(mx, fabsf(x[i]));
+    for (uint64_t i = vec_end + blk * 256 + threadIdx.x; i < n; i += stride) {
+        float const t = fabsf(x[i]);
+        mx = fmaxf(mx, t < __builtin_inff() ? t : 0.f);
+    }
     mx = block_max(mx, red);
     if (threadIdx.x == 0) part[blockIdx.x] = mx;
 }
