@@ -424,6 +424,48 @@ def test_host_entry_staging_of_large_buffers(gpu):
     assert np.max(np.abs(got - ref)) <= 1e-3
 
 
+def test_host_entry_staging_from_concurrent_threads(gpu):
+    """Four threads call the synchronous host entry at once (ctypes releases
+    the GIL), each with its own 40 MiB A, so their pageable buffers go through
+    the shared pinned staging buffers together: every result must equal the
+    same product computed alone."""
+    import threading
+    import ttm_amd
+    na, pia, q, m = [160, 256, 256], [1, 2, 3], 2, 24
+    nc = list(na)
+    nc[q - 1] = m
+    wa, wc = ttm_amd.dense_strides(na, pia), ttm_amd.dense_strides(nc, pia)
+    rng = np.random.default_rng(31)
+    inputs = [(rng.uniform(-1, 1, int(np.prod(na))).astype(np.float32),
+               rng.uniform(-1, 1, m * na[q - 1]).astype(np.float32)) for _ in range(4)]
+
+    def product(a, b):
+        c = np.full(int(np.prod(nc)), np.nan, dtype=np.float32)
+        ttm_amd.ttm_raw(q, 3, a, na, wa, pia, b, [m, na[q - 1]], [2, 1], c, nc, wc)
+        return c
+
+    alone = [product(a, b) for a, b in inputs]
+    together = [None] * len(inputs)
+    errors = []
+
+    def work(i):
+        try:
+            for _ in range(3):
+                together[i] = product(*inputs[i])
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(len(inputs))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors, errors
+    assert not any(t.is_alive() for t in threads)
+    for i in range(len(inputs)):
+        np.testing.assert_array_equal(together[i], alone[i])
+
+
 @pytest.mark.parametrize("program", ["interfaces", "device_tensor"])
 def test_cpp_dropin_program_runs(gpu, program):
     """tests/cpp/interfaces.cpp: the three call forms + error texts, on the GPU.
