@@ -607,8 +607,12 @@ __global__ __launch_bounds__(256) void k_lowp_split_b(const float *__restrict__ 
 // per block) runs three blocks per CU: its register budget drops to 168 (a
 // few values spill), which measured 3.85-4.54 ms against 4.22-4.85 ms with
 // two blocks on the 1024-row float32 products of A 1024x1024x512.
+// Blocks per CU the register budget is set for: three (168 registers) where
+// the tile fits or nearly fits them, two (256) for the 128-row BF16X3 tile.
+constexpr int lp_blocks_per_cu(int mode, int bm) { return mode == LP_BF16X3 && bm == 128 ? 2 : 3; }
+
 template <int MODE, bool TN, int BM>
-__global__ __launch_bounds__(256, MODE == LP_FP16X2 ? 3 : 2) void k_lowp(const float *__restrict__ a, const typename Lp<MODE>::op *__restrict__ bs,
+__global__ __launch_bounds__(256, lp_blocks_per_cu(MODE, BM)) void k_lowp(const float *__restrict__ a, const typename Lp<MODE>::op *__restrict__ bs,
                                                  uint64_t ldb, uint64_t plane, float *__restrict__ c, uint64_t ncols,
                                                  uint64_t nq, uint64_t w, uint64_t m, unsigned nj, unsigned total,
                                                  const int *__restrict__ exps)
@@ -696,7 +700,7 @@ __global__ __launch_bounds__(256) void k_absmax_final(const float *__restrict__ 
 // Many TT products (stages of multi-TTM chains) in one persistent launch;
 // BM rows (128, or 64 for products of at most 64 rows) by 128 columns.
 template <int MODE, int BM>
-__global__ __launch_bounds__(256, 2) void k_lowp_grouped(const GroupProblem *__restrict__ probs,
+__global__ __launch_bounds__(256, lp_blocks_per_cu(MODE, BM)) void k_lowp_grouped(const GroupProblem *__restrict__ probs,
                                                          const unsigned *__restrict__ tile_prob, unsigned total)
 {
     using op = typename Lp<MODE>::op;
