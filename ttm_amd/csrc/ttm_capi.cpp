@@ -72,7 +72,9 @@ unsigned copy_threads()
     return std::max(1u, std::min(8u, hc ? hc : 1u));
 }
 
-// Copy n bytes with several host threads (pieces page-aligned).
+// Copy n bytes with several host threads (pieces page-aligned).  A piece
+// whose thread cannot be started is copied by the calling thread (nothing
+// may throw out of the C entry points).
 void parallel_copy(char *dst, const char *src, size_t n)
 {
     unsigned const nt = copy_threads();
@@ -82,15 +84,21 @@ void parallel_copy(char *dst, const char *src, size_t n)
     }
     size_t per = (n + nt - 1) / nt;
     per = (per + 4095) & ~size_t(4095);
-    std::vector<std::thread> threads;
+    std::thread threads[8];
+    unsigned started = 0;
     for (unsigned t = 1; t < nt; ++t) {
         size_t const off = (size_t)t * per;
         if (off >= n) break;
         size_t const len = std::min(per, n - off);
-        threads.emplace_back([=] { std::memcpy(dst + off, src + off, len); });
+        try {
+            threads[started] = std::thread([=] { std::memcpy(dst + off, src + off, len); });
+            ++started;
+        } catch (...) {
+            std::memcpy(dst + off, src + off, len);
+        }
     }
     std::memcpy(dst, src, std::min(per, n));
-    for (auto &th : threads) th.join();
+    for (unsigned t = 0; t < started; ++t) threads[t].join();
 }
 
 // Set up the pinned buffers, events and stream on the current device (with
@@ -116,10 +124,17 @@ bool stager_ready(Stager &st)
             st.pin[b] = nullptr;
             return false;
         }
+    st.stream = nullptr;
+    st.ev[0] = st.ev[1] = nullptr;
     if (hipStreamCreate(&st.stream) != hipSuccess ||
         hipEventCreateWithFlags(&st.ev[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&st.ev[1], hipEventDisableTiming) != hipSuccess) {
         (void)hipGetLastError();
+        if (st.ev[1]) (void)hipEventDestroy(st.ev[1]);
+        if (st.ev[0]) (void)hipEventDestroy(st.ev[0]);
+        if (st.stream) (void)hipStreamDestroy(st.stream);
+        st.stream = nullptr;
+        st.ev[0] = st.ev[1] = nullptr;
         return false;
     }
     st.device = dev;

@@ -18,6 +18,7 @@
 #include "ttm_gemm.hpp"
 #include "ttm_lowp.hpp"
 
+#include <atomic>
 #include <cstring>
 
 namespace ttm_amd
@@ -556,23 +557,25 @@ int tn_lds_spb_max(const Fold &f, bool is_double)
 constexpr int tn_lds_waves(int spb_max) { return spb_max <= 257 ? 8 : 3; }
 bool tn_lds_ok(const Fold &f, bool is_double) { return f.m <= 16 && tn_lds_spb_max(f, is_double) > 0; }
 
+// Compute units of the current device (queried once per device; concurrent
+// first calls store the same value).
 int device_cus()
 {
-    static int cus[64] = {0};
+    static std::atomic<int> cus[64] = {};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
         (void)hipGetLastError();
         return 256;
     }
-    if (cus[dev] == 0) {
-        int n = 0;
+    int n = cus[dev].load(std::memory_order_relaxed);
+    if (n == 0) {
         if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) {
             (void)hipGetLastError();
             n = 256;
         }
-        cus[dev] = n;
+        cus[dev].store(n, std::memory_order_relaxed);
     }
-    return cus[dev];
+    return n;
 }
 
 template <class T, int SPB_MAX>
