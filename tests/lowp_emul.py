@@ -5,10 +5,13 @@ Written from the format definitions, independent of the GPU code:
 * binary16: numpy's float16 conversion (round-to-nearest-even);
 * BF16X3: x = x0 + x1 + x2 with x_k = bf16(x - x0 - .. - x_{k-1}) and the six
   products of total order <= 2 (Henry, Tang & Heinecke, ARITH 2019);
-* FP16X2: each operand scaled by one power of two 2^(15 - E), where f * 2^E
-  (f in [0.5, 1)) is its largest finite magnitude (none non-zero: 2^0),
-  then x = x0 + x1 with x0 = fp16(x), x1 = fp16(x - x0), and the three
-  products x0 y0 + x0 y1 + x1 y0, scaled back by the two powers of two;
+* FP16X2: each operand scaled by a power of two 2^(15 - E), where f * 2^E
+  (f in [0.5, 1)) is the largest finite magnitude it is taken over (none
+  non-zero: 2^0), then x = x0 + x1 with x0 = fp16(x), x1 = fp16(x - x0),
+  and the three products x0 y0 + x0 y1 + x1 y0, scaled back by the powers of
+  two.  B takes one scale; A one scale in total, or (the tiled kernel's
+  64-row blocks) one per block of 128 columns and slice of 32 contraction
+  indices: the running minimum of the exponents of the block's slices so far;
 * OCP FP8 e4m3fn: 1 sign, 4 exponent (bias 7), 3 mantissa bits, no
   infinities, max 448, subnormal step 2^-9; round-to-nearest-even; with one
   power-of-two scale 2^e per block of 32 consecutive contraction indices,
@@ -105,10 +108,36 @@ def fp8_block_quantum(x, axis):
     return np.moveaxis(out, -1, axis)
 
 
-def emulate(prec, A3, Bm):
+def _fp16x2_sliced(A3, Bm, cols=128, bk=32):
+    """FP16X2 with A scaled per block of `cols` fused columns n = k*w + i and
+    slice of `bk` contraction indices (running minimum of the exponents)."""
+    outer, nq, w = A3.shape
+    eb = fp16x2_exponent(Bm)
+    b0, b1 = fp16x2_parts(Bm, eb)
+    acol = A3.transpose(0, 2, 1).reshape(outer * w, nq)
+    out = np.zeros((outer * w, Bm.shape[0]))
+    for c0 in range(0, outer * w, cols):
+        ea = None
+        for l0 in range(0, nq, bk):
+            blk = acol[c0:c0 + cols, l0:l0 + bk]
+            fin = np.abs(blk.astype(np.float64))
+            fin = fin[np.isfinite(fin)]
+            if fin.size and fin.max() > 0:
+                e = 15 - int(np.frexp(float(fin.max()))[1])
+                ea = e if ea is None else min(ea, e)
+            es = 0 if ea is None else ea
+            a0, a1 = fp16x2_parts(blk, es)
+            p0, p1 = b0[:, l0:l0 + bk], b1[:, l0:l0 + bk]
+            out[c0:c0 + cols] += (a0 @ p0.T + a1 @ p0.T + a0 @ p1.T) * 2.0 ** -(es + eb)
+    return out.reshape(outer, w, -1).transpose(0, 2, 1)
+
+
+def emulate(prec, A3, Bm, sliced=False):
     """C3[k][j][i] = sum_l B(j,l) A3[k][l][i] with the operands as `prec`
     rounds them, products and sums in float64 (reference for the GPU
-    result up to FP32 accumulation error).  A3: (outer, nq, w), Bm: (m, nq)."""
+    result up to FP32 accumulation error).  A3: (outer, nq, w), Bm: (m, nq).
+    `sliced` selects FP16X2's per-block and per-slice scale of A (the tiled
+    kernel's 64-row blocks) instead of one scale for all of A."""
     A3 = np.asarray(A3, dtype=np.float32)
     Bm = np.asarray(Bm, dtype=np.float32)
     if prec == "bf16":
@@ -122,6 +151,8 @@ def emulate(prec, A3, Bm):
         for p, q in ((0, 0), (0, 1), (1, 0), (0, 2), (1, 1), (2, 0)):
             total = total + np.einsum("jl,kli->kji", b[p], a[q])
         return total
+    if prec == "fp16x2" and sliced:
+        return _fp16x2_sliced(A3, Bm)
     if prec == "fp16x2":
         ea, eb = fp16x2_exponent(A3), fp16x2_exponent(Bm)
         a = fp16x2_parts(A3, ea)

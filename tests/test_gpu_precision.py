@@ -64,9 +64,9 @@ def case(rng, na, pia, q, m, colmajor, dist):
     return a, wa, Bm, b, [m, na[q - 1]], ([1, 2] if colmajor else [2, 1]), nc, wc
 
 
-def check_vs_emulation(c, a, na, pia, q, Bm, prec, label):
+def check_vs_emulation(c, a, na, pia, q, Bm, prec, label, sliced=False):
     A3 = folded(a, na, pia, q)
-    ref = le.emulate(prec, A3, Bm)                       # (outer, m, w)
+    ref = le.emulate(prec, A3, Bm, sliced=sliced)        # (outer, m, w)
     mag = {"bf16": lambda x: np.abs(le.bf16(x)), "fp16": lambda x: np.abs(le.fp16(x)),
            "bf16x3": lambda x: np.abs(x.astype(np.float64)) * 1.001,
            "fp16x2": lambda x: np.abs(x.astype(np.float64)) * 1.001,
@@ -99,6 +99,8 @@ def errors_vs_fp64(c, c64):
 @pytest.mark.parametrize("prec", PRECS)
 def test_variant_matches_its_emulation(gpu, prec):
     import torch
+    import ttm_amd
+    from ttm_amd._lib import PRECISION_NAMES
     rng = np.random.default_rng(11)
     shapes = [([64, 48, 32], [1, 2, 3], 2, 40), ([16, 24, 20], [3, 2, 1], 3, 64), ([40, 12, 8], [1, 2, 3], 1, 33),
               ([8, 36, 10], [2, 1, 3], 2, 130), ([20, 8, 5], [1, 3, 2], 2, 7), ([13, 7, 9], [1, 2, 3], 2, 5),
@@ -111,9 +113,13 @@ def test_variant_matches_its_emulation(gpu, prec):
                 a, wa, Bm, b, nb, pib, nc, wc = case(rng, na, pia, q, m, colmajor, dist)
                 for kernel in (None, "lowp_generic"):
                     c = run(torch, gpu, q, a, na, wa, pia, b, nb, pib, nc, wc, prec, kernel)
+                    # the family that ran: FP16X2's tiled kernel scales A per block and
+                    # slice in its 64-row blocks (at most 64 rows of B)
+                    fam = kernel or ttm_amd.plan_kernel(q, na, pia, nb, pib, precision=int(PRECISION_NAMES[prec]))
                     # reorder C into the folded (outer, m, w) order of its layout
                     c3 = folded(c, nc, pia, q)
-                    check_vs_emulation(c3, a, na, pia, q, Bm, prec, f"{prec} {na} {pia} q={q} m={m} {dist} {kernel}")
+                    check_vs_emulation(c3, a, na, pia, q, Bm, prec, f"{prec} {na} {pia} q={q} m={m} {dist} {fam}",
+                                       sliced=fam == "lowp" and m <= 64)
 
 
 def test_precision_error_table_64x48x32(gpu, oracle):
@@ -165,7 +171,7 @@ def test_fp16x2_infinity_sets_no_scale(gpu):
         c3 = folded(c, nc, pia, q).astype(np.float64)
         A3 = folded(a, na, pia, q)
         with np.errstate(invalid="ignore"):
-            ref = le.emulate("fp16x2", A3, Bm)
+            ref = le.emulate("fp16x2", A3, Bm, sliced=kernel is None)  # m = 40: the 64-row blocks
         touched = np.einsum("jl,kli->kji", np.ones(Bm.shape), np.isinf(A3).astype(np.float64)) > 0
         assert touched.sum() == m and not np.isfinite(c3[touched]).any(), kernel
         A3f = np.where(np.isinf(A3), 0, A3).astype(np.float64)

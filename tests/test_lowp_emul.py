@@ -57,9 +57,27 @@ def test_fp16x2_product_error_is_fp32_level():
             A3 = A3 * np.exp2(rng.integers(-30, 30, A3.shape))
         A3, B = A3.astype(np.float32), B.astype(np.float32)
         exact = np.einsum("jl,kli->kji", B.astype(np.float64), A3.astype(np.float64))
-        got = le.emulate("fp16x2", A3, B)
-        err = np.linalg.norm(got - exact) / (np.linalg.norm(A3.astype(np.float64)) * np.linalg.norm(B.astype(np.float64)))
-        assert err < 1e-6, (wide, err)
+        for sliced in (False, True):
+            got = le.emulate("fp16x2", A3, B, sliced=sliced)
+            err = np.linalg.norm(got - exact) / (np.linalg.norm(A3.astype(np.float64)) *
+                                                 np.linalg.norm(B.astype(np.float64)))
+            assert err < 1e-6, (wide, sliced, err)
+
+
+def test_fp16x2_slice_scales_follow_blocks_and_slices():
+    """The per-slice rule: a block whose later slice holds a larger value
+    lowers its exponent from that slice on (earlier partial sums rescaled
+    exactly), other blocks keep theirs; the result equals the float64
+    product for data the parts represent exactly (small integers times
+    powers of two), where one scale for all of A loses a tiny value."""
+    rng = np.random.default_rng(5)
+    A3 = rng.integers(-8, 9, (1, 96, 300)).astype(np.float32)   # 300 columns: blocks of 128, 128, 44
+    A3[0, 70, 5] = 2.0 ** 20           # block 0, slice 2: its exponent drops there
+    A3[0, 10, 200] = 2.0 ** -30        # block 1: kept by the binary16 parts of its own scale
+    B = rng.integers(-8, 9, (5, 96)).astype(np.float32)
+    exact = np.einsum("jl,kli->kji", B.astype(np.float64), A3.astype(np.float64))
+    np.testing.assert_array_equal(le.emulate("fp16x2", A3, B, sliced=True), exact)
+    assert not np.array_equal(le.emulate("fp16x2", A3, B), exact)
 
 
 def test_e4m3fn_known_answers():
@@ -91,3 +109,4 @@ def test_emulation_matches_float64_for_exact_inputs():
     exact = np.einsum("jl,kli->kji", B.astype(np.float64), A3.astype(np.float64))
     for prec in ("bf16", "fp16", "bf16x3", "fp8", "fp16x2"):
         np.testing.assert_array_equal(le.emulate(prec, A3, B), exact, err_msg=prec)
+    np.testing.assert_array_equal(le.emulate("fp16x2", A3, B, sliced=True), exact)

@@ -234,7 +234,7 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                                           float *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w, uint64_t m,
                                           uint64_t j0, uint64_t n0, unsigned flags,
                                           typename Lp<MODE>::op (*sA)[LP_BN * LP_LDK],
-                                          typename Lp<MODE>::op (*sB)[BM * LP_LDK], int ea = 0, int ce = 0)
+                                          typename Lp<MODE>::op (*sB)[BM * LP_LDK], int ea_in = 0, int ce = 0)
 {
     // BM = 128: wave wm owns the 64-row group wm (MFMA tiles t = 0..3);
     // BM = 64 (products of at most 64 rows): both waves along m share the one
@@ -259,6 +259,18 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     int const lane = tid & 63, wave = tid >> 6;
     int const lc = lane & 15, lg = lane >> 4;
     int const wm = wave & 1, wn = wave >> 1;
+    // FP16X2 scale of A.  128-row blocks take one exponent for all of A
+    // (ea_in, from a pass over A before the product).  64-row blocks (at
+    // most 64 rows of B: HBM-bound products, where that pass would read A a
+    // second time) take one per 128-column block and 32-wide slice: the
+    // running minimum over the block's slices so far of 15 - E(largest finite
+    // |A| of the slice), so every scaled value stays below 2^15; the
+    // accumulators are rescaled exactly (by 2^(new - old) <= 1) when it drops.
+    constexpr bool SLICE_SCALE = MODE == LP_FP16X2 && BM == 64;
+    constexpr int NO_EXP = 1 << 20; // no finite non-zero value seen yet
+    __shared__ float smax[4];
+    int ea = SLICE_SCALE ? NO_EXP : ea_in;
+    auto a_exp = [&]() { return ea == NO_EXP ? 0 : ea; };
 
     // ---- loaders --------------------------------------------------------
     // "MN" source (NN A): a thread loads a 4 (k) x 4 (row) micro-block with
@@ -381,8 +393,8 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
             f32x2 x01 = {ra[S][0][x], ra[S][1][x]};
             f32x2 x23 = {ra[S][2][x], ra[S][3][x]};
             if constexpr (MODE == LP_FP16X2) {
-                x01 = scale2(x01, ea);
-                x23 = scale2(x23, ea);
+                x01 = scale2(x01, a_exp());
+                x23 = scale2(x23, a_exp());
             }
             L::split2(x01, lo);
             L::split2(x23, hi);
@@ -396,7 +408,7 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 f32x2 x = {ra[S][2 * h + u / 2][2 * (u % 2)], ra[S][2 * h + u / 2][2 * (u % 2) + 1]};
-                if constexpr (MODE == LP_FP16X2) x = scale2(x, ea);
+                if constexpr (MODE == LP_FP16X2) x = scale2(x, a_exp());
                 L::split2(x, q[u]);
             }
             int const off = lds_at(k_p + 64 * h, k_c);
@@ -482,27 +494,75 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     };
 
     uint64_t const nslices = (nq + BK - 1) / BK;
+
+    // SLICE_SCALE: the block-wide maximum of a slice, reduced at the barrier
+    // the loop already has (smax is read after that barrier and written again
+    // only after the next one).
+    auto slice_max = [&]() {
+        float mx = 0.f;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                float const v = fabsf(ra[0][e][x]);
+                mx = fmaxf(mx, v < __builtin_inff() ? v : 0.f); // infinities and NaNs set no scale
+            }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        if (lane == 0) smax[wave] = mx;
+    };
+    auto update_scale = [&]() {
+        float const m = fmaxf(fmaxf(smax[0], smax[1]), fmaxf(smax[2], smax[3]));
+        if (m > 0.f) {
+            int E = 0;
+            (void)frexpf(m, &E);
+            int const e = 15 - E;
+            if (e < ea) {
+                if (ea != NO_EXP) {
+#pragma unroll
+                    for (int jt = 0; jt < JTL; ++jt)
+#pragma unroll
+                        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) acc[jt][nt][r] = ldexpf(acc[jt][nt][r], e - ea);
+                }
+                ea = e;
+            }
+        }
+    };
+
     // one LDS buffer: store, barrier, prefetch the next slice, multiply,
     // barrier (two workgroups per CU overlap each other's phases)
     auto kloop = [&](auto full_tag) {
         load(0, full_tag, I0());
+        if constexpr (SLICE_SCALE) {
+            slice_max();
+            __syncthreads();
+            update_scale();
+        }
         for (uint64_t s = 0; s < nslices; ++s) {
             store_all(I0(), sA, sB);
             __syncthreads();
             if (s + 1 < nslices) load((s + 1) * BK, full_tag, I0());
             multiply(sA, sB);
+            if constexpr (SLICE_SCALE)
+                if (s + 1 < nslices) slice_max();
             __syncthreads();
+            if constexpr (SLICE_SCALE)
+                if (s + 1 < nslices) update_scale();
         }
     };
     if (full) kloop(std::true_type());
     else kloop(std::false_type());
     if constexpr (MODE == LP_FP16X2) {
+        // ce = ea_in + eb; with slice scales ea_in is 0 and A's exponent is ea
+        int const cfin = SLICE_SCALE ? ce + a_exp() : ce;
 #pragma unroll
         for (int jt = 0; jt < JTL; ++jt)
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc[jt][nt][r] = ldexpf(acc[jt][nt][r], -ce);
+                for (int r = 0; r < 4; ++r) acc[jt][nt][r] = ldexpf(acc[jt][nt][r], -cfin);
     }
 
     // ---- epilogue ------------------------------------------------------------
@@ -977,14 +1037,14 @@ struct ScalePass
     unsigned blocks_a = 0, blocks = 0;
     size_t bytes() const { return (size_t)blocks * sizeof(float) + 2 * sizeof(int); }
 };
-ScalePass scale_pass(const Fold &f)
+ScalePass scale_pass(const Fold &f, bool with_a)
 {
     auto nblk = [](uint64_t n, uint64_t cap) {
         uint64_t const b = (n / 4 + 255) / 256;
         return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
     };
     ScalePass sp;
-    sp.blocks_a = nblk(f.numel_a(), 2048);
+    sp.blocks_a = with_a ? nblk(f.numel_a(), 2048) : 0;
     sp.blocks = sp.blocks_a + nblk(f.numel_b(), 64);
     return sp;
 }
@@ -1023,7 +1083,8 @@ ttm_status_t launch_lp(const Fold &f, const float *a, const float *b, float *c, 
         return TTM_ERR_ARGUMENT;
     }
     size_t const planes_bytes = (P * plane * sizeof(op) + 255) / 256 * 256;
-    ScalePass const sp = SCALED ? scale_pass(f) : ScalePass();
+    // 64-row blocks scale A per block and slice themselves: B's exponent only
+    ScalePass const sp = SCALED ? scale_pass(f, bm != 64) : ScalePass();
     hipMemPool_t const pool = workspace_pool();
     void *ws = nullptr;
     if (!pool || hipMallocFromPoolAsync(&ws, planes_bytes + (SCALED ? sp.bytes() : 0), pool, s) != hipSuccess) {
@@ -1106,7 +1167,7 @@ ttm_status_t launch_lowp(const Fold &f, const float *a, const float *b, float *c
         void *ws = nullptr;
         int const *exps = nullptr;
         if (prec == TTM_PREC_FP16X2) {
-            ScalePass const sp = scale_pass(f);
+            ScalePass const sp = scale_pass(f, true);
             hipMemPool_t const pool = workspace_pool();
             if (!pool || hipMallocFromPoolAsync(&ws, sp.bytes(), pool, s) != hipSuccess) {
                 (void)hipGetLastError();
