@@ -393,15 +393,46 @@ __global__ __launch_bounds__(WAVES * 64) void k_tn_lds(const T *__restrict__ a, 
     int const lc = lane & 15;
     int const lr = lane >> 4;
 
-    // B rows 0..15, spb pieces each (zero past m and past nq)
+    // B rows 0..15, spb pieces each (zero past m and past nq).  A thread
+    // issues up to 16 pieces' loads (16-byte vectors for a row-major B)
+    // before its LDS stores, and zeroes at store time, so the loads overlap.
+    // The row of piece idx is idx / spb, computed as a float reciprocal
+    // multiply (exact for idx < 16 * spb, spb = 32k + 1 <= 513).
     {
-        T *bsl_ = reinterpret_cast<T *>(bs);
-        uint32_t const rowlen = spb * VEC;
-        for (uint32_t idx = threadIdx.x; idx < 16u * rowlen; idx += WAVES * 64) {
-            uint32_t const j = idx / rowlen;
-            uint32_t const l = idx - j * rowlen;
-            bsl_[idx] = (j < m && l < nq) ? b[j * bsj + l * bsl] : T(0);
-        }
+        constexpr int NT = WAVES * 64;
+        constexpr int KMAX = (16 * SPB_MAX + NT - 1) / NT;
+        constexpr int KB = KMAX < 16 ? KMAX : 16;
+        uint32_t const npieces = 16u * spb;
+        float const inv = 1.0f / (float)spb;
+        uint64_t const nvec_b = nq / VEC;
+        auto fill = [&](auto rowvec_tag) {
+            constexpr bool ROWVEC = decltype(rowvec_tag)::value;
+            for (int k0 = 0; k0 < KMAX; k0 += KB) {
+                V tmp[KB];
+                bool ok[KB];
+#pragma unroll
+                for (int k = 0; k < KB; ++k) {
+                    uint32_t const idx = threadIdx.x + (uint32_t)(k0 + k) * NT;
+                    uint32_t const j = (uint32_t)(((float)idx + 0.5f) * inv);
+                    uint32_t const p = idx - j * spb;
+                    ok[k] = idx < npieces && j < m && p < nvec_b;
+                    uint64_t const jj = ok[k] ? j : 0, pp = ok[k] ? p : 0; // in-bounds addresses for every lane
+                    if constexpr (ROWVEC) {
+                        tmp[k] = *reinterpret_cast<const V *>(b + jj * bsj + pp * VEC);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < VEC; ++e) tmp[k][e] = b[jj * bsj + (pp * VEC + e) * bsl];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < KB; ++k) {
+                    uint32_t const idx = threadIdx.x + (uint32_t)(k0 + k) * NT;
+                    if (idx < npieces) bs[idx] = ok[k] ? tmp[k] : splat_zero<V>();
+                }
+            }
+        };
+        if (bsl == 1 && bsj % VEC == 0 && (reinterpret_cast<uintptr_t>(b) & 15u) == 0) fill(std::true_type());
+        else fill(std::false_type());
     }
     __syncthreads();
 
