@@ -284,8 +284,8 @@ def tucker_batch(args):
                        "A x1 U1 x2 U2 x3 U3, orthonormal rows, one hipGraph replay per step",
            "gflop": round(flops / 1e9, 2), "compulsory_gb": round(compulsory / 1e9, 3)}
     stream = torch.cuda.current_stream()
-    for prec in ("fp32", "bf16x3"):
-        plan = ChainPlan(items, precision=None if prec == "fp32" else "bf16x3")
+    for prec in ("fp32", "bf16x3", "fp16x2"):
+        plan = ChainPlan(items, precision=None if prec == "fp32" else prec)
         if prec == "fp32":
             plan.tune()
         plan.run_graph()

@@ -149,7 +149,8 @@ typedef struct ttm_chain_plan_s *ttm_chain_plan_t;
 
 /* Validate `count` descriptors and prepare a plan (allocates device memory for
  * descriptors, tile maps and intermediates).  precision: TTM_PREC_DEFAULT
- * (float or double as is_double says) or, for float, TTM_PREC_BF16X3. */
+ * (float or double as is_double says) or, for float, TTM_PREC_BF16X3 or
+ * TTM_PREC_FP16X2 (scaled per tile and slice of 32 contracted indices). */
 ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, int is_double,
                                    ttm_precision_t precision, ttm_chain_plan_t *plan);
 /* Run every chain of the plan: one persistent grouped launch per stage,

@@ -72,7 +72,7 @@ def check(items, refs, dtype, label, prec=None):
         err = np.linalg.norm(got - ref)
         if dtype == torch.float64:
             assert err <= 1e-12 * max(np.linalg.norm(ref), 1.0), (label, A.shape)
-        elif prec == "bf16x3":
+        elif prec in ("bf16x3", "fp16x2"):
             assert err <= 1e-6 * max(np.linalg.norm(ref), 1e-30) + 1e-12, (label, A.shape, err)
         else:
             nqs = sum(n for n, U in zip(A.shape, Us) if U is not None) + sum(A.shape)
@@ -80,14 +80,14 @@ def check(items, refs, dtype, label, prec=None):
             assert err <= 4 * nqs * eps * scale, (label, A.shape, err / scale)
 
 
-@pytest.mark.parametrize("dt", ["f64", "f32", "bf16x3"])
+@pytest.mark.parametrize("dt", ["f64", "f32", "bf16x3", "fp16x2"])
 def test_random_batch(gpu, dt):
     import torch
     from ttm_amd.chain import ChainPlan
     dtype = torch.float64 if dt == "f64" else torch.float32
-    rng = np.random.default_rng({"f64": 1, "f32": 2, "bf16x3": 3}[dt])
+    rng = np.random.default_rng({"f64": 1, "f32": 2, "bf16x3": 3, "fp16x2": 4}[dt])
     items, refs = random_items(torch, gpu, rng, 24, dtype)
-    plan = ChainPlan(items, precision="bf16x3" if dt == "bf16x3" else None)
+    plan = ChainPlan(items, precision=dt if dt in ("bf16x3", "fp16x2") else None)
     _say(dt, "stage kernels", plan.stage_kernels)
     plan.run()
     check(items, refs, dtype, f"{dt} run", prec=dt)
@@ -139,7 +139,7 @@ def orthonormal_rows(rng, r, n):
     return Q[:, :r].T.copy()  # r x n with orthonormal rows
 
 
-@pytest.mark.parametrize("prec", [None, "bf16x3"])
+@pytest.mark.parametrize("prec", [None, "bf16x3", "fp16x2"])
 def test_tucker_batch_reduced(gpu, prec):
     """Tucker chains A x1 U1 x2 U2 x3 U3 over 32 uneven first-order tensors (extents 16..64, ranks 4..16)."""
     import torch

@@ -114,10 +114,10 @@ ttm_status_t hip_check(hipError_t e)
 
 // Heuristic tile shape for a stage: rows just above the largest rank, wide
 // column blocks (the stages stream their input once).
-GroupConfig default_config(uint64_t max_m, bool bf16x3)
+GroupConfig default_config(uint64_t max_m, bool reduced)
 {
-    // the reduced-precision kernel has two shapes: 64 or 128 rows by 128 columns
-    if (bf16x3) return max_m <= 64 ? GroupConfig{2, 4, 2, 32} : GroupConfig{4, 4, 2, 32};
+    // the reduced-precision kernels have two shapes: 64 or 128 rows by 128 columns
+    if (reduced) return max_m <= 64 ? GroupConfig{2, 4, 2, 32} : GroupConfig{4, 4, 2, 32};
     if (max_m <= 16) return group_config(0);
     if (max_m <= 32) return group_config(1);
     if (max_m <= 64) return group_config(2);
@@ -160,7 +160,7 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
             begin += (unsigned)tiles;
         }
         st.total = begin;
-        int occ = plan->prec == TTM_PREC_BF16X3 ? lowp_group_occupancy(st.cfg.bm())
+        int occ = lowp_group_precision(plan->prec) ? lowp_group_occupancy(plan->prec, st.cfg.bm())
                                                 : group_occupancy(st.cfg, plan->is_double);
         if (occ <= 0) occ = 1;
         uint64_t const resident = (uint64_t)occ * (uint64_t)std::max(plan->num_cu, 1);
@@ -202,8 +202,8 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
 ttm_status_t launch_stage(ttm_chain_plan_t plan, const Stage &st, hipStream_t s)
 {
     if (st.total == 0) return TTM_SUCCESS;
-    if (plan->prec == TTM_PREC_BF16X3)
-        return launch_lowp_grouped(st.cfg.bm(), st.d_probs, st.d_tile_prob, st.total, st.grid, s);
+    if (lowp_group_precision(plan->prec))
+        return launch_lowp_grouped(plan->prec, st.cfg.bm(), st.d_probs, st.d_tile_prob, st.total, st.grid, s);
     if (plan->is_double)
         return launch_gemm_grouped<double>(st.cfg, st.d_probs, st.d_tile_prob, st.total, st.grid, s);
     return launch_gemm_grouped<float>(st.cfg, st.d_probs, st.d_tile_prob, st.total, st.grid, s);
@@ -227,8 +227,8 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
     if (!plan_out) return fail(TTM_ERR_ARGUMENT, "plan pointer is null");
     *plan_out = nullptr;
     if (count > 0 && !descs) return fail(TTM_ERR_ARGUMENT, "descriptor array is null");
-    if (precision != TTM_PREC_DEFAULT && (is_double || precision != TTM_PREC_BF16X3))
-        return fail(TTM_ERR_PRECISION, "chains run in the plain precision or, for float, BF16X3");
+    if (precision != TTM_PREC_DEFAULT && (is_double || !lowp_group_precision(precision)))
+        return fail(TTM_ERR_PRECISION, "chains run in the plain precision or, for float, BF16X3 or FP16X2");
     size_t const esz = is_double ? sizeof(double) : sizeof(float);
     size_t const vec = 16 / esz;
 
@@ -314,7 +314,7 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
     // Stage s of every chain, split by row class of m (<= 16, <= 32, <= 64,
     // larger) so that each launch gets a tile height that fits its ranks;
     // launches of one stage are independent, stages run in order.
-    int const classes = precision == TTM_PREC_BF16X3 ? 2 : 4;
+    int const classes = lowp_group_precision(precision) ? 2 : 4;
     auto m_class = [&](uint64_t m) {
         if (classes == 2) return m <= 64 ? 0 : 1; // the two reduced-precision tile heights
         return m <= 16 ? 0 : m <= 32 ? 1 : m <= 64 ? 2 : 3;
@@ -365,9 +365,9 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
                 << "/nq" << log2_class(stage_nq[g]) << "/cols" << log2_class(stage_cols[g]) << "/n"
                 << log2_class(st.probs.size());
             st.key = key.str();
-            st.cfg = default_config(stage_max_m[g], precision == TTM_PREC_BF16X3);
+            st.cfg = default_config(stage_max_m[g], lowp_group_precision(precision));
             auto it = g_tune_cache.find(st.key);
-            if (precision != TTM_PREC_BF16X3 && it != g_tune_cache.end()) st.cfg = group_config(it->second);
+            if (!lowp_group_precision(precision) && it != g_tune_cache.end()) st.cfg = group_config(it->second);
             plan->stages.push_back(std::move(st));
         }
     }
@@ -428,7 +428,7 @@ ttm_status_t ttm_chain_run_graph(ttm_chain_plan_t plan, ttm_stream_t stream)
 ttm_status_t ttm_chain_tune(ttm_chain_plan_t plan, ttm_stream_t stream)
 {
     if (!plan) return fail(TTM_ERR_ARGUMENT, "plan is null");
-    if (plan->prec == TTM_PREC_BF16X3) return TTM_SUCCESS; // one compiled shape
+    if (lowp_group_precision(plan->prec)) return TTM_SUCCESS; // one compiled shape per tile height
     drop_graph(plan);
     hipStream_t const s = static_cast<hipStream_t>(stream);
     hipEvent_t e0, e1;
@@ -482,7 +482,7 @@ unsigned ttm_chain_stages(ttm_chain_plan_t plan) { return plan ? (unsigned)plan-
 const char *ttm_chain_stage_kernel(ttm_chain_plan_t plan, unsigned s)
 {
     if (!plan || s >= plan->stages.size()) return "";
-    plan->symbol = plan->prec == TTM_PREC_BF16X3 ? lowp_group_symbol(plan->stages[s].cfg.bm())
+    plan->symbol = lowp_group_precision(plan->prec) ? lowp_group_symbol(plan->prec, plan->stages[s].cfg.bm())
                                                  : group_symbol(plan->stages[s].cfg, plan->is_double);
     return plan->symbol.c_str();
 }

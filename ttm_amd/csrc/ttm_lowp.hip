@@ -259,18 +259,23 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     int const lane = tid & 63, wave = tid >> 6;
     int const lc = lane & 15, lg = lane >> 4;
     int const wm = wave & 1, wn = wave >> 1;
-    // FP16X2 scale of A.  128-row blocks take one exponent for all of A
-    // (ea_in, from a pass over A before the product).  64-row blocks (at
-    // most 64 rows of B: HBM-bound products, where that pass would read A a
-    // second time) take one per 128-column block and 32-wide slice: the
-    // running minimum over the block's slices so far of 15 - E(largest finite
-    // |A| of the slice), so every scaled value stays below 2^15; the
-    // accumulators are rescaled exactly (by 2^(new - old) <= 1) when it drops.
-    constexpr bool SLICE_SCALE = MODE == LP_FP16X2 && BM == 64;
+    // FP16X2 scale of A.  128-row blocks of a single product take one
+    // exponent for all of A (ea_in, from a pass over A before the product).
+    // 64-row blocks (at most 64 rows of B: HBM-bound products, where that
+    // pass would read A a second time) and the grouped blocks take one per
+    // 128-column block and 32-wide slice: the running minimum over the
+    // block's slices so far of 15 - E(largest finite |A| of the slice), so
+    // every scaled value stays below 2^15; the accumulators are rescaled
+    // exactly (by 2^(new - old) <= 1) when it drops.  The grouped blocks
+    // convert B themselves and scale it the same way per block and slice.
+    constexpr bool SLICE_SCALE = MODE == LP_FP16X2 && (BM == 64 || !BSPLIT);
+    constexpr bool B_SLICE = MODE == LP_FP16X2 && !BSPLIT;
     constexpr int NO_EXP = 1 << 20; // no finite non-zero value seen yet
-    __shared__ float smax[4];
+    __shared__ float smax[2][4];
     int ea = SLICE_SCALE ? NO_EXP : ea_in;
+    int eb_s = NO_EXP;
     auto a_exp = [&]() { return ea == NO_EXP ? 0 : ea; };
+    auto b_exp = [&]() { return eb_s == NO_EXP ? 0 : eb_s; };
 
     // ---- loaders --------------------------------------------------------
     // "MN" source (NN A): a thread loads a 4 (k) x 4 (row) micro-block with
@@ -432,7 +437,8 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                     op2 q[4][P];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        f32x2 const x = {rb[S][2 * h + u / 2][2 * (u % 2)], rb[S][2 * h + u / 2][2 * (u % 2) + 1]};
+                        f32x2 x = {rb[S][2 * h + u / 2][2 * (u % 2)], rb[S][2 * h + u / 2][2 * (u % 2) + 1]};
+                        if constexpr (MODE == LP_FP16X2) x = scale2(x, b_exp());
                         L::split2(x, q[u]);
                     }
                     int const off = lds_at(k_p + 64 * h, k_c);
@@ -498,6 +504,11 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     // SLICE_SCALE: the block-wide maximum of a slice, reduced at the barrier
     // the loop already has (smax is read after that barrier and written again
     // only after the next one).
+    auto wave_max = [&](float mx) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        return mx;
+    };
     auto slice_max = [&]() {
         float mx = 0.f;
 #pragma unroll
@@ -507,27 +518,43 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                 float const v = fabsf(ra[0][e][x]);
                 mx = fmaxf(mx, v < __builtin_inff() ? v : 0.f); // infinities and NaNs set no scale
             }
+        mx = wave_max(mx);
+        if (lane == 0) smax[0][wave] = mx;
+        if constexpr (B_SLICE) {
+            float mb = 0.f;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-        if (lane == 0) smax[wave] = mx;
+            for (int e = 0; e < 2 * BH; ++e)
+#pragma unroll
+                for (int x = 0; x < 4; ++x) {
+                    float const v = fabsf(rb[0][e][x]);
+                    mb = fmaxf(mb, v < __builtin_inff() ? v : 0.f);
+                }
+            mb = wave_max(mb);
+            if (lane == 0) smax[1][wave] = mb;
+        }
+    };
+    // new exponent of one operand from its slice maximum; returns the drop
+    // (<= 0) to apply to the accumulators
+    auto lower = [&](float m, int &e_cur) {
+        if (!(m > 0.f)) return 0;
+        int E = 0;
+        (void)frexpf(m, &E);
+        int const e = 15 - E;
+        if (e >= e_cur) return 0;
+        int const d = e_cur == NO_EXP ? 0 : e - e_cur; // nothing accumulated yet in the old units
+        e_cur = e;
+        return d;
     };
     auto update_scale = [&]() {
-        float const m = fmaxf(fmaxf(smax[0], smax[1]), fmaxf(smax[2], smax[3]));
-        if (m > 0.f) {
-            int E = 0;
-            (void)frexpf(m, &E);
-            int const e = 15 - E;
-            if (e < ea) {
-                if (ea != NO_EXP) {
+        int d = lower(fmaxf(fmaxf(smax[0][0], smax[0][1]), fmaxf(smax[0][2], smax[0][3])), ea);
+        if constexpr (B_SLICE) d += lower(fmaxf(fmaxf(smax[1][0], smax[1][1]), fmaxf(smax[1][2], smax[1][3])), eb_s);
+        if (d != 0) {
 #pragma unroll
-                    for (int jt = 0; jt < JTL; ++jt)
+            for (int jt = 0; jt < JTL; ++jt)
 #pragma unroll
-                        for (int nt = 0; nt < 4; ++nt)
+                for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-                            for (int r = 0; r < 4; ++r) acc[jt][nt][r] = ldexpf(acc[jt][nt][r], e - ea);
-                }
-                ea = e;
-            }
+                    for (int r = 0; r < 4; ++r) acc[jt][nt][r] = ldexpf(acc[jt][nt][r], d);
         }
     };
 
@@ -555,8 +582,10 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     if (full) kloop(std::true_type());
     else kloop(std::false_type());
     if constexpr (MODE == LP_FP16X2) {
-        // ce = ea_in + eb; with slice scales ea_in is 0 and A's exponent is ea
-        int const cfin = SLICE_SCALE ? ce + a_exp() : ce;
+        // ce = ea_in + eb (B's exponent from the pass over B); with slice
+        // scales ea_in is 0 and A's exponent is ea, and the grouped blocks
+        // hold B's exponent themselves
+        int const cfin = B_SLICE ? a_exp() + b_exp() : (SLICE_SCALE ? ce + a_exp() : ce);
 #pragma unroll
         for (int jt = 0; jt < JTL; ++jt)
 #pragma unroll
@@ -759,8 +788,12 @@ __global__ __launch_bounds__(256) void k_absmax_final(const float *__restrict__ 
 
 // Many TT products (stages of multi-TTM chains) in one persistent launch;
 // BM rows (128, or 64 for products of at most 64 rows) by 128 columns.
+// The grouped blocks also convert B and scale it per slice: 128-row blocks
+// run two per CU.
+constexpr int lp_group_blocks_per_cu(int bm) { return bm == 128 ? 2 : 3; }
+
 template <int MODE, int BM>
-__global__ __launch_bounds__(256, lp_blocks_per_cu(MODE, BM)) void k_lowp_grouped(const GroupProblem *__restrict__ probs,
+__global__ __launch_bounds__(256, lp_group_blocks_per_cu(BM)) void k_lowp_grouped(const GroupProblem *__restrict__ probs,
                                                          const unsigned *__restrict__ tile_prob, unsigned total)
 {
     using op = typename Lp<MODE>::op;
@@ -1210,27 +1243,53 @@ ttm_status_t launch_lowp(const Fold &f, const float *a, const float *b, float *c
 namespace ttm_amd
 {
 
-int lowp_group_occupancy(int bm)
+bool lowp_group_precision(ttm_precision_t prec) { return prec == TTM_PREC_BF16X3 || prec == TTM_PREC_FP16X2; }
+
+namespace
+{
+const void *lowp_group_kernel(ttm_precision_t prec, int bm)
+{
+    if (prec == TTM_PREC_FP16X2)
+        return bm == 64 ? reinterpret_cast<const void *>(&k_lowp_grouped<LP_FP16X2, 64>)
+                        : reinterpret_cast<const void *>(&k_lowp_grouped<LP_FP16X2, 128>);
+    return bm == 64 ? reinterpret_cast<const void *>(&k_lowp_grouped<LP_BF16X3, 64>)
+                    : reinterpret_cast<const void *>(&k_lowp_grouped<LP_BF16X3, 128>);
+}
+} // namespace
+
+int lowp_group_occupancy(ttm_precision_t prec, int bm)
 {
     int blocks = 0;
-    const void *fn = bm == 64 ? reinterpret_cast<const void *>(&k_lowp_grouped<LP_BF16X3, 64>)
-                              : reinterpret_cast<const void *>(&k_lowp_grouped<LP_BF16X3, 128>);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, 256, 0) != hipSuccess) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, lowp_group_kernel(prec, bm), 256, 0) != hipSuccess) {
         (void)hipGetLastError();
         return 0;
     }
     return blocks;
 }
 
-std::string lowp_group_symbol(int bm) { return bm == 64 ? "k_lowp_grouped<2, 64>" : "k_lowp_grouped<2, 128>"; }
-
-ttm_status_t launch_lowp_grouped(int bm, const GroupProblem *d_probs, const unsigned *d_tile_prob, unsigned total,
-                                 unsigned grid, hipStream_t s)
+std::string lowp_group_symbol(ttm_precision_t prec, int bm)
 {
-    if (bm == 64)
-        hipLaunchKernelGGL((k_lowp_grouped<LP_BF16X3, 64>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
-    else
-        hipLaunchKernelGGL((k_lowp_grouped<LP_BF16X3, 128>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
+    return std::string("k_lowp_grouped<") + (prec == TTM_PREC_FP16X2 ? "3" : "2") + (bm == 64 ? ", 64>" : ", 128>");
+}
+
+ttm_status_t launch_lowp_grouped(ttm_precision_t prec, int bm, const GroupProblem *d_probs,
+                                 const unsigned *d_tile_prob, unsigned total, unsigned grid, hipStream_t s)
+{
+    if (prec == TTM_PREC_FP16X2) {
+        if (bm == 64)
+            hipLaunchKernelGGL((k_lowp_grouped<LP_FP16X2, 64>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob,
+                               total);
+        else
+            hipLaunchKernelGGL((k_lowp_grouped<LP_FP16X2, 128>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob,
+                               total);
+    } else {
+        if (bm == 64)
+            hipLaunchKernelGGL((k_lowp_grouped<LP_BF16X3, 64>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob,
+                               total);
+        else
+            hipLaunchKernelGGL((k_lowp_grouped<LP_BF16X3, 128>), dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob,
+                               total);
+    }
     hipError_t const e = hipGetLastError();
     if (e != hipSuccess) {
         set_error(TTM_ERR_HIP, hipGetErrorString(e));

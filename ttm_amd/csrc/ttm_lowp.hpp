@@ -1,4 +1,4 @@
-// Reduced-precision arithmetic (BF16, FP16, BF16X3, FP8 e4m3fn) for float32 products.
+// Reduced-precision arithmetic (BF16, FP16, BF16X3, FP16X2, FP8 e4m3fn) for float32 products.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,12 +23,14 @@ std::string lowp_symbol(const Fold &f, ttm_precision_t prec, bool generic);
 ttm_status_t launch_lowp(const Fold &f, const float *a, const float *b, float *c, ttm_precision_t prec,
                          hipStream_t s, bool generic);
 
-// Grouped BF16X3 launches of TT products (stages of multi-TTM chains) in
-// bm x 128 tiles, bm = 128 or 64 (products of at most 64 rows); tile lists as
-// for launch_gemm_grouped.
-int lowp_group_occupancy(int bm);
-std::string lowp_group_symbol(int bm);
-ttm_status_t launch_lowp_grouped(int bm, const GroupProblem *d_probs, const unsigned *d_tile_prob, unsigned total,
-                                 unsigned grid, hipStream_t s);
+// Grouped BF16X3 or FP16X2 launches of TT products (stages of multi-TTM
+// chains) in bm x 128 tiles, bm = 128 or 64 (products of at most 64 rows);
+// tile lists as for launch_gemm_grouped.  FP16X2 scales both operands per
+// tile and slice (no pass before the launch).
+bool lowp_group_precision(ttm_precision_t prec);
+int lowp_group_occupancy(ttm_precision_t prec, int bm);
+std::string lowp_group_symbol(ttm_precision_t prec, int bm);
+ttm_status_t launch_lowp_grouped(ttm_precision_t prec, int bm, const GroupProblem *d_probs,
+                                 const unsigned *d_tile_prob, unsigned total, unsigned grid, hipStream_t s);
 
 } // namespace ttm_amd
