@@ -4,7 +4,8 @@ A = G x_1 U_1 x_2 U_2 x_3 U_3 with orthonormal U_k (built in float64 with
 numpy.einsum, the textbook definition) has multilinear rank (r_1, r_2, r_3),
 so HOSVD at those ranks reproduces A to rounding: the bar is
 ||reconstruct(hosvd(A)) - A|| <= 1e-10 ||A|| in float64 and 1e-4 ||A|| in
-float32, and the core's norm equals ||A|| (orthonormal factors).
+float32 (also with the FP32-level reduced-precision chains BF16X3 and
+FP16X2), and the core's norm equals ||A|| (orthonormal factors).
 """
 import numpy as np
 import pytest
@@ -27,18 +28,19 @@ def to_layout(torch, dev, X, pi, dtype):
     return T.permute(*[int(i) for i in inv])
 
 
-@pytest.mark.parametrize("dt", ["f64", "f32"])
+@pytest.mark.parametrize("dt", ["f64", "f32", "bf16x3", "fp16x2"])
 def test_hosvd_recovers_low_rank_tensor(gpu, dt):
     import torch
     from ttm_amd.tucker import hosvd, reconstruct
     dtype = torch.float64 if dt == "f64" else torch.float32
+    prec = dt if dt in ("bf16x3", "fp16x2") else None
     rng = np.random.default_rng(11)
     n, r = (40, 33, 25), (6, 9, 4)
     A = low_rank(rng, n, r)
     At = to_layout(torch, gpu, A, [1, 2, 3], dtype)
-    core, facs = hosvd(At, r)
+    core, facs = hosvd(At, r, precision=prec)
     assert tuple(core.shape) == r and [tuple(f.shape) for f in facs] == [(6, 40), (9, 33), (4, 25)]
-    R = reconstruct(core, facs).double().cpu().numpy()
+    R = reconstruct(core, facs, precision=prec).double().cpu().numpy()
     err = np.linalg.norm(R - A) / np.linalg.norm(A)
     tol = 1e-10 if dt == "f64" else 1e-4
     print(f"hosvd {dt}: reconstruction rel err {err:.3e}", flush=True)
