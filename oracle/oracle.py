@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/libttm_oracle.so) -- TEST INFRASTRUCTURE ONLY.
 
 Imported only by tests/, ``__graft_entry__.smoke()`` and the cpu_baseline
-leg of bench.py, as the checker / CPU baseline; the product never uses it.
+leg of bench.py, as the reference result / CPU baseline; the product never uses it.
 Every function states the reference lines its C counterpart follows (see
 ttm_oracle.c).
 """

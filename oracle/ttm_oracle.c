@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
  * calls this file.  Only the test-suite, the smoke check and the
- * cpu_baseline leg of bench.py use it, and only as the checker / the CPU
+ * cpu_baseline leg of bench.py use it, and only as the reference result / the CPU
  * baseline, never as the thing being measured on the GPU.
  *
  * It is a plain-C restatement of the reference algorithm (bassoy/ttm,
