@@ -346,7 +346,7 @@ bool fold_dense(unsigned q, unsigned p, const size_t *na, const size_t *pia, con
                 Fold &f)
 {
     if (!na || !pia || !nb || !pib || p == 0 || p > 64 || q == 0 || q > p) return false;
-    size_t nc[64], wa[64], wc[64];
+    size_t nc[64] = {}, wa[64] = {}, wc[64] = {}; // modes a non-permutation pia skips stay 0 (rejected by plan)
     for (unsigned r = 0; r < p; ++r) nc[r] = na[r];
     nc[q - 1] = nb[0];
     size_t sa = 1, sc = 1;
