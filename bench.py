@@ -185,10 +185,14 @@ def gpu_bench(args, cases):
     return wall / args.steps, split
 
 
+OTHER_PRECISIONS = ("bf16", "fp16", "fp8")
+
+
 def bf16x3_vs_fp32(args, cases, A, C, stream):
-    """The compute-bound float32 products again in BF16X3 and FP16X2: time,
-    and the normwise relative error of each arithmetic against the float64
-    definition on 4096 sampled entries of C."""
+    """The compute-bound float32 products again in BF16X3 and FP16X2 (and, for
+    reference, plain BF16, FP16 and scaled FP8-e4m3): time, and the normwise
+    relative error of each arithmetic against the float64 definition on 4096
+    sampled entries of C."""
     import torch
     import ttm_amd
     rng = np.random.default_rng(17)
@@ -204,7 +208,7 @@ def bf16x3_vs_fp32(args, cases, A, C, stream):
                             precision=prec)
 
         res = {}
-        for prec in ("default", "bf16x3", "fp16x2"):
+        for prec in ("default", "bf16x3", "fp16x2") + OTHER_PRECISIONS:
             for _ in range(2):
                 run(prec)
             ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
@@ -236,7 +240,10 @@ def bf16x3_vs_fp32(args, cases, A, C, stream):
                      "fp16x2_normwise_err": res["fp16x2"][1],
                      "fp16x2_fp32equiv_tflops": round(c["flops"] / res["fp16x2"][0] / 1e9, 2),
                      # three fp16 products per FP32-equivalent product (plus the scale pass over A)
-                     "fp16x2_mfma_issued_tflops": round(3 * c["flops"] / res["fp16x2"][0] / 1e9, 2)})
+                     "fp16x2_mfma_issued_tflops": round(3 * c["flops"] / res["fp16x2"][0] / 1e9, 2),
+                     # the plain single-part arithmetics (errors at their format's level)
+                     "other": {pr: {"ms": round(res[pr][0], 4), "speedup": round(res["default"][0] / res[pr][0], 3),
+                                    "normwise_err": res[pr][1]} for pr in OTHER_PRECISIONS}})
     if not rows:
         return None
     sp = [r["speedup"] for r in rows]
@@ -246,6 +253,9 @@ def bf16x3_vs_fp32(args, cases, A, C, stream):
             "max_fp16x2_normwise_err": max(r["fp16x2_normwise_err"] for r in rows),
             "max_bf16x3_normwise_err": max(r["bf16x3_normwise_err"] for r in rows),
             "max_fp32_normwise_err": max(r["fp32_normwise_err"] for r in rows),
+            "other_speedup_geomean": {pr: round(float(np.exp(np.mean(np.log([r["other"][pr]["speedup"] for r in rows])))), 3)
+                                      for pr in OTHER_PRECISIONS},
+            "other_max_normwise_err": {pr: max(r["other"][pr]["normwise_err"] for r in rows) for pr in OTHER_PRECISIONS},
             "error_reference": "float64 definition on 4096 sampled entries", "cases": rows}
 
 

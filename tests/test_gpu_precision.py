@@ -73,9 +73,11 @@ def check_vs_emulation(c, a, na, pia, q, Bm, prec, label, sliced=False):
            "fp8": lambda x: np.abs(x.astype(np.float64)) * 1.2}[prec]
     # FP32 accumulation bound; the scaled FP8 instruction sums 128 products per
     # step inside the matrix core with undocumented intermediate rounding, so
-    # it gets a 4x wider (still ~1000x tighter than FP8 rounding) allowance.
+    # it gets a 4x wider (still ~1000x tighter than FP8 rounding) allowance,
+    # counted over the 128 terms of one instruction when n_q is shorter.
+    terms = max(na[q - 1], 128) if prec == "fp8" else na[q - 1]
     factor = 16 if prec == "fp8" else 4
-    bound = factor * na[q - 1] * 2.0 ** -24 * np.einsum("jl,kli->kji", mag(Bm), mag(A3)) + 1e-30
+    bound = factor * terms * 2.0 ** -24 * np.einsum("jl,kli->kji", mag(Bm), mag(A3)) + 1e-30
     if prec == "fp8":
         # plus the smallest normal (8 subnormal steps) per operand and product:
         # the matrix core may flush e4m3fn subnormal inputs, which this
@@ -106,7 +108,9 @@ def test_variant_matches_its_emulation(gpu, prec):
               ([8, 36, 10], [2, 1, 3], 2, 130), ([20, 8, 5], [1, 3, 2], 2, 7), ([13, 7, 9], [1, 2, 3], 2, 5),
               ([256, 4, 6], [3, 1, 2], 1, 17),
               # more than 128 rows of B: several row blocks (ragged rows, columns and slices)
-              ([12, 40, 24], [1, 2, 3], 2, 300), ([36, 20, 16], [1, 2, 3], 1, 260), ([64, 10, 14], [1, 2, 3], 1, 512)]
+              ([12, 40, 24], [1, 2, 3], 2, 300), ([36, 20, 16], [1, 2, 3], 1, 260), ([64, 10, 14], [1, 2, 3], 1, 512),
+              # NN view with n_q % 4 != 0: rows of a row-major B are not 16-byte aligned
+              ([8, 7, 12], [1, 2, 3], 2, 20)]
     for na, pia, q, m in shapes:
         for colmajor in (True, False):
             for dist in ("uniform", "loguniform"):
@@ -230,5 +234,55 @@ def test_split_variants_full_size_sampled(gpu, prec):
         rel = np.linalg.norm(got - expect) / np.linalg.norm(expect)
         _say(f"{prec} 1024x1024x512 pi={pia} q={q}: sampled normwise-rel {rel:.3e}")
         assert rel <= 1e-6
+        del C
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp8"])
+def test_plain_variants_full_size_sampled(gpu, prec):
+    """A 1024x1024x512 products (m = 1024; TN and NN views, row- and
+    column-major B) in the plain reduced precisions: 1024 sampled entries of C
+    against the emulation of the arithmetic on their fibres, to the bound of
+    check_vs_emulation."""
+    import torch
+    import ttm_amd
+    na = [1024, 1024, 512]
+    A = torch.empty(int(np.prod(na)), dtype=torch.float32, device=gpu).uniform_(-1, 1)
+    rng = np.random.default_rng(4)
+    for pia, q, colmajor in (([1, 2, 3], 1, False), ([1, 2, 3], 2, False), ([3, 2, 1], 3, True), ([3, 2, 1], 1, True)):
+        m, nq = 1024, na[q - 1]
+        Bt = torch.empty((m, nq), dtype=torch.float32, device=gpu).uniform_(-1, 1)
+        Bdev = Bt.t().contiguous().reshape(-1) if colmajor else Bt.reshape(-1)
+        wa = rc.dense_strides(na, pia)
+        nc = list(na)
+        nc[q - 1] = m
+        wc = rc.dense_strides(nc, pia)
+        C = torch.empty(int(np.prod(nc)), dtype=torch.float32, device=gpu)
+        ttm_amd.ttm_raw(q, 3, A, na, wa, pia, Bdev, [m, nq], [1, 2] if colmajor else [2, 1], C, nc, wc,
+                        precision=prec)
+        torch.cuda.synchronize()
+        idx = [rng.integers(0, nc[r], size=1024) for r in range(3)]
+        offc = sum(idx[r] * wc[r] for r in range(3))
+        base = sum(idx[r] * wa[r] for r in range(3) if r != q - 1)
+        fib_idx = torch.from_numpy((base[:, None] + np.arange(nq)[None, :] * wa[q - 1]).astype(np.int64))
+        a_f = A[fib_idx.to(gpu)].cpu().numpy()
+        b_r = Bt.cpu().numpy()[idx[q - 1]]
+        if prec == "fp8":
+            qa, qb = le.fp8_block_quantize(a_f, axis=1), le.fp8_block_quantize(b_r, axis=1)
+        else:
+            rnd = le.bf16 if prec == "bf16" else le.fp16
+            qa, qb = rnd(a_f), rnd(b_r)
+        expect = np.sum(qa * qb, axis=1)
+        magp = np.abs(qa) * np.abs(qb)
+        if prec == "fp8":
+            magp = np.abs(a_f.astype(np.float64)) * 1.2 * np.abs(b_r.astype(np.float64)) * 1.2
+        bound = (16 if prec == "fp8" else 4) * nq * 2.0 ** -24 * np.sum(magp, axis=1) + 1e-30
+        if prec == "fp8":
+            bound = bound + np.sum(8 * le.fp8_block_quantum(b_r, axis=1) * np.abs(a_f) * 1.2, axis=1) \
+                + np.sum(np.abs(b_r) * 1.2 * 8 * le.fp8_block_quantum(a_f, axis=1), axis=1)
+        got = C[torch.from_numpy(offc.astype(np.int64)).to(gpu)].double().cpu().numpy()
+        err = np.abs(got - expect)
+        _say(f"{prec} 1024x1024x512 pi={pia} q={q} colmajor={colmajor}: max err/bound {float(np.max(err / bound)):.3f}")
+        assert np.all(err <= bound), (prec, pia, q, colmajor, float(np.max(err / bound)))
         del C
     torch.cuda.empty_cache()

@@ -823,18 +823,72 @@ __device__ __forceinline__ int slice_exponent(float amax)
     return e < -126 ? -126 : (e > 127 ? 127 : e);
 }
 
-// 32 floats -> 32 e4m3fn bytes scaled by 2^-e (clamped: the conversion
+// 4 floats -> 4 e4m3fn bytes scaled by 2^-e (clamped: the conversion
 // instruction returns NaN beyond +-448 instead of saturating).
+__device__ __forceinline__ int quantize4(float x0, float x1, float x2, float x3, int e)
+{
+    float const v0 = fminf(fmaxf(ldexpf(x0, -e), -FP8_MAX), FP8_MAX);
+    float const v1 = fminf(fmaxf(ldexpf(x1, -e), -FP8_MAX), FP8_MAX);
+    float const v2 = fminf(fmaxf(ldexpf(x2, -e), -FP8_MAX), FP8_MAX);
+    float const v3 = fminf(fmaxf(ldexpf(x3, -e), -FP8_MAX), FP8_MAX);
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(v0, v1, 0, false);
+    return __builtin_amdgcn_cvt_pk_fp8_f32(v2, v3, w, true);
+}
+
+// 32 floats -> 32 e4m3fn bytes scaled by 2^-e.
 __device__ __forceinline__ void quantize32(const float (&x)[32], int e, i32x8 &out)
 {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        float v[4];
+    for (int q = 0; q < 8; ++q) out[q] = quantize4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3], e);
+}
+
+// Operand tiles whose 64 rows are contiguous along l (A in the TN view, a
+// row-major B; n_q % 4 == 0 and 16-byte aligned rows): each wave instruction
+// reads 2 rows x 512 bytes, 8 lanes per 32-wide scale slice, and the slice
+// maximum is reduced over those 8 lanes.  Slice c = row * 4 + kb of the tile
+// goes to lanes 8 * (c % 8) .. + 7 of wave c / 64 in step (c / 8) % 8; lane
+// s of the 8 holds l = l0 + 32 * kb + 4 * s .. + 3.  The bytes, scales and
+// LDS placement are those of the per-thread path (one row and slice per
+// thread): max is order-independent and each element is quantized alone.
+struct Fp8Rows
+{
+    int q[8]; // 4 e4m3fn bytes per step
+    int e[8]; // the slice's exponent per step
+};
+
+__device__ __forceinline__ void fp8_rows_load(const float *__restrict__ x, uint64_t row0, uint64_t nrows, uint64_t nq,
+                                              uint64_t l0, int tid, Fp8Rows &out)
+{
+    int const lane = tid & 63, wave = tid >> 6;
+    f32x4 v[8];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) v[t] = fminf(fmaxf(ldexpf(x[4 * q + t], -e), -FP8_MAX), FP8_MAX);
-        int w = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
-        w = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], w, true);
-        out[q] = w;
+    for (int i = 0; i < 8; ++i) {
+        int const c = (wave * 8 + i) * 8 + (lane >> 3);
+        uint64_t const r = row0 + (c >> 2);
+        uint64_t const l = l0 + 32 * (c & 3) + 4 * (lane & 7);
+        v[i] = (r < nrows && l < nq) ? *reinterpret_cast<const f32x4 *>(x + r * nq + l) : zero4();
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        float amax = fmaxf(fmaxf(fabsf(v[i][0]), fabsf(v[i][1])), fmaxf(fabsf(v[i][2]), fabsf(v[i][3])));
+        amax = fmaxf(amax, __shfl_xor(amax, 1));
+        amax = fmaxf(amax, __shfl_xor(amax, 2));
+        amax = fmaxf(amax, __shfl_xor(amax, 4));
+        out.e[i] = slice_exponent(amax);
+        out.q[i] = quantize4(v[i][0], v[i][1], v[i][2], v[i][3], out.e[i]);
+    }
+}
+
+template <int LDK>
+__device__ __forceinline__ void fp8_rows_store(const Fp8Rows &in, int tid, unsigned char *s, unsigned char *ss)
+{
+    int const lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        int const c = (wave * 8 + i) * 8 + (lane >> 3);
+        int const row = c >> 2, kb = c & 3;
+        *reinterpret_cast<int *>(&s[row * LDK + 32 * kb + 4 * (lane & 7)]) = in.q[i];
+        if ((lane & 7) == 0) ss[row * 4 + kb] = (unsigned char)(in.e[i] + 127);
     }
 }
 
@@ -879,35 +933,67 @@ __global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const 
 #pragma unroll
         for (int y = 0; y < 2; ++y) acc[x][y] = zero4();
 
+    // a row-major B with 16-byte aligned rows is read row-wise (uniform)
+    bool const b_rows = !BCOL && (nq & 3) == 0;
+
     for (uint64_t l0 = 0; l0 < nq; l0 += BK) {
-        float xa[32], xb[32];
         uint64_t const lb = l0 + 32 * kb;
+        // A: per-thread slice (NN view: lanes read consecutive columns) or
+        // row-wise (TN view)
+        i32x8 qa;
+        int ea = 0;
+        Fp8Rows ra;
+        if constexpr (TN) {
+            fp8_rows_load(a, n0, ncols, nq, l0, tid, ra);
+        } else {
+            float xa[32];
 #pragma unroll
-        for (int t = 0; t < 32; ++t) {
-            uint64_t const l = lb + t;
-            bool const lv = l < nq;
-            if constexpr (!TN) xa[t] = (a_ok && lv) ? a[a_base + l * w] : 0.f;
-            else xa[t] = (a_ok && lv) ? a[a_base + l] : 0.f;
-            if constexpr (BCOL) xb[t] = (b_ok && lv) ? b[j + l * m] : 0.f;
-            else xb[t] = (b_ok && lv) ? b[j * nq + l] : 0.f;
-        }
-        float amax = 0.f, bmax = 0.f;
+            for (int t = 0; t < 32; ++t) {
+                uint64_t const l = lb + t;
+                xa[t] = (a_ok && l < nq) ? a[a_base + l * w] : 0.f;
+            }
+            float amax = 0.f;
 #pragma unroll
-        for (int t = 0; t < 32; ++t) {
-            amax = fmaxf(amax, fabsf(xa[t]));
-            bmax = fmaxf(bmax, fabsf(xb[t]));
+            for (int t = 0; t < 32; ++t) amax = fmaxf(amax, fabsf(xa[t]));
+            ea = slice_exponent(amax);
+            quantize32(xa, ea, qa);
         }
-        int const ea = slice_exponent(amax), eb = slice_exponent(bmax);
-        i32x8 qa, qb;
-        quantize32(xa, ea, qa);
-        quantize32(xb, eb, qb);
+        // B: per-thread slice (column-major: lanes read consecutive rows j) or row-wise
+        i32x8 qb;
+        int eb = 0;
+        Fp8Rows rb;
+        if (b_rows) {
+            fp8_rows_load(b, j0, m, nq, l0, tid, rb);
+        } else {
+            float xb[32];
+#pragma unroll
+            for (int t = 0; t < 32; ++t) {
+                uint64_t const l = lb + t;
+                bool const lv = b_ok && l < nq;
+                if constexpr (BCOL) xb[t] = lv ? b[j + l * m] : 0.f;
+                else xb[t] = lv ? b[j * nq + l] : 0.f;
+            }
+            float bmax = 0.f;
+#pragma unroll
+            for (int t = 0; t < 32; ++t) bmax = fmaxf(bmax, fabsf(xb[t]));
+            eb = slice_exponent(bmax);
+            quantize32(xb, eb, qb);
+        }
         __syncthreads(); // the previous slice's fragments have been read
-        *reinterpret_cast<i32x4 *>(&sA[row * LDK + 32 * kb]) = {qa[0], qa[1], qa[2], qa[3]};
-        *reinterpret_cast<i32x4 *>(&sA[row * LDK + 32 * kb + 16]) = {qa[4], qa[5], qa[6], qa[7]};
-        *reinterpret_cast<i32x4 *>(&sB[row * LDK + 32 * kb]) = {qb[0], qb[1], qb[2], qb[3]};
-        *reinterpret_cast<i32x4 *>(&sB[row * LDK + 32 * kb + 16]) = {qb[4], qb[5], qb[6], qb[7]};
-        sAs[row * 4 + kb] = (unsigned char)(ea + 127);
-        sBs[row * 4 + kb] = (unsigned char)(eb + 127);
+        if constexpr (TN) {
+            fp8_rows_store<LDK>(ra, tid, sA, sAs);
+        } else {
+            *reinterpret_cast<i32x4 *>(&sA[row * LDK + 32 * kb]) = {qa[0], qa[1], qa[2], qa[3]};
+            *reinterpret_cast<i32x4 *>(&sA[row * LDK + 32 * kb + 16]) = {qa[4], qa[5], qa[6], qa[7]};
+            sAs[row * 4 + kb] = (unsigned char)(ea + 127);
+        }
+        if (b_rows) {
+            fp8_rows_store<LDK>(rb, tid, sB, sBs);
+        } else {
+            *reinterpret_cast<i32x4 *>(&sB[row * LDK + 32 * kb]) = {qb[0], qb[1], qb[2], qb[3]};
+            *reinterpret_cast<i32x4 *>(&sB[row * LDK + 32 * kb + 16]) = {qb[4], qb[5], qb[6], qb[7]};
+            sBs[row * 4 + kb] = (unsigned char)(eb + 127);
+        }
         __syncthreads();
         // Fragments: the lane's 32 bytes are k = 16*lg + [0,16) and
         // k = 64 + 16*lg + [0,16); its scale covers k = 32*lg + [0,32).
