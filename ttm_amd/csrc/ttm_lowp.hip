@@ -892,8 +892,38 @@ __device__ __forceinline__ void fp8_rows_store(const Fp8Rows &in, int tid, unsig
     }
 }
 
-template <bool TN, bool BCOL>
-__global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const float *__restrict__ b,
+// B -> e4m3fn bytes and E8M0 scales, once per product (the tile copies them):
+// row j at qb + j * ldq (ldq = n_q rounded up to 128), slice s of 32 bytes
+// with its scale byte at sb[j * (ldq / 32) + s]; zero (scale 2^0) outside
+// m x n_q.  One thread per row and slice.
+__global__ __launch_bounds__(256) void k_fp8_split_b(const float *__restrict__ b, unsigned char *__restrict__ qb,
+                                                     unsigned char *__restrict__ sb, uint64_t m, uint64_t nq,
+                                                     uint64_t bsj, uint64_t bsl, uint64_t ldq, uint64_t rows)
+{
+    uint64_t const slices = ldq / 32;
+    uint64_t const idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= rows * slices) return;
+    uint64_t const j = idx / slices, sl = idx - j * slices;
+    float x[32];
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+        uint64_t const l = 32 * sl + t;
+        x[t] = (j < m && l < nq) ? b[j * bsj + l * bsl] : 0.f;
+    }
+    float amax = 0.f;
+#pragma unroll
+    for (int t = 0; t < 32; ++t) amax = fmaxf(amax, fabsf(x[t]));
+    int const e = slice_exponent(amax);
+    i32x8 q;
+    quantize32(x, e, q);
+    *reinterpret_cast<i32x4 *>(qb + j * ldq + 32 * sl) = {q[0], q[1], q[2], q[3]};
+    *reinterpret_cast<i32x4 *>(qb + j * ldq + 32 * sl + 16) = {q[4], q[5], q[6], q[7]};
+    sb[j * slices + sl] = (unsigned char)(e + 127);
+}
+
+template <bool TN>
+__global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const unsigned char *__restrict__ qb,
+                                             const unsigned char *__restrict__ sb, uint64_t ldq,
                                              float *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w,
                                              uint64_t m, unsigned nj, unsigned total)
 {
@@ -924,17 +954,12 @@ __global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const 
     } else {
         a_base = a_ok ? n * nq : 0;
     }
-    uint64_t const j = j0 + row;
-    bool const b_ok = j < m;
 
     f32x4 acc[2][2];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) acc[x][y] = zero4();
-
-    // a row-major B with 16-byte aligned rows is read row-wise (uniform)
-    bool const b_rows = !BCOL && (nq & 3) == 0;
 
     for (uint64_t l0 = 0; l0 < nq; l0 += BK) {
         uint64_t const lb = l0 + 32 * kb;
@@ -958,27 +983,15 @@ __global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const 
             ea = slice_exponent(amax);
             quantize32(xa, ea, qa);
         }
-        // B: per-thread slice (column-major: lanes read consecutive rows j) or row-wise
-        i32x8 qb;
-        int eb = 0;
-        Fp8Rows rb;
-        if (b_rows) {
-            fp8_rows_load(b, j0, m, nq, l0, tid, rb);
-        } else {
-            float xb[32];
+        // B: the pre-quantized slice, 64 rows x 128 bytes (16-byte piece p:
+        // row p / 8, bytes 16 * (p % 8) of the slice), and its 64 x 4 scales
+        i32x4 pb[2];
 #pragma unroll
-            for (int t = 0; t < 32; ++t) {
-                uint64_t const l = lb + t;
-                bool const lv = b_ok && l < nq;
-                if constexpr (BCOL) xb[t] = lv ? b[j + l * m] : 0.f;
-                else xb[t] = lv ? b[j * nq + l] : 0.f;
-            }
-            float bmax = 0.f;
-#pragma unroll
-            for (int t = 0; t < 32; ++t) bmax = fmaxf(bmax, fabsf(xb[t]));
-            eb = slice_exponent(bmax);
-            quantize32(xb, eb, qb);
+        for (int i = 0; i < 2; ++i) {
+            int const p = tid + 256 * i;
+            pb[i] = *reinterpret_cast<const i32x4 *>(qb + (j0 + (p >> 3)) * ldq + l0 + 16 * (p & 7));
         }
+        unsigned char const eb = sb[(j0 + (tid >> 2)) * (ldq / 32) + l0 / 32 + (tid & 3)];
         __syncthreads(); // the previous slice's fragments have been read
         if constexpr (TN) {
             fp8_rows_store<LDK>(ra, tid, sA, sAs);
@@ -987,13 +1000,12 @@ __global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const 
             *reinterpret_cast<i32x4 *>(&sA[row * LDK + 32 * kb + 16]) = {qa[4], qa[5], qa[6], qa[7]};
             sAs[row * 4 + kb] = (unsigned char)(ea + 127);
         }
-        if (b_rows) {
-            fp8_rows_store<LDK>(rb, tid, sB, sBs);
-        } else {
-            *reinterpret_cast<i32x4 *>(&sB[row * LDK + 32 * kb]) = {qb[0], qb[1], qb[2], qb[3]};
-            *reinterpret_cast<i32x4 *>(&sB[row * LDK + 32 * kb + 16]) = {qb[4], qb[5], qb[6], qb[7]};
-            sBs[row * 4 + kb] = (unsigned char)(eb + 127);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            int const p = tid + 256 * i;
+            *reinterpret_cast<i32x4 *>(&sB[(p >> 3) * LDK + 16 * (p & 7)]) = pb[i];
         }
+        sBs[tid] = eb; // row tid / 4, slice tid % 4
         __syncthreads();
         // Fragments: the lane's 32 bytes are k = 16*lg + [0,16) and
         // k = 64 + 16*lg + [0,16); its scale covers k = 32*lg + [0,32).
@@ -1231,13 +1243,41 @@ ttm_status_t launch_lp(const Fold &f, const float *a, const float *b, float *c, 
     return TTM_SUCCESS;
 }
 
-template <bool TN, bool BCOL>
-void launch_fp8(const Fold &f, const float *a, const float *b, float *c, hipStream_t s)
+// B quantized once (k_fp8_split_b) into pooled workspace, then the tiles.
+template <bool TN>
+ttm_status_t launch_fp8(const Fold &f, const float *a, const float *b, float *c, hipStream_t s)
 {
     uint64_t const ncols = TN ? f.outer : f.outer * f.w;
     unsigned const nj = (unsigned)((f.m + 63) / 64);
     unsigned const total = nj * (unsigned)((ncols + 63) / 64);
-    hipLaunchKernelGGL((k_fp8<TN, BCOL>), dim3(total), dim3(256), 0, s, a, b, c, ncols, f.nq, f.w, f.m, nj, total);
+    uint64_t const ldq = (f.nq + 127) / 128 * 128;
+    uint64_t const rows = (uint64_t)nj * 64;
+    uint64_t const split_blocks = (rows * (ldq / 32) + 255) / 256;
+    if (split_blocks >= (1ull << 31)) {
+        set_error(TTM_ERR_ARGUMENT, "B too large for the reduced-precision kernels");
+        return TTM_ERR_ARGUMENT;
+    }
+    size_t const bytes_q = (rows * ldq + 255) / 256 * 256;
+    hipMemPool_t const pool = workspace_pool();
+    void *ws = nullptr;
+    if (!pool || hipMallocFromPoolAsync(&ws, bytes_q + rows * (ldq / 32), pool, s) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error(TTM_ERR_ALLOC, "workspace for the quantized B operand");
+        return TTM_ERR_ALLOC;
+    }
+    unsigned char *const qb = static_cast<unsigned char *>(ws);
+    unsigned char *const sb = qb + bytes_q;
+    hipLaunchKernelGGL(k_fp8_split_b, dim3((unsigned)split_blocks), dim3(256), 0, s, b, qb, sb, f.m, f.nq,
+                       f.b_stride_j(), f.b_stride_l(), ldq, rows);
+    hipLaunchKernelGGL((k_fp8<TN>), dim3(total), dim3(256), 0, s, a, qb, sb, ldq, c, ncols, f.nq, f.w, f.m, nj,
+                       total);
+    hipError_t const e = hipGetLastError();
+    (void)hipFreeAsync(ws, s);
+    if (e != hipSuccess) {
+        set_error(TTM_ERR_HIP, hipGetErrorString(e));
+        return TTM_ERR_HIP;
+    }
+    return TTM_SUCCESS;
 }
 
 template <int MODE>
@@ -1261,11 +1301,10 @@ bool lowp_supports(const Fold &f, bool aligned)
 std::string lowp_symbol(const Fold &f, ttm_precision_t prec, bool generic)
 {
     char const *tn = f.w == 1 ? "true" : "false";
-    char const *bc = f.b_colmajor ? "true" : "false";
     char buf[96];
     if (generic) return "k_lowp_generic";
     if (prec == TTM_PREC_FP8_E4M3) {
-        std::snprintf(buf, sizeof buf, "k_fp8<%s, %s>", tn, bc);
+        std::snprintf(buf, sizeof buf, "k_fp8<%s>", tn);
     } else {
         int const mode = prec == TTM_PREC_BF16 ? LP_BF16
                          : prec == TTM_PREC_FP16 ? LP_FP16
@@ -1299,14 +1338,7 @@ ttm_status_t launch_lowp(const Fold &f, const float *a, const float *b, float *c
                            f.m, f.b_stride_j(), f.b_stride_l(), (int)prec, exps);
         if (ws) (void)hipFreeAsync(ws, s);
     } else if (prec == TTM_PREC_FP8_E4M3) {
-        bool const tn = f.w == 1;
-        if (tn) {
-            if (f.b_colmajor) launch_fp8<true, true>(f, a, b, c, s);
-            else launch_fp8<true, false>(f, a, b, c, s);
-        } else {
-            if (f.b_colmajor) launch_fp8<false, true>(f, a, b, c, s);
-            else launch_fp8<false, false>(f, a, b, c, s);
-        }
+        return f.w == 1 ? launch_fp8<true>(f, a, b, c, s) : launch_fp8<false>(f, a, b, c, s);
     } else if (prec == TTM_PREC_BF16) {
         return launch_lp_mode<LP_BF16>(f, a, b, c, s);
     } else if (prec == TTM_PREC_FP16) {
