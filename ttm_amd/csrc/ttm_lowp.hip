@@ -921,13 +921,14 @@ __global__ __launch_bounds__(256) void k_fp8_split_b(const float *__restrict__ b
     sb[j * slices + sl] = (unsigned char)(e + 127);
 }
 
-template <bool TN>
+template <bool TN, int BM>
 __global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const unsigned char *__restrict__ qb,
                                              const unsigned char *__restrict__ sb, uint64_t ldq,
                                              float *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w,
                                              uint64_t m, unsigned nj, unsigned total)
 {
-    constexpr int BM = 64, BN = 64, BK = 128;
+    constexpr int BN = 64, BK = 128;
+    constexpr int JT = BM / 32; // 16-row MFMA tiles per wave along j (2 x 2 waves)
     constexpr int LDK = BK + 16; // bytes per row
     __shared__ __attribute__((aligned(16))) unsigned char sA[BN * LDK];
     __shared__ __attribute__((aligned(16))) unsigned char sB[BM * LDK];
@@ -955,9 +956,9 @@ __global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const 
         a_base = a_ok ? n * nq : 0;
     }
 
-    f32x4 acc[2][2];
+    f32x4 acc[JT][2];
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int x = 0; x < JT; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) acc[x][y] = zero4();
 
@@ -983,15 +984,21 @@ __global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const 
             ea = slice_exponent(amax);
             quantize32(xa, ea, qa);
         }
-        // B: the pre-quantized slice, 64 rows x 128 bytes (16-byte piece p:
-        // row p / 8, bytes 16 * (p % 8) of the slice), and its 64 x 4 scales
-        i32x4 pb[2];
+        // B: the pre-quantized slice, BM rows x 128 bytes (16-byte piece p:
+        // row p / 8, bytes 16 * (p % 8) of the slice), and its BM x 4 scales
+        // (scale byte t: row t / 4, slice t % 4)
+        i32x4 pb[BM / 32];
+        unsigned char eb[BM / 64];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < BM / 32; ++i) {
             int const p = tid + 256 * i;
             pb[i] = *reinterpret_cast<const i32x4 *>(qb + (j0 + (p >> 3)) * ldq + l0 + 16 * (p & 7));
         }
-        unsigned char const eb = sb[(j0 + (tid >> 2)) * (ldq / 32) + l0 / 32 + (tid & 3)];
+#pragma unroll
+        for (int i = 0; i < BM / 64; ++i) {
+            int const t = tid + 256 * i;
+            eb[i] = sb[(j0 + (t >> 2)) * (ldq / 32) + l0 / 32 + (t & 3)];
+        }
         __syncthreads(); // the previous slice's fragments have been read
         if constexpr (TN) {
             fp8_rows_store<LDK>(ra, tid, sA, sAs);
@@ -1001,18 +1008,19 @@ __global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const 
             sAs[row * 4 + kb] = (unsigned char)(ea + 127);
         }
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < BM / 32; ++i) {
             int const p = tid + 256 * i;
             *reinterpret_cast<i32x4 *>(&sB[(p >> 3) * LDK + 16 * (p & 7)]) = pb[i];
         }
-        sBs[tid] = eb; // row tid / 4, slice tid % 4
+#pragma unroll
+        for (int i = 0; i < BM / 64; ++i) sBs[tid + 256 * i] = eb[i];
         __syncthreads();
         // Fragments: the lane's 32 bytes are k = 16*lg + [0,16) and
         // k = 64 + 16*lg + [0,16); its scale covers k = 32*lg + [0,32).
-        int sbv[2], sav[2];
+        int sbv[JT], sav[2];
 #pragma unroll
-        for (int jt = 0; jt < 2; ++jt) {
-            int const rj = wm * 32 + jt * 16 + lc;
+        for (int jt = 0; jt < JT; ++jt) {
+            int const rj = wm * (BM / 2) + jt * 16 + lc;
             i32x4 lo = *reinterpret_cast<const i32x4 *>(&sB[rj * LDK + 16 * lg]);
             i32x4 hi = *reinterpret_cast<const i32x4 *>(&sB[rj * LDK + 64 + 16 * lg]);
             i32x8 fb = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
@@ -1032,7 +1040,9 @@ __global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const 
         // no later instruction of the slice reuses a register a matrix
         // instruction may still read its scale from.
         __syncthreads();
-        asm volatile("" ::"v"(sbv[0]), "v"(sbv[1]), "v"(sav[0]), "v"(sav[1]));
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) asm volatile("" ::"v"(sbv[jt]));
+        asm volatile("" ::"v"(sav[0]), "v"(sav[1]));
     }
 
 #pragma unroll
@@ -1050,10 +1060,10 @@ __global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const 
             stride = 1;
         }
 #pragma unroll
-        for (int jt = 0; jt < 2; ++jt)
+        for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                uint64_t const jj = j0 + wm * 32 + jt * 16 + 4 * lg + r;
+                uint64_t const jj = j0 + wm * (BM / 2) + jt * 16 + 4 * lg + r;
                 if (jj < m) cbase[jj * stride] = acc[jt][nt][r];
             }
     }
@@ -1243,15 +1253,20 @@ ttm_status_t launch_lp(const Fold &f, const float *a, const float *b, float *c, 
     return TTM_SUCCESS;
 }
 
+// Rows of B per FP8 block: 128 where B has more than 64 rows (each block
+// then quantizes its slice of A for twice as many rows of C), else 64.
+int fp8_block_rows(const Fold &f) { return f.m > 64 ? 128 : 64; }
+
 // B quantized once (k_fp8_split_b) into pooled workspace, then the tiles.
 template <bool TN>
 ttm_status_t launch_fp8(const Fold &f, const float *a, const float *b, float *c, hipStream_t s)
 {
+    uint64_t const bm = (uint64_t)fp8_block_rows(f);
     uint64_t const ncols = TN ? f.outer : f.outer * f.w;
-    unsigned const nj = (unsigned)((f.m + 63) / 64);
+    unsigned const nj = (unsigned)((f.m + bm - 1) / bm);
     unsigned const total = nj * (unsigned)((ncols + 63) / 64);
     uint64_t const ldq = (f.nq + 127) / 128 * 128;
-    uint64_t const rows = (uint64_t)nj * 64;
+    uint64_t const rows = (uint64_t)nj * bm;
     uint64_t const split_blocks = (rows * (ldq / 32) + 255) / 256;
     if (split_blocks >= (1ull << 31)) {
         set_error(TTM_ERR_ARGUMENT, "B too large for the reduced-precision kernels");
@@ -1269,8 +1284,12 @@ ttm_status_t launch_fp8(const Fold &f, const float *a, const float *b, float *c,
     unsigned char *const sb = qb + bytes_q;
     hipLaunchKernelGGL(k_fp8_split_b, dim3((unsigned)split_blocks), dim3(256), 0, s, b, qb, sb, f.m, f.nq,
                        f.b_stride_j(), f.b_stride_l(), ldq, rows);
-    hipLaunchKernelGGL((k_fp8<TN>), dim3(total), dim3(256), 0, s, a, qb, sb, ldq, c, ncols, f.nq, f.w, f.m, nj,
-                       total);
+    if (bm == 128)
+        hipLaunchKernelGGL((k_fp8<TN, 128>), dim3(total), dim3(256), 0, s, a, qb, sb, ldq, c, ncols, f.nq, f.w, f.m,
+                           nj, total);
+    else
+        hipLaunchKernelGGL((k_fp8<TN, 64>), dim3(total), dim3(256), 0, s, a, qb, sb, ldq, c, ncols, f.nq, f.w, f.m,
+                           nj, total);
     hipError_t const e = hipGetLastError();
     (void)hipFreeAsync(ws, s);
     if (e != hipSuccess) {
@@ -1304,7 +1323,7 @@ std::string lowp_symbol(const Fold &f, ttm_precision_t prec, bool generic)
     char buf[96];
     if (generic) return "k_lowp_generic";
     if (prec == TTM_PREC_FP8_E4M3) {
-        std::snprintf(buf, sizeof buf, "k_fp8<%s>", tn);
+        std::snprintf(buf, sizeof buf, "k_fp8<%s, %d>", tn, fp8_block_rows(f));
     } else {
         int const mode = prec == TTM_PREC_BF16 ? LP_BF16
                          : prec == TTM_PREC_FP16 ? LP_FP16
