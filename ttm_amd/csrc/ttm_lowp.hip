@@ -842,14 +842,14 @@ __device__ __forceinline__ void quantize32(const float (&x)[32], int e, i32x8 &o
     for (int q = 0; q < 8; ++q) out[q] = quantize4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3], e);
 }
 
-// Operand tiles whose 64 rows are contiguous along l (A in the TN view, a
-// row-major B; n_q % 4 == 0 and 16-byte aligned rows): each wave instruction
-// reads 2 rows x 512 bytes, 8 lanes per 32-wide scale slice, and the slice
-// maximum is reduced over those 8 lanes.  Slice c = row * 4 + kb of the tile
-// goes to lanes 8 * (c % 8) .. + 7 of wave c / 64 in step (c / 8) % 8; lane
-// s of the 8 holds l = l0 + 32 * kb + 4 * s .. + 3.  The bytes, scales and
-// LDS placement are those of the per-thread path (one row and slice per
-// thread): max is order-independent and each element is quantized alone.
+// A 64 x 128 tile of rows contiguous along l (A in the TN view; n_q % 4 == 0
+// and 16-byte aligned rows): each wave instruction reads 2 rows x 512 bytes,
+// 8 lanes per 32-wide scale slice, and the slice maximum is reduced over
+// those 8 lanes.  Slice c = row * 4 + kb of the tile goes to lanes
+// 8 * (c % 8) .. + 7 of wave c / 64 in step (c / 8) % 8; lane s of the 8
+// holds l = l0 + 32 * kb + 4 * s .. + 3.  The bytes and scales equal those of
+// one thread per row and slice: max is order-independent and each element is
+// quantized alone.
 struct Fp8Rows
 {
     int q[8]; // 4 e4m3fn bytes per step
@@ -876,19 +876,6 @@ __device__ __forceinline__ void fp8_rows_load(const float *__restrict__ x, uint6
         amax = fmaxf(amax, __shfl_xor(amax, 4));
         out.e[i] = slice_exponent(amax);
         out.q[i] = quantize4(v[i][0], v[i][1], v[i][2], v[i][3], out.e[i]);
-    }
-}
-
-template <int LDK>
-__device__ __forceinline__ void fp8_rows_store(const Fp8Rows &in, int tid, unsigned char *s, unsigned char *ss)
-{
-    int const lane = tid & 63, wave = tid >> 6;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        int const c = (wave * 8 + i) * 8 + (lane >> 3);
-        int const row = c >> 2, kb = c & 3;
-        *reinterpret_cast<int *>(&s[row * LDK + 32 * kb + 4 * (lane & 7)]) = in.q[i];
-        if ((lane & 7) == 0) ss[row * 4 + kb] = (unsigned char)(in.e[i] + 127);
     }
 }
 
@@ -921,11 +908,65 @@ __global__ __launch_bounds__(256) void k_fp8_split_b(const float *__restrict__ b
     sb[j * slices + sl] = (unsigned char)(e + 127);
 }
 
+// A -> e4m3fn bytes and scales in the layout of the quantized B (column n
+// of the fold as row n, padded to `rows` rows and ldq bytes), once per
+// product.  TN view: rows of A are contiguous along l; one 64 x 128 tile per
+// block, read 2 rows x 512 bytes per wave instruction (fp8_rows_load).
+__global__ __launch_bounds__(256) void k_fp8_split_a_rows(const float *__restrict__ a, unsigned char *__restrict__ qa,
+                                                          unsigned char *__restrict__ sa, uint64_t ncols, uint64_t nq,
+                                                          uint64_t ldq, uint64_t lblocks)
+{
+    uint64_t const rb = blockIdx.x / lblocks, l0 = (blockIdx.x - rb * lblocks) * 128;
+    uint64_t const row0 = rb * 64;
+    int const tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    Fp8Rows r;
+    fp8_rows_load(a, row0, ncols, nq, l0, tid, r);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        int const cix = (wave * 8 + i) * 8 + (lane >> 3);
+        uint64_t const row = row0 + (cix >> 2);
+        int const kb = cix & 3;
+        *reinterpret_cast<int *>(qa + row * ldq + l0 + 32 * kb + 4 * (lane & 7)) = r.q[i];
+        if ((lane & 7) == 0) sa[row * (ldq / 32) + l0 / 32 + kb] = (unsigned char)(r.e[i] + 127);
+    }
+}
+
+// NN view: column n = k * w + i of A runs along l with stride w; one thread
+// per column and slice, consecutive threads on consecutive columns.
+__global__ __launch_bounds__(256) void k_fp8_split_a_cols(const float *__restrict__ a, unsigned char *__restrict__ qa,
+                                                          unsigned char *__restrict__ sa, uint64_t ncols, uint64_t nq,
+                                                          uint64_t w, uint64_t ldq, uint64_t rows)
+{
+    uint64_t const idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= rows * (ldq / 32)) return;
+    uint64_t const sl = idx / rows, n = idx - sl * rows;
+    bool const ok = n < ncols;
+    uint64_t const k = ok ? n / w : 0;
+    uint64_t const base = ok ? k * nq * w + (n - k * w) : 0;
+    float x[32];
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+        uint64_t const l = 32 * sl + t;
+        x[t] = (ok && l < nq) ? a[base + l * w] : 0.f;
+    }
+    float amax = 0.f;
+#pragma unroll
+    for (int t = 0; t < 32; ++t) amax = fmaxf(amax, fabsf(x[t]));
+    int const e = slice_exponent(amax);
+    i32x8 q;
+    quantize32(x, e, q);
+    *reinterpret_cast<i32x4 *>(qa + n * ldq + 32 * sl) = {q[0], q[1], q[2], q[3]};
+    *reinterpret_cast<i32x4 *>(qa + n * ldq + 32 * sl + 16) = {q[4], q[5], q[6], q[7]};
+    sa[n * (ldq / 32) + sl] = (unsigned char)(e + 127);
+}
+
+// The product over the quantized operands: BM rows of B x 64 columns of A
+// per block, slices of 128; both tiles are byte copies into LDS.
 template <bool TN, int BM>
-__global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const unsigned char *__restrict__ qb,
-                                             const unsigned char *__restrict__ sb, uint64_t ldq,
-                                             float *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w,
-                                             uint64_t m, unsigned nj, unsigned total)
+__global__ __launch_bounds__(256) void k_fp8(const unsigned char *__restrict__ qa, const unsigned char *__restrict__ sa,
+                                             const unsigned char *__restrict__ qb, const unsigned char *__restrict__ sb,
+                                             uint64_t ldq, float *__restrict__ c, uint64_t ncols, uint64_t nq,
+                                             uint64_t w, uint64_t m, unsigned nj, unsigned total)
 {
     constexpr int BN = 64, BK = 128;
     constexpr int JT = BM / 32; // 16-row MFMA tiles per wave along j (2 x 2 waves)
@@ -944,18 +985,6 @@ __global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const 
     int const lc = lane & 15, lg = lane >> 4;
     int const wm = wave & 1, wn = wave >> 1;
 
-    // every thread owns one (row, K-block) of each tile: row = tid % 64, block = tid / 64
-    int const row = tid & 63, kb = tid >> 6;
-    uint64_t const n = n0 + row;
-    bool const a_ok = n < ncols;
-    uint64_t a_base;
-    if constexpr (!TN) {
-        uint64_t const k = a_ok ? n / w : 0;
-        a_base = k * nq * w + (a_ok ? n - k * w : 0);
-    } else {
-        a_base = a_ok ? n * nq : 0;
-    }
-
     f32x4 acc[JT][2];
 #pragma unroll
     for (int x = 0; x < JT; ++x)
@@ -963,27 +992,15 @@ __global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const 
         for (int y = 0; y < 2; ++y) acc[x][y] = zero4();
 
     for (uint64_t l0 = 0; l0 < nq; l0 += BK) {
-        uint64_t const lb = l0 + 32 * kb;
-        // A: per-thread slice (NN view: lanes read consecutive columns) or
-        // row-wise (TN view)
-        i32x8 qa;
-        int ea = 0;
-        Fp8Rows ra;
-        if constexpr (TN) {
-            fp8_rows_load(a, n0, ncols, nq, l0, tid, ra);
-        } else {
-            float xa[32];
+        // A: the pre-quantized slice, 64 rows (columns n) x 128 bytes, and
+        // its 64 x 4 scales, laid out as B's
+        i32x4 pa[2];
 #pragma unroll
-            for (int t = 0; t < 32; ++t) {
-                uint64_t const l = lb + t;
-                xa[t] = (a_ok && l < nq) ? a[a_base + l * w] : 0.f;
-            }
-            float amax = 0.f;
-#pragma unroll
-            for (int t = 0; t < 32; ++t) amax = fmaxf(amax, fabsf(xa[t]));
-            ea = slice_exponent(amax);
-            quantize32(xa, ea, qa);
+        for (int i = 0; i < 2; ++i) {
+            int const p = tid + 256 * i;
+            pa[i] = *reinterpret_cast<const i32x4 *>(qa + (n0 + (p >> 3)) * ldq + l0 + 16 * (p & 7));
         }
+        unsigned char const ea = sa[(n0 + (tid >> 2)) * (ldq / 32) + l0 / 32 + (tid & 3)];
         // B: the pre-quantized slice, BM rows x 128 bytes (16-byte piece p:
         // row p / 8, bytes 16 * (p % 8) of the slice), and its BM x 4 scales
         // (scale byte t: row t / 4, slice t % 4)
@@ -1000,13 +1017,12 @@ __global__ __launch_bounds__(256) void k_fp8(const float *__restrict__ a, const 
             eb[i] = sb[(j0 + (t >> 2)) * (ldq / 32) + l0 / 32 + (t & 3)];
         }
         __syncthreads(); // the previous slice's fragments have been read
-        if constexpr (TN) {
-            fp8_rows_store<LDK>(ra, tid, sA, sAs);
-        } else {
-            *reinterpret_cast<i32x4 *>(&sA[row * LDK + 32 * kb]) = {qa[0], qa[1], qa[2], qa[3]};
-            *reinterpret_cast<i32x4 *>(&sA[row * LDK + 32 * kb + 16]) = {qa[4], qa[5], qa[6], qa[7]};
-            sAs[row * 4 + kb] = (unsigned char)(ea + 127);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            int const p = tid + 256 * i;
+            *reinterpret_cast<i32x4 *>(&sA[(p >> 3) * LDK + 16 * (p & 7)]) = pa[i];
         }
+        sAs[tid] = ea;
 #pragma unroll
         for (int i = 0; i < BM / 32; ++i) {
             int const p = tid + 256 * i;
@@ -1253,11 +1269,13 @@ ttm_status_t launch_lp(const Fold &f, const float *a, const float *b, float *c, 
     return TTM_SUCCESS;
 }
 
-// Rows of B per FP8 block: 128 where B has more than 64 rows (each block
-// then quantizes its slice of A for twice as many rows of C), else 64.
+// Rows of B per FP8 block: 128 where B has more than 64 rows (each block's
+// copy of A's slice then serves twice as many rows of C), else 64.
 int fp8_block_rows(const Fold &f) { return f.m > 64 ? 128 : 64; }
 
-// B quantized once (k_fp8_split_b) into pooled workspace, then the tiles.
+// A and B quantized once per product (k_fp8_split_a_*, k_fp8_split_b) into
+// pooled workspace: N_A + m n_q bytes (rounded up to whole blocks and
+// slices) plus a scale byte per 32 of them; then the tiles.
 template <bool TN>
 ttm_status_t launch_fp8(const Fold &f, const float *a, const float *b, float *c, hipStream_t s)
 {
@@ -1267,29 +1285,41 @@ ttm_status_t launch_fp8(const Fold &f, const float *a, const float *b, float *c,
     unsigned const total = nj * (unsigned)((ncols + 63) / 64);
     uint64_t const ldq = (f.nq + 127) / 128 * 128;
     uint64_t const rows = (uint64_t)nj * bm;
+    uint64_t const rows_a = (ncols + 63) / 64 * 64;
     uint64_t const split_blocks = (rows * (ldq / 32) + 255) / 256;
-    if (split_blocks >= (1ull << 31)) {
-        set_error(TTM_ERR_ARGUMENT, "B too large for the reduced-precision kernels");
+    uint64_t const split_a_blocks = TN ? (rows_a / 64) * (ldq / 128) : (rows_a * (ldq / 32) + 255) / 256;
+    if (split_blocks >= (1ull << 31) || split_a_blocks >= (1ull << 31)) {
+        set_error(TTM_ERR_ARGUMENT, "operands too large for the FP8 kernels");
         return TTM_ERR_ARGUMENT;
     }
-    size_t const bytes_q = (rows * ldq + 255) / 256 * 256;
+    auto round256 = [](uint64_t x) { return (x + 255) / 256 * 256; };
+    size_t const bytes_qb = round256(rows * ldq), bytes_sb = round256(rows * (ldq / 32));
+    size_t const bytes_qa = round256(rows_a * ldq), bytes_sa = round256(rows_a * (ldq / 32));
     hipMemPool_t const pool = workspace_pool();
     void *ws = nullptr;
-    if (!pool || hipMallocFromPoolAsync(&ws, bytes_q + rows * (ldq / 32), pool, s) != hipSuccess) {
+    if (!pool || hipMallocFromPoolAsync(&ws, bytes_qb + bytes_sb + bytes_qa + bytes_sa, pool, s) != hipSuccess) {
         (void)hipGetLastError();
-        set_error(TTM_ERR_ALLOC, "workspace for the quantized B operand");
+        set_error(TTM_ERR_ALLOC, "workspace for the quantized operands");
         return TTM_ERR_ALLOC;
     }
     unsigned char *const qb = static_cast<unsigned char *>(ws);
-    unsigned char *const sb = qb + bytes_q;
+    unsigned char *const sb = qb + bytes_qb;
+    unsigned char *const qa = sb + bytes_sb;
+    unsigned char *const sa = qa + bytes_qa;
     hipLaunchKernelGGL(k_fp8_split_b, dim3((unsigned)split_blocks), dim3(256), 0, s, b, qb, sb, f.m, f.nq,
                        f.b_stride_j(), f.b_stride_l(), ldq, rows);
-    if (bm == 128)
-        hipLaunchKernelGGL((k_fp8<TN, 128>), dim3(total), dim3(256), 0, s, a, qb, sb, ldq, c, ncols, f.nq, f.w, f.m,
-                           nj, total);
+    if constexpr (TN)
+        hipLaunchKernelGGL(k_fp8_split_a_rows, dim3((unsigned)split_a_blocks), dim3(256), 0, s, a, qa, sa, ncols,
+                           f.nq, ldq, ldq / 128);
     else
-        hipLaunchKernelGGL((k_fp8<TN, 64>), dim3(total), dim3(256), 0, s, a, qb, sb, ldq, c, ncols, f.nq, f.w, f.m,
-                           nj, total);
+        hipLaunchKernelGGL(k_fp8_split_a_cols, dim3((unsigned)split_a_blocks), dim3(256), 0, s, a, qa, sa, ncols,
+                           f.nq, f.w, ldq, rows_a);
+    if (bm == 128)
+        hipLaunchKernelGGL((k_fp8<TN, 128>), dim3(total), dim3(256), 0, s, qa, sa, qb, sb, ldq, c, ncols, f.nq, f.w,
+                           f.m, nj, total);
+    else
+        hipLaunchKernelGGL((k_fp8<TN, 64>), dim3(total), dim3(256), 0, s, qa, sa, qb, sb, ldq, c, ncols, f.nq, f.w,
+                           f.m, nj, total);
     hipError_t const e = hipGetLastError();
     (void)hipFreeAsync(ws, s);
     if (e != hipSuccess) {
