@@ -991,21 +991,22 @@ __global__ __launch_bounds__(256) void k_fp8(const unsigned char *__restrict__ q
 #pragma unroll
         for (int y = 0; y < 2; ++y) acc[x][y] = zero4();
 
-    for (uint64_t l0 = 0; l0 < nq; l0 += BK) {
-        // A: the pre-quantized slice, 64 rows (columns n) x 128 bytes, and
-        // its 64 x 4 scales, laid out as B's
-        i32x4 pa[2];
+    // A: the pre-quantized slice, 64 rows (columns n) x 128 bytes, and its
+    // 64 x 4 scales, laid out as B's.  B: BM rows x 128 bytes (16-byte piece
+    // p: row p / 8, bytes 16 * (p % 8) of the slice) and BM x 4 scales (scale
+    // byte t: row t / 4, slice t % 4).  Slice l0 + BK is loaded into these
+    // registers while slice l0 is multiplied.
+    i32x4 pa[2];
+    unsigned char ea;
+    i32x4 pb[BM / 32];
+    unsigned char eb[BM / 64];
+    auto load = [&](uint64_t l0) {
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             int const p = tid + 256 * i;
             pa[i] = *reinterpret_cast<const i32x4 *>(qa + (n0 + (p >> 3)) * ldq + l0 + 16 * (p & 7));
         }
-        unsigned char const ea = sa[(n0 + (tid >> 2)) * (ldq / 32) + l0 / 32 + (tid & 3)];
-        // B: the pre-quantized slice, BM rows x 128 bytes (16-byte piece p:
-        // row p / 8, bytes 16 * (p % 8) of the slice), and its BM x 4 scales
-        // (scale byte t: row t / 4, slice t % 4)
-        i32x4 pb[BM / 32];
-        unsigned char eb[BM / 64];
+        ea = sa[(n0 + (tid >> 2)) * (ldq / 32) + l0 / 32 + (tid & 3)];
 #pragma unroll
         for (int i = 0; i < BM / 32; ++i) {
             int const p = tid + 256 * i;
@@ -1016,7 +1017,11 @@ __global__ __launch_bounds__(256) void k_fp8(const unsigned char *__restrict__ q
             int const t = tid + 256 * i;
             eb[i] = sb[(j0 + (t >> 2)) * (ldq / 32) + l0 / 32 + (t & 3)];
         }
-        __syncthreads(); // the previous slice's fragments have been read
+    };
+    load(0);
+
+    for (uint64_t l0 = 0; l0 < nq; l0 += BK) {
+        // (the barrier at the end of the previous slice: its fragments have been read)
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             int const p = tid + 256 * i;
@@ -1031,6 +1036,7 @@ __global__ __launch_bounds__(256) void k_fp8(const unsigned char *__restrict__ q
 #pragma unroll
         for (int i = 0; i < BM / 64; ++i) sBs[tid + 256 * i] = eb[i];
         __syncthreads();
+        if (l0 + BK < nq) load(l0 + BK);
         // Fragments: the lane's 32 bytes are k = 16*lg + [0,16) and
         // k = 64 + 16*lg + [0,16); its scale covers k = 32*lg + [0,32).
         int sbv[JT], sav[2];
@@ -1052,9 +1058,10 @@ __global__ __launch_bounds__(256) void k_fp8(const unsigned char *__restrict__ q
                     __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fb, fa, acc[jt][nt], 0, 0, 0, sbv[jt], 0, sav[nt]);
             }
         }
-        // Keep the scale registers allocated until the slice is done, so that
-        // no later instruction of the slice reuses a register a matrix
-        // instruction may still read its scale from.
+        // All fragments of the slice have been read before the next slice is
+        // stored.  Keep the scale registers allocated until then, so that no
+        // later instruction reuses a register a matrix instruction may still
+        // read its scale from.
         __syncthreads();
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) asm volatile("" ::"v"(sbv[jt]));
