@@ -931,33 +931,54 @@ __global__ __launch_bounds__(256) void k_fp8_split_a_rows(const float *__restric
     }
 }
 
-// NN view: column n = k * w + i of A runs along l with stride w; one thread
-// per column and slice, consecutive threads on consecutive columns.
+// NN view: column n = k * w + i of A runs along l with stride w.  One block
+// per 64 columns x 128 contraction indices (w % 4 == 0, so 4 consecutive
+// columns share a slab and a 16-byte vector): the tile is read row by row in
+// 16-byte vectors (16 lanes per 256-byte row), transposed through LDS, and
+// quantized and written as in the TN view (eight lanes per 32-wide slice,
+// 2 columns x 128 bytes per wave instruction).
 __global__ __launch_bounds__(256) void k_fp8_split_a_cols(const float *__restrict__ a, unsigned char *__restrict__ qa,
                                                           unsigned char *__restrict__ sa, uint64_t ncols, uint64_t nq,
-                                                          uint64_t w, uint64_t ldq, uint64_t rows)
+                                                          uint64_t w, uint64_t ldq, uint64_t lblocks)
 {
-    uint64_t const idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= rows * (ldq / 32)) return;
-    uint64_t const sl = idx / rows, n = idx - sl * rows;
+    constexpr int LDT = 128 + 4; // floats per transposed column (16-byte aligned rows)
+    __shared__ __attribute__((aligned(16))) float sT[64 * LDT];
+    uint64_t const cb = blockIdx.x / lblocks, l0 = (blockIdx.x - cb * lblocks) * 128;
+    uint64_t const n0 = cb * 64;
+    int const tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // load: thread tid covers columns n0 + 4 (tid % 16) .. + 3 of rows l0 + tid / 16 + 16 i
+    int const cg = tid & 15;
+    uint64_t const n = n0 + 4 * cg;
     bool const ok = n < ncols;
     uint64_t const k = ok ? n / w : 0;
     uint64_t const base = ok ? k * nq * w + (n - k * w) : 0;
-    float x[32];
+    f32x4 v[8];
 #pragma unroll
-    for (int t = 0; t < 32; ++t) {
-        uint64_t const l = 32 * sl + t;
-        x[t] = (ok && l < nq) ? a[base + l * w] : 0.f;
+    for (int i = 0; i < 8; ++i) {
+        uint64_t const l = l0 + (tid >> 4) + 16 * i;
+        v[i] = (ok && l < nq) ? *reinterpret_cast<const f32x4 *>(a + base + l * w) : zero4();
     }
-    float amax = 0.f;
 #pragma unroll
-    for (int t = 0; t < 32; ++t) amax = fmaxf(amax, fabsf(x[t]));
-    int const e = slice_exponent(amax);
-    i32x8 q;
-    quantize32(x, e, q);
-    *reinterpret_cast<i32x4 *>(qa + n * ldq + 32 * sl) = {q[0], q[1], q[2], q[3]};
-    *reinterpret_cast<i32x4 *>(qa + n * ldq + 32 * sl + 16) = {q[4], q[5], q[6], q[7]};
-    sa[n * (ldq / 32) + sl] = (unsigned char)(e + 127);
+    for (int i = 0; i < 8; ++i) {
+        int const lr = (tid >> 4) + 16 * i;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) sT[(4 * cg + e) * LDT + lr] = v[i][e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        int const c = (wave * 8 + i) * 8 + (lane >> 3);
+        int const col = c >> 2, kb = c & 3;
+        f32x4 const x = *reinterpret_cast<const f32x4 *>(&sT[col * LDT + 32 * kb + 4 * (lane & 7)]);
+        float amax = fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fmaxf(fabsf(x[2]), fabsf(x[3])));
+        amax = fmaxf(amax, __shfl_xor(amax, 1));
+        amax = fmaxf(amax, __shfl_xor(amax, 2));
+        amax = fmaxf(amax, __shfl_xor(amax, 4));
+        int const ex = slice_exponent(amax);
+        uint64_t const row = n0 + col;
+        *reinterpret_cast<int *>(qa + row * ldq + l0 + 32 * kb + 4 * (lane & 7)) = quantize4(x[0], x[1], x[2], x[3], ex);
+        if ((lane & 7) == 0) sa[row * (ldq / 32) + l0 / 32 + kb] = (unsigned char)(ex + 127);
+    }
 }
 
 // The product over the quantized operands: BM rows of B x 64 columns of A
@@ -1294,7 +1315,7 @@ ttm_status_t launch_fp8(const Fold &f, const float *a, const float *b, float *c,
     uint64_t const rows = (uint64_t)nj * bm;
     uint64_t const rows_a = (ncols + 63) / 64 * 64;
     uint64_t const split_blocks = (rows * (ldq / 32) + 255) / 256;
-    uint64_t const split_a_blocks = TN ? (rows_a / 64) * (ldq / 128) : (rows_a * (ldq / 32) + 255) / 256;
+    uint64_t const split_a_blocks = (rows_a / 64) * (ldq / 128);
     if (split_blocks >= (1ull << 31) || split_a_blocks >= (1ull << 31)) {
         set_error(TTM_ERR_ARGUMENT, "operands too large for the FP8 kernels");
         return TTM_ERR_ARGUMENT;
@@ -1320,7 +1341,7 @@ ttm_status_t launch_fp8(const Fold &f, const float *a, const float *b, float *c,
                            f.nq, ldq, ldq / 128);
     else
         hipLaunchKernelGGL(k_fp8_split_a_cols, dim3((unsigned)split_a_blocks), dim3(256), 0, s, a, qa, sa, ncols,
-                           f.nq, f.w, ldq, rows_a);
+                           f.nq, f.w, ldq, ldq / 128);
     if (bm == 128)
         hipLaunchKernelGGL((k_fp8<TN, 128>), dim3(total), dim3(256), 0, s, qa, sa, qb, sb, ldq, c, ncols, f.nq, f.w,
                            f.m, nj, total);
