@@ -48,8 +48,8 @@ METRIC = "TTM TFLOP/s + HBM GB/s vs roofline; BF16x3 vs FP32 at equal FP64-oracl
 PMC_FETCH_SCALE = 2.0
 PMC_WRITE_SCALE = 1.0
 PMC_TRAFFIC_KIB = {
-    "k_gemm<double, 4, 4, 2, 16, false, false>": {"f64-first-q2-m1024": (6264796.8, 4198676.5)},
-    "k_gemm<float, 4, 4, 2, 16, false, false>": {"f32-first-q2-m1024": (3084220.3, 2097152.0)},
+    "k_gemm<double, 4, 4, 2, 16, false, false, false>": {"f64-first-q2-m1024": (6264796.8, 4198676.5)},
+    "k_gemm<float, 4, 4, 2, 16, false, false, false>": {"f32-first-q2-m1024": (3084220.3, 2097152.0)},
     "k_nn_stream<float, 1, 6, true>": {"f32-first-q2-m16": (1048888.5, 32768.0),
                                         "f32-first-q3-m16": (1048761.0, 65536.0)},
     "k_tn_lds<float, 8, 2, 257>": {"f32-first-q1-m16": (1053797.3, 32768.0)},

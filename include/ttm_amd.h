@@ -124,6 +124,31 @@ ttm_status_t ttm_sttm_host_ex(unsigned q, unsigned p,
                               const size_t *pib, float *c, const size_t *nc, const size_t *wc,
                               ttm_precision_t precision);
 
+/* --- contraction-split autotuner ------------------------------------------ */
+/* No reference counterpart (the reference leaves blocking to its BLAS).
+ * Products whose blocks cannot fill the GPU (few columns, or a long
+ * contraction with a small C: the GEMV-like cases, p = 1 included) split the
+ * contraction over the chip and add the partial products in a second,
+ * deterministic pass.  ttm_?tune times the split counts for the problem on
+ * its own device buffers (C is overwritten with the product) and keeps the
+ * fastest for the problem's shape class -- kernel family, element type, view
+ * (q fastest in the layout or not), B's orientation and the power-of-two
+ * classes of m, n_q and the column count -- for every later call of the
+ * process (and in the file named by TTM_AMD_SPLIT_CACHE when set).  *splits
+ * receives the chosen count (1 for kernels that do not split). */
+ttm_status_t ttm_stune(unsigned q, unsigned p,
+                       const float *a, const size_t *na, const size_t *wa, const size_t *pia,
+                       const float *b, const size_t *nb, const size_t *pib,
+                       float *c, const size_t *nc, const size_t *wc,
+                       ttm_stream_t stream, unsigned *splits);
+ttm_status_t ttm_dtune(unsigned q, unsigned p,
+                       const double *a, const size_t *na, const size_t *wa, const size_t *pia,
+                       const double *b, const size_t *nb, const size_t *pib,
+                       double *c, const size_t *nc, const size_t *wc,
+                       ttm_stream_t stream, unsigned *splits);
+/* Forget every tuned split of this process (the file is left alone). */
+void ttm_tune_clear(void);
+
 /* --- batched multi-TTM chains (Tucker-style) ------------------------------ */
 /* No single reference counterpart: generalises ttmpy.ttms (the sequential
  * multi-TTM chain of ttmpy/src/wrapped_ttm.cpp:81-222) to many tensors of
@@ -200,6 +225,16 @@ const char *ttm_plan_kernel(unsigned q, unsigned p, int is_double, const size_t 
 const char *ttm_plan_kernel_symbol(unsigned q, unsigned p, int is_double, const size_t *na,
                                    const size_t *pia, const size_t *nb, const size_t *pib,
                                    ttm_precision_t precision);
+/* Contraction splits the planner uses for a dense problem with 16-byte
+ * aligned buffers (1 = none; 0 on invalid input). */
+unsigned ttm_plan_splits(unsigned q, unsigned p, int is_double, const size_t *na, const size_t *pia,
+                         const size_t *nb, const size_t *pib, ttm_precision_t precision);
+/* Template instance of the kernel that the last successful product call of
+ * this thread ran ("" before any), and its contraction splits (0 before any). */
+const char *ttm_last_kernel(void);
+unsigned ttm_last_splits(void);
+/* Device memory held by the current device's workspace pool (bytes). */
+size_t ttm_workspace_bytes(void);
 /* Library version string. */
 const char *ttm_version(void);
 

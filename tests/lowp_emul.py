@@ -75,20 +75,31 @@ def e4m3fn(v):
     return np.sign(v) * q
 
 
+def _finite_abs(blk):
+    """|x| of finite values, 0 for infinities and NaNs (they set no scale)."""
+    a = np.abs(blk)
+    return np.where(np.isfinite(a), a, 0.0)
+
+
 def fp8_block_quantize(x, axis):
-    """Quantize `x` (float32) along `axis` in blocks of 32 with E8M0 scales."""
+    """Quantize `x` (float32) along `axis` in blocks of 32 with E8M0 scales.
+    Infinities and NaNs set no scale and quantize to NaN (e4m3fn has no
+    infinity)."""
     x = np.asarray(x, dtype=np.float64)
     x = np.moveaxis(x, axis, -1)
     n = x.shape[-1]
     out = np.empty_like(x)
     for b0 in range(0, n, 32):
         blk = x[..., b0:b0 + 32]
-        amax = np.max(np.abs(blk), axis=-1, keepdims=True)
+        amax = np.max(_finite_abs(blk), axis=-1, keepdims=True)
         with np.errstate(divide="ignore"):
             e = np.where(amax > 0, np.ceil(np.log2(np.where(amax > 0, amax, 1.0) / 448.0)), 0.0)
         e = np.where(amax * np.exp2(-e) > 448.0, e + 1, e)
         e = np.clip(e, -126, 127)
-        out[..., b0:b0 + 32] = e4m3fn(np.clip(blk * np.exp2(-e), -448.0, 448.0)) * np.exp2(e)
+        fin = np.isfinite(blk)
+        with np.errstate(invalid="ignore"):
+            q = e4m3fn(np.clip(np.where(fin, blk, 0.0) * np.exp2(-e), -448.0, 448.0)) * np.exp2(e)
+        out[..., b0:b0 + 32] = np.where(fin, q, np.nan)
     return np.moveaxis(out, -1, axis)
 
 
@@ -100,7 +111,7 @@ def fp8_block_quantum(x, axis):
     out = np.empty_like(x)
     for b0 in range(0, x.shape[-1], 32):
         blk = x[..., b0:b0 + 32]
-        amax = np.max(np.abs(blk), axis=-1, keepdims=True)
+        amax = np.max(_finite_abs(blk), axis=-1, keepdims=True)
         with np.errstate(divide="ignore"):
             e = np.where(amax > 0, np.ceil(np.log2(np.where(amax > 0, amax, 1.0) / 448.0)), 0.0)
         e = np.where(amax * np.exp2(-e) > 448.0, e + 1, e)

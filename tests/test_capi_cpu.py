@@ -182,10 +182,36 @@ def test_plan_reference_configs(ttmlib):
                         continue
                     assert k == "gemm", (pia, q, m, k)
                     assert sym.startswith("k_gemm<" + ("double" if dbl else "float"))
-                    assert sym.endswith(("true, true>" if qh == 1 else "false, true>")), sym
-    # odd shapes fall back to the generic kernel
-    assert ttm_amd.plan_kernel(2, [3, 5, 7], [1, 2, 3], [4, 5], [1, 2]) == "generic"
-    assert ttm_amd.plan_kernel(1, [3, 5, 7], [1, 2, 3], [4, 3], [1, 2]) == "generic"
+                    assert sym.endswith(("true, true, false>" if qh == 1 else "false, true, false>")), sym
+
+
+def test_plan_odd_extents_and_underfilled_grids(ttmlib):
+    """Odd extents take the tile kernels (padded NN slabs, element-aligned
+    rows), GEMV-like shapes the skinny kernel, and long contractions with few
+    blocks a contraction split (no GPU needed)."""
+    import ttm_amd
+    S, K, N = ttm_amd.plan_kernel_symbol, ttm_amd.plan_kernel, ttm_amd.plan_splits
+    for dbl in (False, True):
+        t = "double" if dbl else "float"
+        for m in (16, 1024):
+            # w = 999: NN tile with padded slabs
+            assert S(2, [999, 1000, 500], [1, 2, 3], [m, 1000], [1, 2], dbl).endswith("false, true, true>")
+            # n_q = 1023 / 513 with q fastest: TN tile on element-aligned rows
+            assert S(1, [1023, 1024, 512], [1, 2, 3], [m, 1023], [1, 2], dbl).startswith(f"k_gemm<{t}")
+        assert S(3, [1024, 1024, 513], [3, 2, 1], [16, 513], [1, 2], dbl).startswith(f"k_gemm<{t}")
+        assert K(2, [3, 5, 7], [1, 2, 3], [4, 5], [1, 2], dbl) == "gemm"
+        # p = 1 (the reference's GEMV case) and [2^20, 4] with q = 1: skinny, split
+        for m in (1, 16):
+            assert K(1, [1 << 24], [1], [m, 1 << 24], [2, 1], dbl) == "skinny"
+            assert N(1, [1 << 24], [1], [m, 1 << 24], [2, 1], dbl) > 64
+        assert K(1, [1 << 20, 4], [1, 2], [1024, 1 << 20], [1, 2], dbl) == "skinny"
+        assert N(1, [1 << 20, 4], [1, 2], [1024, 1 << 20], [1, 2], dbl) > 8
+        # [16, 16, 2^20] with q = 3: 16 (or 2) blocks -> split along the contraction
+        for m in (16, 1024):
+            assert K(3, [16, 16, 1 << 20], [1, 2, 3], [m, 1 << 20], [1, 2], dbl) == "gemm"
+            assert N(3, [16, 16, 1 << 20], [1, 2, 3], [m, 1 << 20], [1, 2], dbl) >= 16
+        # the bench shapes fill the chip: no split
+        assert N(2, [1024, 1024, 512], [1, 2, 3], [1024, 1024], [2, 1], dbl) == 1
 
 
 def _gxx():

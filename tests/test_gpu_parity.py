@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 
 ROOT = pathlib.Path(__file__).resolve().parents[1]
 GOLD = json.loads((ROOT / "tests" / "golden" / "reference_known_answers.json").read_text())
-FAMILIES = ["generic", "nn_stream", "tn_stream", "gemm"]
+FAMILIES = ["generic", "nn_stream", "tn_stream", "gemm", "skinny"]
 
 
 def _say(*args):

@@ -184,6 +184,41 @@ def test_fp16x2_infinity_sets_no_scale(gpu):
         assert np.all(err <= bound[~touched]), (kernel, float(np.max(err / bound[~touched])))
 
 
+def test_fp8_non_finite_values_set_no_scale(gpu):
+    """FP8 with one infinity and one NaN in A and one infinity in B: the slice
+    scales come from the finite values (the emulation's rule), so every entry
+    of C that involves none of them matches the emulation, and every entry
+    that does is NaN (e4m3fn has no infinity).  A scale taken from an
+    infinity would flush every finite value of its slice to zero."""
+    import torch
+    rng = np.random.default_rng(22)
+    na, pia, q, m = [64, 300, 24], [1, 2, 3], 2, 40
+    for layout, kernel in (([1, 2, 3], None), ([2, 1, 3], None), ([1, 2, 3], "lowp_generic")):
+        pia = layout
+        a, wa, Bm, b, nb, pib, nc, wc = case(rng, na, pia, q, m, False, "uniform")
+        A3 = folded(a, na, pia, q)
+        k0, l0, i0 = 3, 77, min(5, A3.shape[2] - 1)
+        A3[k0, l0, i0] = np.inf
+        A3[k0 + 1, 201, 0] = np.nan
+        Bm[7, 130] = -np.inf
+        b = Bm.reshape(-1).copy()  # A3 is a view of a: a holds the planted values too
+        c = run(torch, gpu, q, a, na, wa, pia, b, nb, pib, nc, wc, "fp8", kernel)
+        c3 = folded(c, nc, pia, q).astype(np.float64)
+        with np.errstate(invalid="ignore"):
+            ref = le.emulate("fp8", A3, Bm)
+        bad_a = ~np.isfinite(A3)
+        bad_b = ~np.isfinite(Bm)
+        touched = (np.einsum("jl,kli->kji", np.ones(Bm.shape), bad_a.astype(np.float64)) > 0) | \
+                  (np.einsum("jl,kli->kji", bad_b.astype(np.float64), np.ones(A3.shape)) > 0)
+        assert np.isnan(c3[touched]).all(), (layout, kernel)
+        assert np.isfinite(c3[~touched]).all(), (layout, kernel)
+        qa = le.fp8_block_quantize(np.where(bad_a, 0, A3), axis=1)
+        qb = le.fp8_block_quantize(np.where(bad_b, 0, Bm), axis=1)
+        bound = 4 * na[q - 1] * 2.0 ** -24 * np.einsum("jl,kli->kji", np.abs(qb), np.abs(qa)) * 1.001
+        err = np.abs(c3 - ref)[~touched]
+        assert np.all(err <= bound[~touched] + 1e-30), (layout, kernel, float(np.max(err)))
+
+
 @pytest.mark.parametrize("prec", ["bf16x3", "fp16x2"])
 def test_split_variants_integer_data_is_exact(gpu, prec):
     """Integers below 2^24 split exactly into three bfloat16 parts, and the

@@ -15,7 +15,7 @@ import refcases as rc
 
 pytestmark = pytest.mark.gpu
 
-FAMILIES = [None, "generic", "nn_stream", "tn_stream", "gemm"]
+FAMILIES = [None, "generic", "nn_stream", "tn_stream", "gemm", "skinny"]
 
 
 def test_random_sweep_all_families_vs_fp64(gpu, oracle):

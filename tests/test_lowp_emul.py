@@ -110,3 +110,25 @@ def test_emulation_matches_float64_for_exact_inputs():
     for prec in ("bf16", "fp16", "bf16x3", "fp8", "fp16x2"):
         np.testing.assert_array_equal(le.emulate(prec, A3, B), exact, err_msg=prec)
     np.testing.assert_array_equal(le.emulate("fp16x2", A3, B, sliced=True), exact)
+
+
+def test_fp8_non_finite_values_set_no_scale():
+    """A block with an infinity and a NaN keeps the scale of its finite values
+    (the same quantized values as with those entries zeroed) and quantizes
+    both non-finite entries to NaN; other blocks are unaffected."""
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-5, 5, (3, 96)).astype(np.float32)
+    y = x.copy()
+    y[1, 40] = np.inf
+    y[1, 41] = np.nan
+    y[2, 90] = -np.inf
+    qx = le.fp8_block_quantize(np.where(np.isfinite(y), y, 0), axis=1)
+    qy = le.fp8_block_quantize(y, axis=1)
+    bad = ~np.isfinite(y)
+    assert np.isnan(qy[bad]).all()
+    assert np.array_equal(qy[~bad], qx[~bad])
+    # the scale is the finite maximum's: a value at the block maximum keeps
+    # e4m3fn precision (relative step 2^-3), not flushed by a huge scale
+    blk = y[1, 32:64]
+    j = int(np.argmax(np.where(np.isfinite(blk), np.abs(blk), 0)))
+    assert abs(qy[1, 32 + j] - blk[j]) <= 2.0 ** -3 * abs(blk[j])

@@ -8,13 +8,16 @@ must cover ceil(n_q / vec) pieces rounded to 32, plus one, and B plus the waves'
 8 KB buffers must fit the 160 KB of LDS of a gfx950 CU.  A forced family whose
 preconditions a problem does not meet is ignored for it.  Invalid metadata
 (the reference's validation failures, include/tlib/ttm.h:65-90) plans nothing.
+The one-thread-per-element kernel runs only when forced: every valid dense
+problem gets a streaming, skinny or tile kernel, the NN tile padding its slabs
+exactly when w % vec != 0.
 """
 import re
 
 import numpy as np
 import pytest
 
-FAMILIES = {"generic", "nn_stream", "tn_stream", "gemm", "lowp", "lowp_generic", "none"}
+FAMILIES = {"generic", "nn_stream", "tn_stream", "gemm", "lowp", "lowp_generic", "skinny", "none"}
 LDS_BYTES = 160 * 1024
 
 
@@ -37,10 +40,11 @@ def _random_problem(rng):
     return q, na, pia, [m, na[q - 1]], pib
 
 
-def _check(q, na, pia, nb, pib, dbl, prec, k, sym):
+def _check(q, na, pia, nb, pib, dbl, prec, k, sym, forced=False):
     m = nb[0]
     vec = 2 if dbl else 4
     w, nq = _fold(q, na, pia)
+    outer = int(np.prod(na, dtype=np.int64)) // (w * nq)
     case = (q, na, pia, nb, pib, dbl, prec, k, sym)
     assert k in FAMILIES, case
     if dbl and prec != 0:
@@ -50,6 +54,13 @@ def _check(q, na, pia, nb, pib, dbl, prec, k, sym):
         assert k in ("lowp", "lowp_generic"), case
         return
     assert k not in ("lowp", "lowp_generic", "none"), case
+    if not forced:
+        assert k != "generic", case
+    if k == "skinny":
+        assert w == 1 and outer <= 8 and nq >= 512, case
+    if k == "gemm":
+        pad = sym.rstrip(">").split(", ")[-1]
+        assert pad == ("true" if (w != 1 and w % vec != 0) else "false"), case
     if k == "nn_stream":
         assert m <= 64 and w % vec == 0, case
     if k == "tn_stream":
@@ -73,11 +84,11 @@ def test_planner_preconditions_on_random_problems(ttmlib):
         sym = ttm_amd.plan_kernel_symbol(q, na, pia, nb, pib, is_double=dbl, precision=prec)
         _check(q, na, pia, nb, pib, dbl, prec, k, sym)
         seen.add(sym.split("<")[0] if sym else k)
-    # the random shapes reach every plain-precision family, both TN kernels included
-    assert {"k_nn_stream", "k_tn_lds", "k_gemm", "k_generic"} <= seen, seen
+    # the random shapes reach every plain-precision family the planner picks
+    assert {"k_nn_stream", "k_tn_lds", "k_gemm"} <= seen, seen
 
 
-@pytest.mark.parametrize("family", ["generic", "nn_stream", "tn_stream", "gemm"])
+@pytest.mark.parametrize("family", ["generic", "nn_stream", "tn_stream", "gemm", "skinny"])
 def test_forced_family_only_where_it_can_run(ttmlib, family):
     import ttm_amd
     rng = np.random.default_rng(11)
@@ -89,7 +100,7 @@ def test_forced_family_only_where_it_can_run(ttmlib, family):
             dbl = bool(rng.random() < 0.4)
             k = ttm_amd.plan_kernel(q, na, pia, nb, pib, is_double=dbl)
             sym = ttm_amd.plan_kernel_symbol(q, na, pia, nb, pib, is_double=dbl)
-            _check(q, na, pia, nb, pib, dbl, 0, k, sym)
+            _check(q, na, pia, nb, pib, dbl, 0, k, sym, forced=True)
             hits += k == family
     finally:
         ttm_amd.force_kernel(None)

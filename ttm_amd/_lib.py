@@ -86,6 +86,14 @@ SIGNATURES = [
     ("ttm_device_copy", ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     ("ttm_device_zero", ctypes.c_int, [_vp, ctypes.c_size_t]),
     ("ttm_device_synchronize", ctypes.c_int, []),
+    ("ttm_stune", ctypes.c_int, _TTM_ARGS + [_vp, ctypes.POINTER(ctypes.c_uint)]),
+    ("ttm_dtune", ctypes.c_int, _TTM_ARGS + [_vp, ctypes.POINTER(ctypes.c_uint)]),
+    ("ttm_tune_clear", None, []),
+    ("ttm_plan_splits", ctypes.c_uint,
+     [ctypes.c_uint, ctypes.c_uint, ctypes.c_int, _size_p, _size_p, _size_p, _size_p, ctypes.c_int]),
+    ("ttm_last_kernel", ctypes.c_char_p, []),
+    ("ttm_last_splits", ctypes.c_uint, []),
+    ("ttm_workspace_bytes", ctypes.c_size_t, []),
 ]
 
 _lib = None
@@ -146,6 +154,29 @@ def plan_kernel_symbol(q: int, na, pia, nb, pib, is_double: bool = False, precis
     p = len(na)
     return lib().ttm_plan_kernel_symbol(q, p, int(is_double), sizes(na), sizes(pia), sizes(nb), sizes(pib),
                                         int(precision)).decode()
+
+
+def plan_splits(q: int, na, pia, nb, pib, is_double: bool = False, precision: int = 0) -> int:
+    """Contraction splits the planner uses for a dense, aligned problem."""
+    p = len(na)
+    return int(lib().ttm_plan_splits(q, p, int(is_double), sizes(na), sizes(pia), sizes(nb), sizes(pib),
+                                     int(precision)))
+
+
+def last_kernel() -> tuple[str, int]:
+    """(kernel template instance, contraction splits) of this thread's last product."""
+    L = lib()
+    return L.ttm_last_kernel().decode(), int(L.ttm_last_splits())
+
+
+def tune_clear() -> None:
+    """Forget every tuned contraction split of this process."""
+    lib().ttm_tune_clear()
+
+
+def workspace_bytes() -> int:
+    """Device memory held by the current device's workspace pool."""
+    return int(lib().ttm_workspace_bytes())
 
 
 def version() -> str:

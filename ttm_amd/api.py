@@ -38,8 +38,86 @@ def _pointer(x) -> int:
     raise TypeError(f"unsupported buffer type {type(x).__name__}")
 
 
-def _on_device(*xs) -> bool:
-    return all(_is_torch(x) and x.is_cuda for x in xs)
+def _residency(x) -> str:
+    """'device' (ROCm torch tensor), 'host' (NumPy array or CPU torch tensor)
+    or 'raw' (an integer device address)."""
+    if _is_torch(x):
+        return "device" if x.is_cuda else "host"
+    if isinstance(x, np.ndarray):
+        return "host"
+    if isinstance(x, int) and not isinstance(x, bool):
+        return "raw"
+    raise TypeError(f"unsupported buffer type {type(x).__name__}")
+
+
+def _dense_numel(x):
+    """Elements of x when its strides describe a gap-free, non-overlapping
+    block starting at its data pointer (any mode order), else None."""
+    if _is_torch(x):
+        shape, strides = list(x.shape), list(x.stride())
+    else:
+        item = x.dtype.itemsize
+        if any(s % item for s in x.strides):
+            return None
+        shape, strides = list(x.shape), [s // item for s in x.strides]
+    n = 1
+    for e in shape:
+        n *= int(e)
+    if n == 0:
+        return 0
+    expect = 1
+    for s, e in sorted((s, e) for s, e in zip(strides, shape) if e > 1):
+        if s != expect:
+            return None
+        expect *= e
+    return n
+
+
+def _prod(xs) -> int:
+    n = 1
+    for v in xs:
+        n *= int(v)
+    return n
+
+
+def _check_buffers(p, a, na, b, nb, c, nc, dtype=None):
+    """Residency, element type and size of A, B and C, checked before any
+    call into the library (which sees only pointers).  Returns (device, dt)."""
+    res = {_residency(x) for x in (a, b, c)}
+    if len(res) != 1:
+        raise TypeError("A, B and C must all be ROCm device tensors, all host arrays, or all raw device "
+                        f"addresses (got {sorted(res)})")
+    kind = res.pop()
+    if kind == "raw":
+        if dtype is None:
+            raise TypeError("raw device addresses need dtype= (float32 or float64)")
+        dt = np.dtype(dtype).type
+        if dt not in (np.float32, np.float64):
+            raise TypeError("ttm supports float32 and float64 elements")
+        return True, dt
+    dt = _dtype_of(a)
+    if dt not in (np.float32, np.float64):
+        raise TypeError("ttm supports float32 and float64 elements")
+    for name, x in (("B", b), ("C", c)):
+        if _dtype_of(x) != dt:
+            raise TypeError(f"{name} must have A's element type {np.dtype(dt).name}")
+    if kind == "device" and not (a.device == b.device == c.device):
+        raise TypeError("A, B and C must be on the same device")
+    # Buffer sizes, when the extents are well formed (malformed metadata is
+    # left to the library, which reports the reference's error for it).
+    try:
+        shapes_ok = (len(na) == p and len(nc) == p and len(nb) == 2 and
+                     all(int(v) >= 0 for v in (*na, *nb, *nc)))
+    except TypeError:
+        shapes_ok = False
+    if shapes_ok:
+        for name, x, need in (("A", a, _prod(na)), ("B", b, _prod(nb)), ("C", c, _prod(nc))):
+            have = _dense_numel(x)
+            if have is None:
+                raise ValueError(f"{name} must be one dense block of memory")
+            if have < need:
+                raise ValueError(f"{name} holds {have} elements, its extents need {need}")
+    return kind == "device", dt
 
 
 def _precision(precision) -> int:
@@ -57,18 +135,18 @@ def _stream_handle(stream) -> int:
     return int(torch.cuda.current_stream().cuda_stream)
 
 
-def ttm_raw(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, *, precision=None, stream=None):
+def ttm_raw(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, *, precision=None, stream=None, dtype=None):
     """The reference's low-level call: C (dense, layout pia) = A x_q B.
 
     a, b, c are NumPy arrays (host memory, synchronous call), torch tensors on
     a ROCm device (asynchronous on ``stream``, default the current stream), or
-    raw device addresses.  na, wa, pia, nb, pib, nc, wc are integer sequences
+    raw device addresses (all three, with ``dtype``).  Mixed residencies,
+    element types other than A's and buffers smaller than their extents raise
+    before the library is called.  na, wa, pia, nb, pib, nc, wc are integer sequences
     as in the reference.  Raises ``TTMError`` with the reference's message on
     invalid arguments.
     """
-    dt = _dtype_of(a)
-    if dt not in (np.float32, np.float64):
-        raise TypeError("ttm supports float32 and float64 elements")
+    device, dt = _check_buffers(p, a, na, b, nb, c, nc, dtype)
     prec = _precision(precision)
     args = [ctypes.c_uint(int(q)), ctypes.c_uint(int(p)),
             _pointer(a), sizes(na), sizes(wa), sizes(pia),
@@ -77,7 +155,6 @@ def ttm_raw(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, *, precision=None, stre
     L = lib()
     if dt == np.float64 and prec:
         raise TTMError(int(Status.PRECISION), L.ttm_status_message(int(Status.PRECISION)).decode())
-    device = _on_device(a, b, c) or isinstance(a, int)
     if device:
         s = ctypes.c_void_p(_stream_handle(stream))
         if dt == np.float32:
@@ -90,6 +167,24 @@ def ttm_raw(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, *, precision=None, stre
         st = L.ttm_dttm_host(*args)
     check(st)
     return c
+
+
+def tune_raw(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, *, stream=None, dtype=None):
+    """Time the contraction-split counts of this product on its own device
+    buffers (C receives the product) and keep the fastest for its shape class
+    for later calls of the process.  Returns the chosen split count (1 for
+    kernels that do not split)."""
+    device, dt = _check_buffers(p, a, na, b, nb, c, nc, dtype)
+    if not device:
+        raise TypeError("tuning needs device buffers (ROCm torch tensors or raw device addresses)")
+    L = lib()
+    out = ctypes.c_uint(0)
+    args = [ctypes.c_uint(int(q)), ctypes.c_uint(int(p)),
+            _pointer(a), sizes(na), sizes(wa), sizes(pia),
+            _pointer(b), sizes(nb), sizes(pib),
+            _pointer(c), sizes(nc), sizes(wc), ctypes.c_void_p(_stream_handle(stream)), ctypes.byref(out)]
+    check((L.ttm_stune if dt == np.float32 else L.ttm_dtune)(*args))
+    return int(out.value)
 
 
 def layout_from_strides(shape, strides):
@@ -149,6 +244,17 @@ def ttm(q, A, B, *, layout=None, b_layout="row", precision=None, out=None, strea
         raise ValueError("ttm: negative strides are not supported")
     pia = list(layout) if layout is not None else layout_from_strides(na, wa)
     colmajor = b_layout in ("col", "column", "F")
+    # B takes A's element type and residency; `out` must already have both.
+    if _is_torch(A):
+        if not _is_torch(B):
+            B = torch.from_numpy(np.asarray(B))
+        B = B.to(device=A.device, dtype=A.dtype)
+    else:
+        B = B.cpu().numpy() if _is_torch(B) else np.asarray(B)
+        B = B.astype(A.dtype, copy=False)
+    if out is not None:
+        if _residency(out) != _residency(A) or _dtype_of(out) != _dtype_of(A):
+            raise TypeError("out must have A's element type and residency")
     if _is_torch(B):
         Bc = B.t().contiguous().t() if colmajor else B.contiguous()
     else:

@@ -72,6 +72,8 @@ def make_desc(A, factors, C, layout=None):
         raise ValueError("one factor (or None) per mode is required")
     if A.dtype != C.dtype or A.dtype not in (torch.float32, torch.float64):
         raise TypeError("A and C must both be float32 or both float64")
+    if not (A.is_cuda and C.is_cuda) or A.device != C.device:
+        raise TypeError("A and C must be ROCm device tensors on one device")
     na = list(A.shape)
     pia = list(layout) if layout is not None else layout_from_strides(na, list(A.stride()))
     if not _dense_in(na, list(A.stride()), pia):
@@ -83,6 +85,8 @@ def make_desc(A, factors, C, layout=None):
         d.pi[k] = pia[k]
         U = factors[k]
         if U is not None:
+            if not isinstance(U, torch.Tensor) or not U.is_cuda or U.device != A.device:
+                raise TypeError(f"factor {k + 1} must be a ROCm device tensor on A's device")
             if U.dim() != 2 or U.shape[1] != na[k] or U.dtype != A.dtype or not U.is_contiguous():
                 raise ValueError(f"factor {k + 1} must be a contiguous r x {na[k]} matrix of A's dtype")
             d.u[k] = U.data_ptr()
@@ -108,6 +112,9 @@ class ChainPlan:
         self._items = list(items)
         if not self._items:
             raise ValueError("a chain plan needs at least one tensor")
+        dtypes = {it[0].dtype for it in self._items}
+        if len(dtypes) != 1:
+            raise TypeError("every tensor of a chain plan must have the same element type")
         descs = (ChainDesc * len(self._items))(*[make_desc(*it) for it in self._items])
         self._descs = descs
         is_double = self._items[0][0].dtype == torch.float64

@@ -12,6 +12,8 @@
 #include "ttm_amd.h"
 #include "ttm_kernels.hpp"
 #include "ttm_plan.hpp"
+#include "ttm_tune.hpp"
+#include "ttm_workspace.hpp"
 
 using namespace ttm_amd;
 
@@ -193,6 +195,46 @@ hipError_t copy_d2h(void *dst, const void *src, size_t bytes)
     return e != hipSuccess ? e : e2;
 }
 
+// The last product this thread launched (ttm_last_kernel / ttm_last_splits).
+struct LastRun
+{
+    LaunchInfo info;
+    Fold fold;
+    bool is_double = false;
+    ttm_precision_t prec = TTM_PREC_DEFAULT;
+    bool valid = false;
+};
+thread_local LastRun t_last;
+thread_local std::string t_last_symbol;
+
+void note_run(const LaunchInfo &info, const Fold &f, bool is_double, ttm_precision_t prec)
+{
+    t_last.info = info;
+    t_last.fold = f;
+    t_last.is_double = is_double;
+    t_last.prec = prec;
+    t_last.valid = true;
+}
+
+template <class T>
+ttm_status_t launch_typed(const Fold &f, const T *a, const T *b, T *c, ttm_precision_t prec, hipStream_t s,
+                          unsigned want_splits)
+{
+    LaunchInfo info;
+    ttm_status_t st;
+    if constexpr (std::is_same_v<T, float>) {
+        st = launch_f32(f, a, b, c, prec, s, g_force, &info, want_splits);
+    } else {
+        if (prec != TTM_PREC_DEFAULT) {
+            set_error(TTM_ERR_PRECISION);
+            return TTM_ERR_PRECISION;
+        }
+        st = launch_f64(f, a, b, c, s, g_force, &info, want_splits);
+    }
+    if (st == TTM_SUCCESS) note_run(info, f, std::is_same_v<T, double>, prec);
+    return st;
+}
+
 // Device pointer path shared by the typed entries.
 template <class T>
 ttm_status_t run_device(unsigned q, unsigned p, const T *a, const size_t *na, const size_t *wa,
@@ -202,18 +244,75 @@ ttm_status_t run_device(unsigned q, unsigned p, const T *a, const size_t *na, co
     Fold f;
     ttm_status_t st = plan(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, f);
     if (st != TTM_SUCCESS) return st;
-    Kernel used = Kernel::none;
-    if constexpr (std::is_same_v<T, float>) {
-        st = launch_f32(f, a, b, c, prec, s, g_force, &used);
-    } else {
-        if (prec != TTM_PREC_DEFAULT) {
-            set_error(TTM_ERR_PRECISION);
-            return TTM_ERR_PRECISION;
-        }
-        st = launch_f64(f, a, b, c, s, g_force, &used);
-    }
+    st = launch_typed<T>(f, a, b, c, prec, s, 0);
     if (st == TTM_SUCCESS) clear_error();
     return st;
+}
+
+// Time the split candidates of a product on its own buffers (C is
+// overwritten) and cache the fastest for its shape class.
+template <class T>
+ttm_status_t run_tune(unsigned q, unsigned p, const T *a, const size_t *na, const size_t *wa, const size_t *pia,
+                      const T *b, const size_t *nb, const size_t *pib, T *c, const size_t *nc, const size_t *wc,
+                      hipStream_t s, unsigned *splits_out)
+{
+    Fold f;
+    ttm_status_t st = plan(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, f);
+    if (st != TTM_SUCCESS) return st;
+    bool const dbl = std::is_same_v<T, double>;
+    // the family the planner picks for these buffers (no split yet)
+    LaunchInfo info;
+    st = launch_typed<T>(f, a, b, c, TTM_PREC_DEFAULT, s, 1);
+    if (st != TTM_SUCCESS) return st;
+    info = t_last.info;
+    if (info.kernel != Kernel::gemm && info.kernel != Kernel::skinny) {
+        if (splits_out) *splits_out = 1;
+        clear_error();
+        return TTM_SUCCESS;
+    }
+    // candidates: 1 and powers of two up to the family's limit, deduplicated
+    // by the split count they actually give
+    std::vector<unsigned> cand;
+    for (unsigned want = 1; want <= 65535; want *= 2) {
+        unsigned const got = launch_splits(info.kernel, f, dbl, want);
+        if (std::find(cand.begin(), cand.end(), got) == cand.end()) cand.push_back(got);
+        if (got < want) break;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return hip_fail(hipGetLastError());
+    }
+    unsigned best = 1;
+    float best_ms = 0;
+    for (unsigned want : cand) {
+        float ms_min = 0;
+        for (int rep = 0; rep < 4 && st == TTM_SUCCESS; ++rep) {
+            (void)hipEventRecord(e0, s);
+            st = launch_typed<T>(f, a, b, c, TTM_PREC_DEFAULT, s, want);
+            (void)hipEventRecord(e1, s);
+            if (st != TTM_SUCCESS) break;
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                st = hip_fail(hipGetLastError());
+                break;
+            }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (rep > 0 && (rep == 1 || ms < ms_min)) ms_min = ms; // rep 0 warms up
+        }
+        if (st != TTM_SUCCESS) break;
+        if (want == cand.front() || ms_min < best_ms) {
+            best = want;
+            best_ms = ms_min;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != TTM_SUCCESS) return st;
+    store_tuned_splits(info.kernel, f, dbl, best);
+    if (splits_out) *splits_out = best;
+    clear_error();
+    return TTM_SUCCESS;
 }
 
 // Host (or mixed) pointer path: validates first, stages every buffer that is
@@ -238,42 +337,39 @@ ttm_status_t run_host(unsigned q, unsigned p, const T *a, const size_t *na, cons
     size_t const bytes_c = f.numel_c() * sizeof(T);
     bool const a_dev = is_device_ptr(a), b_dev = is_device_ptr(b), c_dev = is_device_ptr(c);
 
+    // Device copies of the host buffers come from the per-device pool, which
+    // keeps its memory: repeated calls of one size allocate nothing new.
     T *da = nullptr, *db = nullptr, *dc = nullptr;
     auto release = [&]() {
-        if (!a_dev && da) (void)hipFree(da);
-        if (!b_dev && db) (void)hipFree(db);
-        if (!c_dev && dc) (void)hipFree(dc);
+        if (!a_dev && da) ws_release(da, nullptr);
+        if (!b_dev && db) ws_release(db, nullptr);
+        if (!c_dev && dc) ws_release(dc, nullptr);
     };
     auto alloc = [&](T **dst, size_t bytes) -> bool {
-        if (hipMalloc(reinterpret_cast<void **>(dst), bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            *dst = nullptr;
-            return false;
-        }
+        void *ptr = nullptr;
+        if (ws_acquire(bytes, nullptr, &ptr, "device copy of a host buffer") != TTM_SUCCESS) return false;
+        *dst = static_cast<T *>(ptr);
         return true;
     };
     if ((!a_dev && !alloc(&da, bytes_a)) || (!b_dev && !alloc(&db, bytes_b)) ||
         (!c_dev && !alloc(&dc, bytes_c))) {
         release();
-        set_error(TTM_ERR_ALLOC);
         return TTM_ERR_ALLOC;
     }
     if (a_dev) da = const_cast<T *>(a);
     if (b_dev) db = const_cast<T *>(b);
     if (c_dev) dc = c;
 
-    hipError_t e = hipSuccess;
-    if (!a_dev) e = copy_h2d(da, a, bytes_a);
+    // the pool hands out memory in null-stream order; the staging copies run
+    // on their own stream
+    hipError_t e = hipStreamSynchronize(nullptr);
+    if (e == hipSuccess && !a_dev) e = copy_h2d(da, a, bytes_a);
     if (e == hipSuccess && !b_dev) e = copy_h2d(db, b, bytes_b);
     if (e != hipSuccess) {
         release();
         return hip_fail(e);
     }
-    Kernel used = Kernel::none;
-    if constexpr (std::is_same_v<T, float>)
-        st = launch_f32(f, da, db, dc, prec, nullptr, g_force, &used);
-    else
-        st = launch_f64(f, da, db, dc, nullptr, g_force, &used);
+    st = launch_typed<T>(f, da, db, dc, prec, nullptr, 0);
     if (st == TTM_SUCCESS) {
         e = hipDeviceSynchronize(); // the product ran on the null stream; the staging stream is separate
         if (e == hipSuccess && !c_dev) e = copy_d2h(c, dc, bytes_c);
@@ -362,6 +458,52 @@ bool fold_dense(unsigned q, unsigned p, const size_t *na, const size_t *pia, con
 }
 thread_local std::string g_symbol;
 } // namespace
+
+unsigned ttm_plan_splits(unsigned q, unsigned p, int is_double, const size_t *na, const size_t *pia,
+                         const size_t *nb, const size_t *pib, ttm_precision_t precision)
+{
+    Fold f;
+    if (!fold_dense(q, p, na, pia, nb, pib, f)) return 0;
+    Kernel const k = choose_kernel(f, is_double != 0, precision, true, g_force);
+    return launch_splits(k, f, is_double != 0, 0);
+}
+
+const char *ttm_last_kernel(void)
+{
+    if (!t_last.valid) return "";
+    t_last_symbol = kernel_symbol(t_last.info.kernel, t_last.fold, t_last.is_double, t_last.prec, t_last.info.aligned);
+    return t_last_symbol.c_str();
+}
+
+unsigned ttm_last_splits(void) { return t_last.valid ? t_last.info.splits : 0; }
+
+ttm_status_t ttm_stune(unsigned q, unsigned p, const float *a, const size_t *na, const size_t *wa,
+                       const size_t *pia, const float *b, const size_t *nb, const size_t *pib, float *c,
+                       const size_t *nc, const size_t *wc, ttm_stream_t stream, unsigned *splits)
+{
+    return run_tune<float>(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, static_cast<hipStream_t>(stream), splits);
+}
+
+ttm_status_t ttm_dtune(unsigned q, unsigned p, const double *a, const size_t *na, const size_t *wa,
+                       const size_t *pia, const double *b, const size_t *nb, const size_t *pib, double *c,
+                       const size_t *nc, const size_t *wc, ttm_stream_t stream, unsigned *splits)
+{
+    return run_tune<double>(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, static_cast<hipStream_t>(stream), splits);
+}
+
+void ttm_tune_clear(void) { clear_tuned_splits(); }
+
+size_t ttm_workspace_bytes(void)
+{
+    hipMemPool_t const pool = workspace_pool();
+    if (!pool) return 0;
+    uint64_t v = 0;
+    if (hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &v) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return (size_t)v;
+}
 
 const char *ttm_plan_kernel(unsigned q, unsigned p, int is_double, const size_t *na, const size_t *pia,
                             const size_t *nb, const size_t *pib, ttm_precision_t precision)

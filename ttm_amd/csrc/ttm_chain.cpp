@@ -344,7 +344,9 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
             P.c = out;
             bool const a_al = (reinterpret_cast<uintptr_t>(P.a) & 15u) == 0;
             bool const c_al = (reinterpret_cast<uintptr_t>(P.c) & 15u) == 0;
-            P.flags = ((a_al && P.nq % vec == 0) ? TILE_A_VEC : 0u) | ((c_al && ncols % vec == 0) ? TILE_C_VEC : 0u);
+            bool const b_al = (reinterpret_cast<uintptr_t>(P.b) & 15u) == 0;
+            P.flags = ((a_al && P.nq % vec == 0) ? TILE_A_VEC : 0u) | ((c_al && ncols % vec == 0) ? TILE_C_VEC : 0u) |
+                      (b_al ? TILE_B_VEC : 0u);
             size_t const g = (size_t)s * classes + m_class(P.m);
             groups[g].probs.push_back(P);
             stage_max_m[g] = std::max(stage_max_m[g], P.m);

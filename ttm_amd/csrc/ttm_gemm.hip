@@ -3,8 +3,15 @@
 //     D[j][n] = sum_l B(j, l) * At[l][n]       j < m, n < ncols, l < nq
 //
 // with two views of the column index n:
-//   NN (w % VEC == 0):  n = k*w + i,  At[l][n] = A[k*nq*w + l*w + i],  C at k*m*w + j*w + i
-//   TN (w == 1):        n = k,        At[l][n] = A[k*nq + l],          C at k*m + j
+//   NN (any w):   n = k*wp + i (i < wp, the columns i >= w of each slab are
+//                 padding, wp = w rounded up to the vector width unless w is
+//                 a multiple of it), At[l][n] = A[k*nq*w + l*w + i], C at
+//                 k*m*w + j*w + i
+//   TN (w == 1):  n = k,  At[l][n] = A[k*nq + l],  C at k*m + j
+//
+// A block may cover only the contraction range [l0, l0 + len) of a split
+// product (split-K): A and B are then offset to l0 by the caller, `nq` is the
+// block's length and `ld` the full extent that the strides are made of.
 //
 // A workgroup of 4 waves (WAVES_M x WAVES_N) computes a BM x BN block of D,
 // stepping through the contraction in BK-slices that are staged in LDS with
@@ -22,7 +29,9 @@
 // Afrag[lc][lr], srcB holds Bfrag[lr][lc], accumulator element r holds
 // D[acc_row(lr, r)][lc] with acc_row = 4*lr + r (f32) or 4*r + lr (f64).
 #include "ttm_gemm.hpp"
+#include "ttm_workspace.hpp"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -172,10 +181,12 @@ struct TileShape
 
 // One BM x BN block of D starting at (j0, n0); sA / sB are the block's two
 // LDS buffers per operand.  Begins and ends with all waves synchronised.
-template <class T, int JT, int NT, int WAVES_M, int BK, int VIEW, bool BCOL>
+// PAD (NN view only): slabs of any width w and element-aligned A and C; off,
+// the NN view requires w % VEC == 0 and 16-byte aligned A and C (wp == w).
+template <class T, int JT, int NT, int WAVES_M, int BK, int VIEW, bool BCOL, bool PAD = false>
 __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__restrict__ b, T *__restrict__ c,
-                                          uint64_t ncols, uint64_t nq, uint64_t w, uint64_t m, uint64_t j0,
-                                          uint64_t n0, unsigned flags,
+                                          uint64_t ncols, uint64_t nq, uint64_t ld, uint64_t w, uint64_t wp,
+                                          uint64_t m, uint64_t j0, uint64_t n0, unsigned flags,
                                           T (*sA)[BK * TileShape<T, JT, NT, WAVES_M, BK>::LDA],
                                           T (*sB)[BK * TileShape<T, JT, NT, WAVES_M, BK>::LDB])
 {
@@ -194,6 +205,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     bool const a_vec = (flags & TILE_A_VEC) != 0;
     bool const c_vec = (flags & TILE_C_VEC) != 0;
     bool const stream = (flags & TILE_STREAM) != 0;
+    bool const nt_c = stream && (flags & TILE_KEEP_C) == 0;
     // 16-byte loads of A: non-temporal when A is streamed once (measured on
     // MI355X: ~7.1 TB/s for non-temporal against ~6.2 TB/s for ordinary
     // 16-byte read streams).  In the TN/TT views a slice reads BK * sizeof(T)
@@ -202,7 +214,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     // otherwise consecutive slices share lines and a non-temporal line is gone
     // before its second half is read (FETCH_SIZE 1.33x the bytes of A).
     constexpr bool TN_WHOLE_LINES = BK * sizeof(T) % 128 == 0;
-    bool const nt_a = stream && (!TN || (TN_WHOLE_LINES && nq * sizeof(T) % 128 == 0));
+    bool const nt_a = stream && (!TN || (TN_WHOLE_LINES && ld * sizeof(T) % 128 == 0));
     auto load_a = [&](const T *p) -> V {
         return nt_a ? ldg_nt<V>(p) : ldg<V>(p);
     };
@@ -225,27 +237,37 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     constexpr int B_VECS = BK * BM / VEC;
     constexpr int B_PER = (B_VECS + 255) / 256;
 
-    // per-thread fixed part of the A addresses
+    // per-thread fixed part of the A addresses; NN vectors are VEC padded
+    // columns of one slab, of which a_nv are real (i < w)
     uint64_t a_base[A_PER];
     bool a_ok[A_PER];
-    int a_row[A_PER], a_col[A_PER];
+    int a_row[A_PER], a_col[A_PER], a_nv[A_PER];
 #pragma unroll
     for (int e = 0; e < A_PER; ++e) {
         int const v = tid + 256 * e;
         a_ok[e] = v < A_VECS;
+        a_nv[e] = VEC;
         if constexpr (!TN) {
             a_row[e] = v / (BN / VEC);
             a_col[e] = (v % (BN / VEC)) * VEC;
             uint64_t const n = n0 + a_col[e];
             a_ok[e] = a_ok[e] && n < ncols;
-            uint64_t const k = a_ok[e] ? n / w : 0;
-            a_base[e] = k * nq * w + (a_ok[e] ? n - k * w : 0);
+            if constexpr (PAD) {
+                uint64_t const k = a_ok[e] ? n / wp : 0;
+                uint64_t const i = a_ok[e] ? n - k * wp : 0;
+                a_ok[e] = a_ok[e] && i < w;
+                a_nv[e] = a_ok[e] ? (int)(w - i < (uint64_t)VEC ? w - i : VEC) : 0;
+                a_base[e] = a_ok[e] ? k * ld * w + i : 0;
+            } else {
+                uint64_t const k = a_ok[e] ? n / w : 0;
+                a_base[e] = k * ld * w + (a_ok[e] ? n - k * w : 0);
+            }
         } else {
             a_col[e] = v / (BK / VEC);          // column n within the tile
             a_row[e] = (v % (BK / VEC)) * VEC;  // first l within the slice
             uint64_t const n = n0 + a_col[e];
             a_ok[e] = a_ok[e] && n < ncols;
-            a_base[e] = a_ok[e] ? n * nq : 0;
+            a_base[e] = a_ok[e] ? n * ld : 0;
         }
     }
     int b_row[B_PER], b_col[B_PER];
@@ -262,7 +284,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
             b_row[e] = (v % (BK / VEC)) * VEC;
         }
     }
-    bool const b_vec_ok = BCOL ? (m % VEC == 0) : (nq % VEC == 0);
+    bool const b_vec_ok = (flags & TILE_B_VEC) != 0 && (BCOL ? (m % VEC == 0) : (ld % VEC == 0));
 
     // Register sets of slices in flight: two for tiles of at most 32 rows per
     // wave (little matrix work per slice to hide a load behind: the loads of
@@ -278,7 +300,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
 
     // Interior tile (every row and column in range, whole slices, 16-byte
     // vectors throughout): unguarded loads, all in flight together.
-    bool const full = n0 + BN <= ncols && j0 + BM <= m && nq % BK == 0 && (!TN || a_vec) && b_vec_ok;
+    bool const full = n0 + BN <= ncols && j0 + BM <= m && nq % BK == 0 && (a_vec || (!TN && !PAD)) && b_vec_ok;
     uint64_t a_fix[A_PER]; // A offset of the thread's vector in slice 0
 #pragma unroll
     for (int e = 0; e < A_PER; ++e) a_fix[e] = TN ? a_base[e] + a_row[e] : a_base[e] + (uint64_t)a_row[e] * w;
@@ -292,7 +314,24 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
             ra_dm[S][e] = DM_WHOLE;
             if constexpr (!TN) {
                 uint64_t const l = l0 + a_row[e];
-                if (a_ok[e] && l < nq) ra[S][e] = load_a(a + a_base[e] + l * w);
+                if (a_ok[e] && l < nq) {
+                    if (!PAD || a_vec) {
+                        ra[S][e] = load_a(a + a_base[e] + l * w);
+                    } else if (w >= (uint64_t)VEC) {
+                        // element-aligned vector; a partial last vector of the
+                        // slab row is read as the row's last VEC elements and
+                        // shifted into place when stored
+                        int const d = VEC - a_nv[e];
+                        V const v = ldg<typename Gt::vec_u>(a + a_base[e] + l * w - d);
+                        ra[S][e] = v;
+                        ra_dm[S][e] = d | a_nv[e] << 8;
+                    } else {
+                        // rows shorter than a vector
+#pragma unroll
+                        for (int x = 0; x < VEC; ++x)
+                            if (x < a_nv[e]) ra[S][e][x] = ldg<T>(a + a_base[e] + l * w + x);
+                    }
+                }
             } else if (nq >= (uint64_t)VEC) {
                 // One in-bounds vector per thread and no divergent branch (a
                 // per-element fill of a vector register whose load is still in
@@ -353,7 +392,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
                 uint64_t const l = l0 + b_row[e];
                 uint64_t const last = nq - VEC;
                 uint64_t const lc = l < last ? l : last;
-                V const v = ldg<typename Gt::vec_u>(b + jc * nq + lc);
+                V const v = ldg<typename Gt::vec_u>(b + jc * ld + lc);
                 int const nvalid = l >= nq ? 0 : (int)(nq - l < (uint64_t)VEC ? nq - l : VEC);
                 rb[S][e] = v;
                 rb_dm[S][e] = (int)(l - lc) | nvalid << 8;
@@ -362,13 +401,13 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
                 uint64_t const l = l0 + b_row[e];
                 if (j < m) {
                     if (l + VEC <= nq) {
-                        // element-aligned vector inside row j (16-byte aligned when nq % VEC == 0)
-                        V const v = ldg<typename Gt::vec_u>(b + j * nq + l);
+                        // element-aligned vector inside row j (16-byte aligned when ld % VEC == 0)
+                        V const v = ldg<typename Gt::vec_u>(b + j * ld + l);
                         rb[S][e] = v;
                     } else {
 #pragma unroll
                         for (int x = 0; x < VEC; ++x)
-                            if (l + x < nq) rb[S][e][x] = ldg<T>(b + j * nq + l + x);
+                            if (l + x < nq) rb[S][e][x] = ldg<T>(b + j * ld + l + x);
                     }
                 }
             }
@@ -390,7 +429,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
             uint64_t const jj = b_in[e] ? (uint64_t)b_col[e] : 0;
             uint64_t const ll = b_in[e] ? (uint64_t)b_row[e] : 0;
             if constexpr (BCOL) rb[S][e] = ldg<V>(b + j0 + jj + (l0 + ll) * m);
-            else rb[S][e] = ldg<V>(b + (j0 + jj) * nq + l0 + ll);
+            else rb[S][e] = ldg<V>(b + (j0 + jj) * ld + l0 + ll);
         }
     };
 
@@ -411,7 +450,10 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
         for (int e = 0; e < A_PER; ++e) {
             if (tid + 256 * e >= A_VECS) continue;
             if constexpr (!TN) {
-                *reinterpret_cast<V *>(&sA[buf][a_row[e] * LDA + a_col[e]]) = ra[S][e];
+                V const v = (PAD && GUARDED && !a_vec)
+                                ? shift_mask(ra[S][e], ra_dm[S][e] & 0xff, ra_dm[S][e] >> 8)
+                                : ra[S][e];
+                *reinterpret_cast<V *>(&sA[buf][a_row[e] * LDA + a_col[e]]) = v;
             } else {
                 V const v = GUARDED ? shift_mask(ra[S][e], ra_dm[S][e] & 0xff, ra_dm[S][e] >> 8) : ra[S][e];
 #pragma unroll
@@ -505,17 +547,27 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     uint64_t const ncol0 = n0 + a_off; // first of this lane's NT adjacent columns
     if constexpr (!TSTORE) {
         // one row j, NT adjacent columns; each VEC-group of them lies in one
-        // slab (w % VEC == 0), but the NT columns may span two slabs.
+        // slab (wp % VEC == 0), of which gnv are real columns, but the NT
+        // columns may span two slabs.
         constexpr int NG = NT / VEC;
         static_assert(NT % VEC == 0, "NT");
         T *cgroup[NG];
-        bool gok[NG];
+        int gnv[NG];
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
             uint64_t const n = ncol0 + g * VEC;
-            gok[g] = n < ncols;
-            uint64_t const k = gok[g] ? n / w : 0;
-            cgroup[g] = c + k * m * w + (gok[g] ? n - k * w : 0);
+            bool const in = n < ncols;
+            if constexpr (VIEW == VIEW_NN && !PAD) {
+                // w % VEC == 0: every group is whole
+                uint64_t const k = in ? n / w : 0;
+                gnv[g] = in ? VEC : 0;
+                cgroup[g] = c + k * m * w + (in ? n - k * w : 0);
+            } else {
+                uint64_t const k = in ? n / wp : 0;
+                uint64_t const i = in ? n - k * wp : 0;
+                gnv[g] = (in && i < w) ? (int)(w - i < (uint64_t)VEC ? w - i : VEC) : 0;
+                cgroup[g] = c + k * m * w + i;
+            }
         }
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt)
@@ -525,18 +577,24 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
                 if (j >= m) continue;
 #pragma unroll
                 for (int g = 0; g < NG; ++g) {
-                    if (!gok[g]) continue;
-                    if (c_vec) {
+                    if (gnv[g] == 0) continue;
+                    if (c_vec || (VIEW == VIEW_NN && !PAD)) {
                         V o;
 #pragma unroll
                         for (int x = 0; x < VEC; ++x) o[x] = acc[jt][g * VEC + x][r];
-                        if (stream) stg_nt<V>(cgroup[g] + j * w, o);
+                        if (nt_c) stg_nt<V>(cgroup[g] + j * w, o);
                         else stg<V>(cgroup[g] + j * w, o);
+                    } else if (VIEW == VIEW_NN && gnv[g] == VEC) {
+                        // whole group at an element-aligned address
+                        V o;
+#pragma unroll
+                        for (int x = 0; x < VEC; ++x) o[x] = acc[jt][g * VEC + x][r];
+                        stg<typename Gt::vec_u>(cgroup[g] + j * w, o);
                     } else {
-                        // TT with ncols % VEC != 0: the group may run past the end
+                        // the group runs past the slab row's (or C's) end
 #pragma unroll
                         for (int x = 0; x < VEC; ++x)
-                            if (ncol0 + g * VEC + x < ncols) stg<T>(cgroup[g] + j * w + x, acc[jt][g * VEC + x][r]);
+                            if (x < gnv[g]) stg<T>(cgroup[g] + j * w + x, acc[jt][g * VEC + x][r]);
                     }
                 }
             }
@@ -551,13 +609,13 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
             for (int r = 0; r < 4; ++r) {
                 uint64_t const jr = j0 + wm * 16 * JT + JT * Gt::acc_row(lr, r);
                 if constexpr (JT % VEC == 0) {
-                    if ((m % VEC) == 0 && jr + JT <= m) {
+                    if (c_vec && jr + JT <= m) {
 #pragma unroll
                         for (int jt = 0; jt < JT; jt += VEC) {
                             V o;
 #pragma unroll
                             for (int x = 0; x < VEC; ++x) o[x] = acc[jt + x][nt][r];
-                            if (stream) stg_nt<V>(crow + jr + jt, o);
+                            if (nt_c) stg_nt<V>(crow + jr + jt, o);
                             else stg<V>(crow + jr + jt, o);
                         }
                         continue;
@@ -571,20 +629,53 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     }
 }
 
-// One product: a grid of BM x BN blocks (row blocks fastest, XCD-grouped).
-template <class T, int JT, int NT, int WAVES_M, int BK, bool TN, bool BCOL>
+// One product: a grid of BM x BN blocks (row blocks fastest, XCD-grouped)
+// times gridDim.y contraction splits.  Split s covers [s*kc, s*kc + kc) of
+// the contraction and writes its partial product, in C's layout, at
+// c + s * c_split (the partials are summed by k_sum_splits); with one split
+// the block writes C itself.
+template <class T, int JT, int NT, int WAVES_M, int BK, bool TN, bool BCOL, bool PAD = false>
 __global__ __launch_bounds__(256, 2) void k_gemm(const T *__restrict__ a, const T *__restrict__ b,
                                               T *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w,
-                                              uint64_t m, unsigned nj, unsigned total)
+                                              uint64_t wp, uint64_t m, unsigned nj, unsigned total, uint64_t kc,
+                                              uint64_t c_split, unsigned flags)
 {
     using S = TileShape<T, JT, NT, WAVES_M, BK>;
     __shared__ __attribute__((aligned(16))) T sA[2][BK * S::LDA];
     __shared__ __attribute__((aligned(16))) T sB[2][BK * S::LDB];
     unsigned jb, nb;
     block_coords(nj, total, jb, nb);
-    gemm_tile<T, JT, NT, WAVES_M, BK, TN ? VIEW_TN : VIEW_NN, BCOL>(
-        a, b, c, ncols, nq, w, m, (uint64_t)jb * S::BM, (uint64_t)nb * S::BN,
-        TILE_A_VEC | TILE_C_VEC | (nj == 1 ? TILE_STREAM : 0u), sA, sB);
+    uint64_t const l0 = (uint64_t)blockIdx.y * kc;
+    uint64_t const len = nq - l0 < kc ? nq - l0 : kc;
+    gemm_tile<T, JT, NT, WAVES_M, BK, TN ? VIEW_TN : VIEW_NN, BCOL, PAD>(
+        a + l0 * (TN ? 1 : w), b + l0 * (BCOL ? m : 1), c + (uint64_t)blockIdx.y * c_split, ncols, len, nq, w, wp,
+        m, (uint64_t)jb * S::BM, (uint64_t)nb * S::BN, flags, sA, sB);
+}
+
+// C[i] = sum_{s < S} P[s * n + i] in the order s = 0, 1, ... (deterministic).
+// VEC-wide when n % VEC == 0 and C is 16-byte aligned (the partials are).
+template <class T>
+__global__ __launch_bounds__(256) void k_sum_splits(const T *__restrict__ p, uint64_t n, unsigned nsplit,
+                                                    T *__restrict__ c, int vec)
+{
+    constexpr int VEC = G<T>::VEC;
+    using V = typename G<T>::vec;
+    uint64_t const stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t const t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (vec) {
+        uint64_t const nv = n / VEC;
+        for (uint64_t v = t0; v < nv; v += stride) {
+            V acc = ldg<V>(p + v * VEC);
+            for (unsigned s = 1; s < nsplit; ++s) acc += ldg<V>(p + (uint64_t)s * n + v * VEC);
+            stg<V>(c + v * VEC, acc);
+        }
+    } else {
+        for (uint64_t i = t0; i < n; i += stride) {
+            T acc = p[i];
+            for (unsigned s = 1; s < nsplit; ++s) acc += p[(uint64_t)s * n + i];
+            c[i] = acc;
+        }
+    }
 }
 
 // Many products in one persistent launch: workgroup g processes tiles
@@ -620,7 +711,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_grouped(const GroupProblem *__r
         unsigned const jb = local % cur.nj, nb = local / cur.nj;
         gemm_tile<T, JT, NT, WAVES_M, BK, VIEW_TT, false>(
             static_cast<const T *>(cur.a), static_cast<const T *>(cur.b), static_cast<T *>(cur.c), cur.ncols,
-            cur.nq, cur.ncols, cur.m, (uint64_t)jb * S::BM, (uint64_t)nb * S::BN, cur.flags, sA, sB);
+            cur.nq, cur.nq, cur.ncols, cur.ncols, cur.m, (uint64_t)jb * S::BM, (uint64_t)nb * S::BN, cur.flags, sA,
+            sB);
         if (!has_next) break;
         t = tn;
         cur = nxt;
@@ -652,78 +744,166 @@ Config pick_config(const Fold &f, bool is_double)
     return {4, 4, 2, 16};
 }
 
-template <class T, int JT, int NT, int WM, int BK, bool TN, bool BCOL>
-void launch_cfg(const Fold &f, const T *a, const T *b, T *c, hipStream_t s)
+// Geometry of a product in the gemm family: tile shape, padded slab width
+// (NN view), column count in that padded space and the block grid.
+struct Geom
 {
-    constexpr int BM = WM * 16 * JT;
-    constexpr int BN = (4 / WM) * 16 * NT;
-    uint64_t const ncols = TN ? f.outer : f.outer * f.w;
-    unsigned const nj = (unsigned)((f.m + BM - 1) / BM);
-    unsigned const nn = (unsigned)((ncols + BN - 1) / BN);
-    unsigned const total = nj * nn;
-    hipLaunchKernelGGL((k_gemm<T, JT, NT, WM, BK, TN, BCOL>), dim3(total), dim3(256), 0, s, a, b, c, ncols, f.nq,
-                       f.w, f.m, nj, total);
+    Config cfg;
+    bool tn = false;
+    bool pad = false; // NN view that needs the PAD tile (set by the launcher from w and the pointers)
+    uint64_t wp = 1, ncols = 0;
+    unsigned nj = 0, nn = 0;
+    int bm() const { return cfg.waves_m * 16 * cfg.jt; }
+    int bn() const { return (4 / cfg.waves_m) * 16 * cfg.nt; }
+};
+
+Geom geom(const Fold &f, bool is_double)
+{
+    uint64_t const vec = is_double ? 2 : 4;
+    Geom g;
+    g.cfg = pick_config(f, is_double);
+    g.tn = f.w == 1;
+    g.wp = g.tn ? 1 : (f.w + vec - 1) / vec * vec;
+    g.ncols = g.tn ? f.outer : f.outer * g.wp;
+    g.nj = (unsigned)((f.m + g.bm() - 1) / g.bm());
+    g.nn = (unsigned)((g.ncols + g.bn() - 1) / g.bn());
+    return g;
+}
+
+// Contraction splits: when the blocks of a product cannot fill the chip
+// (fewer than two per compute unit, the occupancy of the tiles), the
+// contraction is cut into `splits` ranges of kc indices (a multiple of 64,
+// so that every range starts 16-byte aligned and on a whole 128-byte line of
+// a float32 row) whose partial products are summed by a second pass.  Ranges
+// keep at least 512 indices, and the partials stay within 256 MiB.
+constexpr uint64_t kSplitMinChunk = 512;
+constexpr uint64_t kSplitMaxBytes = 256ull << 20;
+
+Split split_for(const Fold &f, bool is_double, const Geom &g, unsigned want)
+{
+    Split sp;
+    sp.kc = f.nq;
+    uint64_t const tiles = (uint64_t)g.nj * g.nn;
+    uint64_t s = want;
+    if (s == 0) {
+        uint64_t const target = (uint64_t)device_cus() * 2;
+        s = tiles >= target ? 1 : (target + tiles - 1) / tiles;
+    }
+    uint64_t const by_len = f.nq / kSplitMinChunk;
+    uint64_t const by_bytes = kSplitMaxBytes / (f.numel_c() * (is_double ? 8 : 4) + 1);
+    s = std::min<uint64_t>({s, by_len, by_bytes, 65535});
+    if (s <= 1) return sp;
+    uint64_t kc = (f.nq + s - 1) / s;
+    kc = (kc + 63) / 64 * 64;
+    sp.kc = kc;
+    sp.splits = (unsigned)((f.nq + kc - 1) / kc);
+    if (sp.splits <= 1) sp = Split{1, f.nq};
+    return sp;
+}
+
+bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+template <class T, int JT, int NT, int WM, int BK, bool TN, bool BCOL>
+void launch_cfg(const Fold &f, const Geom &g, const Split &sp, unsigned flags, const T *a, const T *b, T *c,
+                hipStream_t s)
+{
+    unsigned const total = g.nj * g.nn;
+    uint64_t const c_split = sp.splits > 1 ? f.numel_c() : 0;
+    if (!TN && g.pad)
+        hipLaunchKernelGGL((k_gemm<T, JT, NT, WM, BK, TN, BCOL, true>), dim3(total, sp.splits), dim3(256), 0, s, a,
+                           b, c, g.ncols, f.nq, f.w, g.wp, f.m, g.nj, total, sp.kc, c_split, flags);
+    else
+        hipLaunchKernelGGL((k_gemm<T, JT, NT, WM, BK, TN, BCOL>), dim3(total, sp.splits), dim3(256), 0, s, a, b, c,
+                           g.ncols, f.nq, f.w, g.wp, f.m, g.nj, total, sp.kc, c_split, flags);
 }
 
 template <class T, bool TN, bool BCOL>
-void launch_layout(const Fold &f, const T *a, const T *b, T *c, hipStream_t s, Config cfg)
+void launch_layout(const Fold &f, const Geom &g, const Split &sp, unsigned flags, const T *a, const T *b, T *c,
+                   hipStream_t s)
 {
+    Config const cfg = g.cfg;
     if constexpr (sizeof(T) == 4 && TN) {
         if (cfg.jt == 1 && cfg.bk == 32) {
-            launch_cfg<T, 1, 4, 1, 32, TN, BCOL>(f, a, b, c, s);
+            launch_cfg<T, 1, 4, 1, 32, TN, BCOL>(f, g, sp, flags, a, b, c, s);
             return;
         }
     }
     if constexpr (sizeof(T) == 8 && TN) {
         if (cfg.nt == 2) {
-            launch_cfg<T, 1, 2, 1, 16, TN, BCOL>(f, a, b, c, s);
+            launch_cfg<T, 1, 2, 1, 16, TN, BCOL>(f, g, sp, flags, a, b, c, s);
             return;
         }
     }
-    if (cfg.jt == 1) launch_cfg<T, 1, 4, 1, 16, TN, BCOL>(f, a, b, c, s);
-    else if (cfg.jt == 2) launch_cfg<T, 2, 4, 1, 16, TN, BCOL>(f, a, b, c, s);
-    else if (cfg.waves_m == 1) launch_cfg<T, 4, 4, 1, 16, TN, BCOL>(f, a, b, c, s);
-    else launch_cfg<T, 4, 4, 2, 16, TN, BCOL>(f, a, b, c, s);
+    if (cfg.jt == 1) launch_cfg<T, 1, 4, 1, 16, TN, BCOL>(f, g, sp, flags, a, b, c, s);
+    else if (cfg.jt == 2) launch_cfg<T, 2, 4, 1, 16, TN, BCOL>(f, g, sp, flags, a, b, c, s);
+    else if (cfg.waves_m == 1) launch_cfg<T, 4, 4, 1, 16, TN, BCOL>(f, g, sp, flags, a, b, c, s);
+    else launch_cfg<T, 4, 4, 2, 16, TN, BCOL>(f, g, sp, flags, a, b, c, s);
 }
 
 } // namespace
 
-std::string gemm_symbol(const Fold &f, bool is_double)
+std::string gemm_symbol(const Fold &f, bool is_double, bool aligned)
 {
     Config const cfg = pick_config(f, is_double);
+    uint64_t const vec = is_double ? 2 : 4;
+    bool const pad = f.w != 1 && (!aligned || f.w % vec != 0);
     char buf[128];
-    std::snprintf(buf, sizeof buf, "k_gemm<%s, %d, %d, %d, %d, %s, %s>", is_double ? "double" : "float", cfg.jt,
-                  cfg.nt, cfg.waves_m, cfg.bk, f.w == 1 ? "true" : "false", f.b_colmajor ? "true" : "false");
+    std::snprintf(buf, sizeof buf, "k_gemm<%s, %d, %d, %d, %d, %s, %s, %s>", is_double ? "double" : "float", cfg.jt,
+                  cfg.nt, cfg.waves_m, cfg.bk, f.w == 1 ? "true" : "false", f.b_colmajor ? "true" : "false",
+                  pad ? "true" : "false");
     return buf;
 }
 
-bool gemm_supports(const Fold &f, bool is_double, bool aligned)
+bool gemm_supports(const Fold &f, bool is_double)
 {
-    uint64_t const vec = is_double ? 2 : 4;
-    if (!aligned) return false;
-    bool const tn = f.w == 1;
-    if (tn) {
-        if (f.nq % vec != 0) return false;
-    } else if (f.w % vec != 0) {
-        return false;
-    }
-    uint64_t const ncols = tn ? f.outer : f.outer * f.w;
-    // grid size limits (the grid is one-dimensional)
-    uint64_t const blocks = ((f.m + 15) / 16) * ((ncols + 127) / 128);
-    return blocks < (1ull << 31);
+    Geom const g = geom(f, is_double);
+    // one-dimensional grid of blocks (the splits go along y)
+    return (uint64_t)g.nj * g.nn < (1ull << 31) && g.nj > 0 && g.nn > 0;
+}
+
+Split gemm_split(const Fold &f, bool is_double, unsigned want)
+{
+    return split_for(f, is_double, geom(f, is_double), want);
+}
+
+bool gemm_underfilled(const Fold &f, bool is_double)
+{
+    Geom const g = geom(f, is_double);
+    return (uint64_t)g.nj * g.nn < (uint64_t)device_cus() * 2;
 }
 
 template <class T>
-ttm_status_t launch_gemm(const Fold &f, const T *a, const T *b, T *c, hipStream_t s)
+ttm_status_t launch_gemm(const Fold &f, const T *a, const T *b, T *c, hipStream_t s, unsigned want_splits)
 {
-    Config const cfg = pick_config(f, sizeof(T) == 8);
-    bool const tn = f.w == 1;
-    if (tn) {
-        if (f.b_colmajor) launch_layout<T, true, true>(f, a, b, c, s, cfg);
-        else launch_layout<T, true, false>(f, a, b, c, s, cfg);
+    constexpr uint64_t VEC = 16 / sizeof(T);
+    Geom g = geom(f, sizeof(T) == 8);
+    Split const sp = split_for(f, sizeof(T) == 8, g, want_splits);
+    T *out = c;
+    void *ws = nullptr;
+    if (sp.splits > 1) {
+        size_t const bytes = (size_t)sp.splits * f.numel_c() * sizeof(T);
+        if (ttm_status_t const st = ws_acquire(bytes, s, &ws, "workspace for the split-contraction partials");
+            st != TTM_SUCCESS)
+            return st;
+        out = static_cast<T *>(ws);
+    }
+    bool const c_al = sp.splits > 1 ? f.numel_c() % VEC == 0 : aligned16(c);
+    unsigned flags = 0;
+    if (aligned16(a) && (g.tn ? f.nq % VEC == 0 : f.w % VEC == 0)) flags |= TILE_A_VEC;
+    if (c_al && (g.tn ? f.m % VEC == 0 : f.w % VEC == 0)) flags |= TILE_C_VEC;
+    if (aligned16(b)) flags |= TILE_B_VEC;
+    if (g.nj == 1) flags |= TILE_STREAM | (sp.splits > 1 ? TILE_KEEP_C : 0u);
+    g.pad = !g.tn && !((flags & TILE_A_VEC) && (flags & TILE_C_VEC));
+    if (g.tn) {
+        if (f.b_colmajor) launch_layout<T, true, true>(f, g, sp, flags, a, b, out, s);
+        else launch_layout<T, true, false>(f, g, sp, flags, a, b, out, s);
     } else {
-        if (f.b_colmajor) launch_layout<T, false, true>(f, a, b, c, s, cfg);
-        else launch_layout<T, false, false>(f, a, b, c, s, cfg);
+        if (f.b_colmajor) launch_layout<T, false, true>(f, g, sp, flags, a, b, out, s);
+        else launch_layout<T, false, false>(f, g, sp, flags, a, b, out, s);
+    }
+    if (sp.splits > 1) {
+        launch_sum_splits<T>(out, f.numel_c(), sp.splits, c, s);
+        ws_release(ws, s);
     }
     hipError_t const e = hipGetLastError();
     if (e != hipSuccess) {
@@ -733,8 +913,25 @@ ttm_status_t launch_gemm(const Fold &f, const T *a, const T *b, T *c, hipStream_
     return TTM_SUCCESS;
 }
 
-template ttm_status_t launch_gemm<float>(const Fold &, const float *, const float *, float *, hipStream_t);
-template ttm_status_t launch_gemm<double>(const Fold &, const double *, const double *, double *, hipStream_t);
+template <class T>
+void launch_sum_splits(const T *partials, uint64_t n, unsigned splits, T *c, hipStream_t s)
+{
+    constexpr uint64_t VEC = 16 / sizeof(T);
+    int const vec = (n % VEC == 0 && aligned16(c) && aligned16(partials)) ? 1 : 0;
+    uint64_t const work = vec ? n / VEC : n;
+    uint64_t blocks = (work + 255) / 256;
+    uint64_t const cap = (uint64_t)device_cus() * 8;
+    if (blocks > cap) blocks = cap;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_sum_splits<T>, dim3((unsigned)blocks), dim3(256), 0, s, partials, n, splits, c, vec);
+}
+
+template void launch_sum_splits<float>(const float *, uint64_t, unsigned, float *, hipStream_t);
+template void launch_sum_splits<double>(const double *, uint64_t, unsigned, double *, hipStream_t);
+
+template ttm_status_t launch_gemm<float>(const Fold &, const float *, const float *, float *, hipStream_t, unsigned);
+template ttm_status_t launch_gemm<double>(const Fold &, const double *, const double *, double *, hipStream_t,
+                                          unsigned);
 
 // ---- grouped launches -------------------------------------------------------
 namespace

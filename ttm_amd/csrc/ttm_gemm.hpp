@@ -12,23 +12,49 @@
 namespace ttm_amd
 {
 
-// True when the gemm family can run the fold: 16-byte aligned buffers and
-// either w == 1 with nq % vec == 0 (TN view) or w % vec == 0 (NN view).
-bool gemm_supports(const Fold &f, bool is_double, bool aligned);
+// True when the gemm family can run the fold (any extents and element-aligned
+// buffers; false only beyond its grid limits).
+bool gemm_supports(const Fold &f, bool is_double);
 
-// Template instance the gemm launcher runs for a fold (as profilers print it).
-std::string gemm_symbol(const Fold &f, bool is_double);
+// Template instance the gemm launcher runs for a fold (as profilers print it);
+// `aligned`: A and C 16-byte aligned.
+std::string gemm_symbol(const Fold &f, bool is_double, bool aligned = true);
 
+// Contraction split of a product: `splits` ranges of kc indices.
+struct Split
+{
+    unsigned splits = 1;
+    uint64_t kc = 0;
+};
+// The split the launcher uses: `want` splits (0: the planner's choice, which
+// is 1 unless the blocks cannot fill the chip), clamped to ranges of at least
+// 512 indices and 256 MiB of partials.
+Split gemm_split(const Fold &f, bool is_double, unsigned want = 0);
+// Fewer blocks than two per compute unit.
+bool gemm_underfilled(const Fold &f, bool is_double);
+
+// C[i] = sum over the splits of partials[s * n + i], in split order
+// (deterministic), for i < n; partials is 16-byte aligned.
 template <class T>
-ttm_status_t launch_gemm(const Fold &f, const T *a, const T *b, T *c, hipStream_t s);
+void launch_sum_splits(const T *partials, uint64_t n, unsigned splits, T *c, hipStream_t s);
+
+// want_splits as for gemm_split.  Partials come from the call's workspace.
+template <class T>
+ttm_status_t launch_gemm(const Fold &f, const T *a, const T *b, T *c, hipStream_t s, unsigned want_splits = 0);
 
 // ---- grouped (persistent, many products per launch) ------------------------
 
 enum TileFlags : unsigned
 {
-    TILE_A_VEC = 1u, // rows of A may be read with 16-byte loads (nq % vec == 0, aligned)
-    TILE_C_VEC = 2u, // rows of C may be written with 16-byte stores (ncols % vec == 0, aligned)
-    TILE_STREAM = 4u // A is read exactly once (one row block): non-temporal loads of A and stores of C
+    // A may be read with aligned 16-byte loads: TN/TT views, rows of A start
+    // 16-byte aligned (row extent % vec == 0); NN view, w % vec == 0 and A aligned
+    TILE_A_VEC = 1u,
+    // C may be written with aligned 16-byte stores (NN/TT: w % vec == 0, TN:
+    // m % vec == 0, and C aligned)
+    TILE_C_VEC = 2u,
+    TILE_STREAM = 4u, // A is read exactly once (one row block): non-temporal loads of A and stores of C
+    TILE_B_VEC = 8u,  // B is 16-byte aligned (its row or column stride is checked in the tile)
+    TILE_KEEP_C = 16u // C stores stay temporal under TILE_STREAM (partials that are read back)
 };
 
 // One product of a grouped launch, in the TT view:

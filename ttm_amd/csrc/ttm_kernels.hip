@@ -17,6 +17,8 @@
 #include "ttm_kernels.hpp"
 #include "ttm_gemm.hpp"
 #include "ttm_lowp.hpp"
+#include "ttm_tune.hpp"
+#include "ttm_workspace.hpp"
 
 #include <atomic>
 #include <cstring>
@@ -588,27 +590,6 @@ int tn_lds_spb_max(const Fold &f, bool is_double)
 constexpr int tn_lds_waves(int spb_max) { return spb_max <= 257 ? 8 : 3; }
 bool tn_lds_ok(const Fold &f, bool is_double) { return f.m <= 16 && tn_lds_spb_max(f, is_double) > 0; }
 
-// Compute units of the current device (queried once per device; concurrent
-// first calls store the same value).
-int device_cus()
-{
-    static std::atomic<int> cus[64] = {};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
-        (void)hipGetLastError();
-        return 256;
-    }
-    int n = cus[dev].load(std::memory_order_relaxed);
-    if (n == 0) {
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) {
-            (void)hipGetLastError();
-            n = 256;
-        }
-        cus[dev].store(n, std::memory_order_relaxed);
-    }
-    return n;
-}
-
 template <class T, int SPB_MAX>
 void launch_tn_lds(const Fold &f, const T *a, const T *b, T *c, hipStream_t s)
 {
@@ -658,7 +639,7 @@ void launch_stream(const Fold &f, const T *a, const T *b, T *c, hipStream_t s, b
 }
 
 template <class T>
-ttm_status_t launch_plain(const Fold &f, const T *a, const T *b, T *c, hipStream_t s, Kernel kern)
+ttm_status_t launch_plain(const Fold &f, const T *a, const T *b, T *c, hipStream_t s, Kernel kern, unsigned splits)
 {
     uint64_t const bsj = f.b_stride_j(), bsl = f.b_stride_l();
     constexpr int VEC = Traits<T>::VEC;
@@ -682,7 +663,8 @@ ttm_status_t launch_plain(const Fold &f, const T *a, const T *b, T *c, hipStream
         else bv ? launch_stream<T, 4, true>(f, a, b, c, s, tn) : launch_stream<T, 4, false>(f, a, b, c, s, tn);
         break;
     }
-    case Kernel::gemm: return launch_gemm<T>(f, a, b, c, s);
+    case Kernel::gemm: return launch_gemm<T>(f, a, b, c, s, splits);
+    case Kernel::skinny: return launch_skinny<T>(f, a, b, c, s, splits);
     default: set_error(TTM_ERR_ARGUMENT, "no kernel for this problem"); return TTM_ERR_ARGUMENT;
     }
     hipError_t const e = hipGetLastError();
@@ -693,7 +675,9 @@ ttm_status_t launch_plain(const Fold &f, const T *a, const T *b, T *c, hipStream
     return TTM_SUCCESS;
 }
 
-const char *const k_kernel_names[] = {"none", "generic", "nn_stream", "tn_stream", "gemm", "lowp", "lowp_generic"};
+const char *const k_kernel_names[] = {"none",  "generic", "nn_stream",    "tn_stream",
+                                      "gemm",  "lowp",    "lowp_generic", "skinny"};
+static_assert(sizeof(k_kernel_names) / sizeof(k_kernel_names[0]) == static_cast<int>(Kernel::count), "names");
 
 } // namespace
 
@@ -703,7 +687,7 @@ const char *kernel_name(Kernel k)
     return (i >= 0 && i < static_cast<int>(Kernel::count)) ? k_kernel_names[i] : "none";
 }
 
-std::string kernel_symbol(Kernel k, const Fold &f, bool is_double, ttm_precision_t prec)
+std::string kernel_symbol(Kernel k, const Fold &f, bool is_double, ttm_precision_t prec, bool aligned)
 {
     std::string const t = is_double ? "double" : "float";
     std::string const jt = std::to_string(pick_jt(f.m));
@@ -722,7 +706,8 @@ std::string kernel_symbol(Kernel k, const Fold &f, bool is_double, ttm_precision
                std::to_string(stream_depth(j, tn)) + ", " + (stream_bvec(f, is_double ? 2 : 4) ? "true" : "false") +
                ">";
     }
-    case Kernel::gemm: return gemm_symbol(f, is_double);
+    case Kernel::gemm: return gemm_symbol(f, is_double, aligned);
+    case Kernel::skinny: return skinny_symbol(f, is_double);
     case Kernel::lowp: return lowp_symbol(f, prec, false);
     case Kernel::lowp_generic: return lowp_symbol(f, prec, true);
     default: return "";
@@ -750,48 +735,77 @@ Kernel choose_kernel(const Fold &f, bool is_double, ttm_precision_t prec, bool a
     bool const small_m = f.m <= 64;
     bool const nn_ok = aligned && small_m && f.w % vec == 0;
     bool const tn_ok = aligned && small_m && f.w == 1 && f.nq % vec == 0;
-    bool const gemm_ok = gemm_supports(f, is_double, aligned);
+    bool const skinny_ok = skinny_supports(f);
+    bool const gemm_ok = gemm_supports(f, is_double);
     if (force == Kernel::generic) return Kernel::generic;
     if (force == Kernel::nn_stream && nn_ok) return Kernel::nn_stream;
     if (force == Kernel::tn_stream && tn_ok) return Kernel::tn_stream;
     if (force == Kernel::gemm && gemm_ok) return Kernel::gemm;
+    if (force == Kernel::skinny && skinny_ok) return Kernel::skinny;
+    // q fastest with at most eight slabs (the GEMV-like cases, p = 1
+    // included): B is the large operand, streamed on the vector ALUs with
+    // the contraction split over the chip
+    if (skinny_ok) return Kernel::skinny;
+    // The streams give a wave (nn) or a 16-row tile (tn) a whole strip of
+    // the product: with fewer workgroups than compute units the blocks of
+    // the gemm family, split along the contraction, fill the chip instead.
+    int const cus = device_cus();
+    bool const nn_full = (f.outer * f.w + 64 * vec - 1) / (64 * vec) >= (uint64_t)cus;
+    bool const tn_full = (f.outer + 16 * 8 - 1) / (16 * 8) >= (uint64_t)cus;
     // m <= 16 in the NN view: the register-streaming kernel reads A at
     // 6.4-6.8 TB/s on the bench shapes, the LDS-staged tile at 5.3-5.8
-    if (nn_ok && f.m <= 16) return Kernel::nn_stream;
+    if (nn_ok && f.m <= 16 && nn_full) return Kernel::nn_stream;
     // m <= 16 in the TN view with all of B in LDS: tn_lds reads A at
     // 5.7-6.1 TB/s (float32) and 5.7-6.2 TB/s (float64) on the bench shapes,
     // the LDS-staged tile at 5.3-5.7 and 5.4-5.6
-    if (tn_ok && tn_lds_ok(f, is_double)) return Kernel::tn_stream;
+    if (tn_ok && tn_lds_ok(f, is_double) && tn_full) return Kernel::tn_stream;
     if (gemm_ok) return Kernel::gemm;
     if (tn_ok) return Kernel::tn_stream;
     if (nn_ok) return Kernel::nn_stream;
     return Kernel::generic;
 }
 
-ttm_status_t launch_f32(const Fold &f, const float *a, const float *b, float *c, ttm_precision_t prec,
-                        hipStream_t stream, Kernel force, Kernel *used)
+unsigned launch_splits(Kernel k, const Fold &f, bool is_double, unsigned want)
 {
-    bool const aligned = ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b) |
-                           reinterpret_cast<uintptr_t>(c)) & 15u) == 0;
-    Kernel const k = choose_kernel(f, false, prec, aligned, force);
-    if (used) *used = k;
+    if (k != Kernel::gemm && k != Kernel::skinny) return 1;
+    if (want == 0) want = tuned_splits(k, f, is_double);
+    return k == Kernel::gemm ? gemm_split(f, is_double, want).splits : skinny_splits(f, is_double, want);
+}
+
+ttm_status_t launch_f32(const Fold &f, const float *a, const float *b, float *c, ttm_precision_t prec,
+                        hipStream_t stream, Kernel force, LaunchInfo *info, unsigned want_splits)
+{
+    uintptr_t const bits = reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b) |
+                           reinterpret_cast<uintptr_t>(c);
+    bool const aligned = (bits & 15u) == 0;
+    // pointers that are not even element-aligned only run element by element
+    Kernel const k = (bits & 3u) ? (prec == TTM_PREC_DEFAULT ? Kernel::generic : Kernel::lowp_generic)
+                                 : choose_kernel(f, false, prec, aligned, force);
     if (k == Kernel::none) {
         set_error(TTM_ERR_PRECISION);
         return TTM_ERR_PRECISION;
     }
-    if (k == Kernel::lowp || k == Kernel::lowp_generic)
+    bool const ac_aligned = ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(c)) & 15u) == 0;
+    if (k == Kernel::lowp || k == Kernel::lowp_generic) {
+        if (info) *info = LaunchInfo{k, 1, ac_aligned};
         return launch_lowp(f, a, b, c, prec, stream, k == Kernel::lowp_generic);
-    return launch_plain<float>(f, a, b, c, stream, k);
+    }
+    if (want_splits == 0) want_splits = tuned_splits(k, f, false);
+    if (info) *info = LaunchInfo{k, launch_splits(k, f, false, want_splits), ac_aligned};
+    return launch_plain<float>(f, a, b, c, stream, k, want_splits);
 }
 
 ttm_status_t launch_f64(const Fold &f, const double *a, const double *b, double *c, hipStream_t stream,
-                        Kernel force, Kernel *used)
+                        Kernel force, LaunchInfo *info, unsigned want_splits)
 {
-    bool const aligned = ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b) |
-                           reinterpret_cast<uintptr_t>(c)) & 15u) == 0;
-    Kernel const k = choose_kernel(f, true, TTM_PREC_DEFAULT, aligned, force);
-    if (used) *used = k;
-    return launch_plain<double>(f, a, b, c, stream, k);
+    uintptr_t const bits = reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b) |
+                           reinterpret_cast<uintptr_t>(c);
+    bool const aligned = (bits & 15u) == 0;
+    Kernel const k = (bits & 7u) ? Kernel::generic : choose_kernel(f, true, TTM_PREC_DEFAULT, aligned, force);
+    if (want_splits == 0) want_splits = tuned_splits(k, f, true);
+    bool const ac_aligned = ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(c)) & 15u) == 0;
+    if (info) *info = LaunchInfo{k, launch_splits(k, f, true, want_splits), ac_aligned};
+    return launch_plain<double>(f, a, b, c, stream, k, want_splits);
 }
 
 } // namespace ttm_amd

@@ -27,6 +27,7 @@
 //   accumulator element r holds D[4*lg + r][lc].
 // LDS tiles are therefore stored K-contiguous: s[row][k].
 #include "ttm_lowp.hpp"
+#include "ttm_workspace.hpp"
 
 #include <cstdint>
 #include <cstdio>
@@ -814,25 +815,39 @@ __global__ __launch_bounds__(256, lp_group_blocks_per_cu(BM)) void k_lowp_groupe
 // ---------------------------------------------------------------------------
 constexpr float FP8_MAX = 448.f;
 
-// Power-of-two exponent e with amax * 2^-e <= 448 (0 for an all-zero slice).
+// Non-finite values take no part in a slice's scale: |x| for finite x, 0 for
+// infinities and NaNs (which the comparison sends to the 0 branch).
+__device__ __forceinline__ float finite_abs(float x)
+{
+    float const a = fabsf(x);
+    return a < __builtin_inff() ? a : 0.f;
+}
+
+// Power-of-two exponent e with amax * 2^-e <= 448 (0 for an all-zero slice;
+// amax is a maximum of finite_abs values, so it is finite).
 __device__ __forceinline__ int slice_exponent(float amax)
 {
-    if (!(amax > 0.f)) return 0;
+    if (!(amax > 0.f) || !(amax < __builtin_inff())) return 0;
     int e = (int)ceilf(log2f(amax / FP8_MAX));
     if (ldexpf(amax, -e) > FP8_MAX) ++e;
     return e < -126 ? -126 : (e > 127 ? 127 : e);
 }
 
-// 4 floats -> 4 e4m3fn bytes scaled by 2^-e (clamped: the conversion
-// instruction returns NaN beyond +-448 instead of saturating).
+// x * 2^-e ready for the conversion: finite values clamped to +-448 (the
+// conversion instruction returns NaN beyond +-448 instead of saturating);
+// infinities and NaNs become NaN (e4m3fn has no infinity), so they reach
+// every entry of C they take part in.
+__device__ __forceinline__ float fp8_scaled(float x, int e)
+{
+    float const v = fminf(fmaxf(ldexpf(x, -e), -FP8_MAX), FP8_MAX);
+    return fabsf(x) < __builtin_inff() ? v : __builtin_nanf("");
+}
+
+// 4 floats -> 4 e4m3fn bytes scaled by 2^-e.
 __device__ __forceinline__ int quantize4(float x0, float x1, float x2, float x3, int e)
 {
-    float const v0 = fminf(fmaxf(ldexpf(x0, -e), -FP8_MAX), FP8_MAX);
-    float const v1 = fminf(fmaxf(ldexpf(x1, -e), -FP8_MAX), FP8_MAX);
-    float const v2 = fminf(fmaxf(ldexpf(x2, -e), -FP8_MAX), FP8_MAX);
-    float const v3 = fminf(fmaxf(ldexpf(x3, -e), -FP8_MAX), FP8_MAX);
-    int w = __builtin_amdgcn_cvt_pk_fp8_f32(v0, v1, 0, false);
-    return __builtin_amdgcn_cvt_pk_fp8_f32(v2, v3, w, true);
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(fp8_scaled(x0, e), fp8_scaled(x1, e), 0, false);
+    return __builtin_amdgcn_cvt_pk_fp8_f32(fp8_scaled(x2, e), fp8_scaled(x3, e), w, true);
 }
 
 // 32 floats -> 32 e4m3fn bytes scaled by 2^-e.
@@ -870,7 +885,8 @@ __device__ __forceinline__ void fp8_rows_load(const float *__restrict__ x, uint6
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        float amax = fmaxf(fmaxf(fabsf(v[i][0]), fabsf(v[i][1])), fmaxf(fabsf(v[i][2]), fabsf(v[i][3])));
+        float amax = fmaxf(fmaxf(finite_abs(v[i][0]), finite_abs(v[i][1])),
+                           fmaxf(finite_abs(v[i][2]), finite_abs(v[i][3])));
         amax = fmaxf(amax, __shfl_xor(amax, 1));
         amax = fmaxf(amax, __shfl_xor(amax, 2));
         amax = fmaxf(amax, __shfl_xor(amax, 4));
@@ -899,7 +915,7 @@ __global__ __launch_bounds__(256) void k_fp8_split_b(const float *__restrict__ b
     }
     float amax = 0.f;
 #pragma unroll
-    for (int t = 0; t < 32; ++t) amax = fmaxf(amax, fabsf(x[t]));
+    for (int t = 0; t < 32; ++t) amax = fmaxf(amax, finite_abs(x[t]));
     int const e = slice_exponent(amax);
     i32x8 q;
     quantize32(x, e, q);
@@ -970,7 +986,7 @@ __global__ __launch_bounds__(256) void k_fp8_split_a_cols(const float *__restric
         int const c = (wave * 8 + i) * 8 + (lane >> 3);
         int const col = c >> 2, kb = c & 3;
         f32x4 const x = *reinterpret_cast<const f32x4 *>(&sT[col * LDT + 32 * kb + 4 * (lane & 7)]);
-        float amax = fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fmaxf(fabsf(x[2]), fabsf(x[3])));
+        float amax = fmaxf(fmaxf(finite_abs(x[0]), finite_abs(x[1])), fmaxf(finite_abs(x[2]), finite_abs(x[3])));
         amax = fmaxf(amax, __shfl_xor(amax, 1));
         amax = fmaxf(amax, __shfl_xor(amax, 2));
         amax = fmaxf(amax, __shfl_xor(amax, 4));
@@ -1119,8 +1135,7 @@ __global__ __launch_bounds__(256) void k_fp8(const unsigned char *__restrict__ q
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float fp8_value(float x, int e)
 {
-    float v = fminf(fmaxf(ldexpf(x, -e), -FP8_MAX), FP8_MAX);
-    int const wq = __builtin_amdgcn_cvt_pk_fp8_f32(v, 0.f, 0, false);
+    int const wq = __builtin_amdgcn_cvt_pk_fp8_f32(fp8_scaled(x, e), 0.f, 0, false);
     return ldexpf(__builtin_amdgcn_cvt_f32_fp8(wq, 0), e);
 }
 
@@ -1145,8 +1160,8 @@ __global__ __launch_bounds__(256) void k_lowp_generic(const float *__restrict__ 
             if (prec == TTM_PREC_FP8_E4M3) {
                 float amax = 0.f, bmax = 0.f;
                 for (uint64_t l = l0; l < l1; ++l) {
-                    amax = fmaxf(amax, fabsf(ap[l * w]));
-                    bmax = fmaxf(bmax, fabsf(bp[l * bsl]));
+                    amax = fmaxf(amax, finite_abs(ap[l * w]));
+                    bmax = fmaxf(bmax, finite_abs(bp[l * bsl]));
                 }
                 ea = slice_exponent(amax);
                 eb = slice_exponent(bmax);
@@ -1178,35 +1193,6 @@ __global__ __launch_bounds__(256) void k_lowp_generic(const float *__restrict__ 
         }
         c[idx] = prec == TTM_PREC_FP16X2 ? ldexpf(s, -(exps[0] + exps[1])) : s;
     }
-}
-
-// Stream-ordered device workspace (the pre-converted B operand): one pool per
-// device that keeps its memory between calls, so a steady stream of products
-// allocates no new device memory after the first.
-hipMemPool_t workspace_pool()
-{
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-    static std::mutex mu;
-    static std::vector<hipMemPool_t> pools;
-    std::lock_guard<std::mutex> lock(mu);
-    if ((int)pools.size() <= dev) pools.resize(dev + 1, nullptr);
-    if (!pools[dev]) {
-        hipMemPoolProps props = {};
-        props.allocType = hipMemAllocationTypePinned;
-        props.handleTypes = hipMemHandleTypeNone;
-        props.location.type = hipMemLocationTypeDevice;
-        props.location.id = dev;
-        hipMemPool_t pool = nullptr;
-        if (hipMemPoolCreate(&pool, &props) != hipSuccess) return nullptr;
-        uint64_t keep = UINT64_MAX;
-        if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) {
-            (void)hipMemPoolDestroy(pool);
-            return nullptr;
-        }
-        pools[dev] = pool;
-    }
-    return pools[dev];
 }
 
 // Block height of the single-product tiles.
@@ -1270,13 +1256,11 @@ ttm_status_t launch_lp(const Fold &f, const float *a, const float *b, float *c, 
     size_t const planes_bytes = (P * plane * sizeof(op) + 255) / 256 * 256;
     // 64-row blocks scale A per block and slice themselves: B's exponent only
     ScalePass const sp = SCALED ? scale_pass(f, bm != 64) : ScalePass();
-    hipMemPool_t const pool = workspace_pool();
     void *ws = nullptr;
-    if (!pool || hipMallocFromPoolAsync(&ws, planes_bytes + (SCALED ? sp.bytes() : 0), pool, s) != hipSuccess) {
-        (void)hipGetLastError();
-        set_error(TTM_ERR_ALLOC, "workspace for the converted B operand");
-        return TTM_ERR_ALLOC;
-    }
+    if (ttm_status_t const st = ws_acquire(planes_bytes + (SCALED ? sp.bytes() : 0), s, &ws,
+                                           "workspace for the converted B operand");
+        st != TTM_SUCCESS)
+        return st;
     op *const bs = static_cast<op *>(ws);
     int const *exps = nullptr;
     if constexpr (SCALED) exps = launch_scale_pass(f, a, b, sp, static_cast<char *>(ws) + planes_bytes, s);
@@ -1289,7 +1273,7 @@ ttm_status_t launch_lp(const Fold &f, const float *a, const float *b, float *c, 
         hipLaunchKernelGGL((k_lowp<MODE, TN, 128>), dim3(total), dim3(256), 0, s, a, bs, ldb, plane, c, ncols, f.nq,
                            f.w, f.m, nj, total, exps);
     hipError_t const e = hipGetLastError();
-    (void)hipFreeAsync(ws, s);
+    ws_release(ws, s);
     if (e != hipSuccess) {
         set_error(TTM_ERR_HIP, hipGetErrorString(e));
         return TTM_ERR_HIP;
@@ -1323,13 +1307,11 @@ ttm_status_t launch_fp8(const Fold &f, const float *a, const float *b, float *c,
     auto round256 = [](uint64_t x) { return (x + 255) / 256 * 256; };
     size_t const bytes_qb = round256(rows * ldq), bytes_sb = round256(rows * (ldq / 32));
     size_t const bytes_qa = round256(rows_a * ldq), bytes_sa = round256(rows_a * (ldq / 32));
-    hipMemPool_t const pool = workspace_pool();
     void *ws = nullptr;
-    if (!pool || hipMallocFromPoolAsync(&ws, bytes_qb + bytes_sb + bytes_qa + bytes_sa, pool, s) != hipSuccess) {
-        (void)hipGetLastError();
-        set_error(TTM_ERR_ALLOC, "workspace for the quantized operands");
-        return TTM_ERR_ALLOC;
-    }
+    if (ttm_status_t const st = ws_acquire(bytes_qb + bytes_sb + bytes_qa + bytes_sa, s, &ws,
+                                           "workspace for the quantized operands");
+        st != TTM_SUCCESS)
+        return st;
     unsigned char *const qb = static_cast<unsigned char *>(ws);
     unsigned char *const sb = qb + bytes_qb;
     unsigned char *const qa = sb + bytes_sb;
@@ -1349,7 +1331,7 @@ ttm_status_t launch_fp8(const Fold &f, const float *a, const float *b, float *c,
         hipLaunchKernelGGL((k_fp8<TN, 64>), dim3(total), dim3(256), 0, s, qa, sa, qb, sb, ldq, c, ncols, f.nq, f.w,
                            f.m, nj, total);
     hipError_t const e = hipGetLastError();
-    (void)hipFreeAsync(ws, s);
+    ws_release(ws, s);
     if (e != hipSuccess) {
         set_error(TTM_ERR_HIP, hipGetErrorString(e));
         return TTM_ERR_HIP;
@@ -1403,17 +1385,14 @@ ttm_status_t launch_lowp(const Fold &f, const float *a, const float *b, float *c
         int const *exps = nullptr;
         if (prec == TTM_PREC_FP16X2) {
             ScalePass const sp = scale_pass(f, true);
-            hipMemPool_t const pool = workspace_pool();
-            if (!pool || hipMallocFromPoolAsync(&ws, sp.bytes(), pool, s) != hipSuccess) {
-                (void)hipGetLastError();
-                set_error(TTM_ERR_ALLOC, "workspace for the FP16X2 scales");
-                return TTM_ERR_ALLOC;
-            }
+            if (ttm_status_t const st = ws_acquire(sp.bytes(), s, &ws, "workspace for the FP16X2 scales");
+                st != TTM_SUCCESS)
+                return st;
             exps = launch_scale_pass(f, a, b, sp, ws, s);
         }
         hipLaunchKernelGGL(k_lowp_generic, dim3((unsigned)blocks), dim3(256), 0, s, a, b, c, f.outer, f.nq, f.w,
                            f.m, f.b_stride_j(), f.b_stride_l(), (int)prec, exps);
-        if (ws) (void)hipFreeAsync(ws, s);
+        ws_release(ws, s);
     } else if (prec == TTM_PREC_FP8_E4M3) {
         return f.w == 1 ? launch_fp8<true>(f, a, b, c, s) : launch_fp8<false>(f, a, b, c, s);
     } else if (prec == TTM_PREC_BF16) {
