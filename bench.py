@@ -75,6 +75,8 @@ def parse():
                     help="skip the host-pointer (PCIe-inclusive) drop-in measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--no-extras", dest="extras", action="store_false",
+                    help="skip the configuration-1 latency, order-5, column-major-B, split and odd-extent sections")
     return ap.parse_args()
 
 
@@ -92,6 +94,48 @@ def copy_calibration():
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n,
                       "launches": 3}), flush=True)
+
+
+# An untimed write of 512 MiB before every timed launch evicts the 256 MiB
+# Infinity Cache (and the L2s), so that no product reads its inputs from a
+# cache the previous one filled.
+FLUSH_BYTES = 512 << 20
+
+
+class Flusher:
+    def __init__(self, torch, dev):
+        self.buf = torch.empty(FLUSH_BYTES // 4, dtype=torch.float32, device=dev)
+        self.k = 0
+
+    def __call__(self):
+        self.k += 1
+        self.buf.fill_(float(self.k))
+
+
+def timed_ms(torch, run, steps, stream, flush=None, warmup=2):
+    """Mean and minimum HIP-event time of `run` on `stream` over `steps`
+    launches, each after an untimed cache flush."""
+    for _ in range(warmup):
+        run()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for e0, e1 in ev:
+        if flush is not None:
+            flush()
+        e0.record(stream)
+        run()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    ms = [e0.elapsed_time(e1) for e0, e1 in ev]
+    return float(np.mean(ms)), float(np.min(ms))
+
+
+def roof(flops, bytes_, ms, dtype):
+    """Bound and fraction of the roofline of one product."""
+    t = ms * 1e-3
+    t_hbm = bytes_ / PEAK_HBM
+    t_cmp = flops / PEAK_FLOPS[dtype]
+    return {"bound": "hbm" if t_hbm >= t_cmp else "mfma", "frac": round(max(t_hbm, t_cmp) / t, 4),
+            "gbps": round(bytes_ / t / 1e9, 1), "tflops": round(flops / t / 1e12, 3)}
 
 
 def case_list(dtypes):
@@ -152,12 +196,14 @@ def gpu_bench(args, cases):
             run(c)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
+    flush = Flusher(torch, dev)
     events = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in cases]
               for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
         for c, (e0, e1) in zip(cases, events[s]):
+            flush()
             e0.record(stream)
             run(c)
             e1.record(stream)
@@ -180,9 +226,34 @@ def gpu_bench(args, cases):
     split = None
     if args.bf16x3 and "f32" in A:
         split = bf16x3_vs_fp32(args, cases, A["f32"], Cbuf["f32"], stream)
-    del A, Cbuf
+    colb = colmajor_m16(args, cases, A, Cbuf, stream, flush) if args.extras else None
+    del A, Cbuf, flush
     torch.cuda.empty_cache()
-    return wall / args.steps, split
+    return wall / args.steps, split, colb
+
+
+def colmajor_m16(args, cases, A, Cbuf, stream, flush):
+    """The m = 16 products of the sweep again with B column-major (pib = {1,2},
+    the layout of the reference's tensor object, include/tlib/detail/tensor.h:84-87)."""
+    import torch
+    import ttm_amd
+    rows = []
+    for c in cases:
+        if c["m"] != 16:
+            continue
+        nq = EXTENTS[c["q"] - 1]
+        bcol = c["B"].t().contiguous()  # n_q x m storage = B column-major
+        dbl = c["dtype"] == "f64"
+        run = lambda: ttm_amd.ttm_raw(c["q"], 3, A[c["dtype"]], c["na"], c["wa"], c["pia"], bcol, [c["m"], nq],
+                                      [1, 2], Cbuf[c["dtype"]], c["nc"], c["wc"])
+        ms, ms_min = timed_ms(torch, run, args.steps, stream, flush)
+        r = roof(c["flops"], c["bytes"], ms, c["dtype"])
+        rows.append({"label": c["label"] + "-bcol",
+                     "symbol": ttm_amd.plan_kernel_symbol(c["q"], EXTENTS, c["pia"], [c["m"], nq], [1, 2], is_double=dbl),
+                     "ms": round(ms, 4), "ms_min": round(ms_min, 4), **r,
+                     "row_major_frac": round(c["frac"], 4)})
+        del bcol
+    return rows
 
 
 OTHER_PRECISIONS = ("bf16", "fp16", "fp8")
@@ -259,21 +330,20 @@ def bf16x3_vs_fp32(args, cases, A, C, stream):
             "error_reference": "float64 definition on 4096 sampled entries", "cases": rows}
 
 
-def tucker_batch(args):
-    """Tucker batch: 256 first-order float32 tensors with extents drawn from
-    U{16..256}, each multiplied in all three modes by factors with orthonormal
-    rows and ranks drawn from U{4..64} (QR of a Gaussian matrix); the whole
-    batch as one plan, replayed as one hipGraph (FP32 and BF16X3)."""
-    import torch
-    from ttm_amd.chain import ChainPlan
-    dev = torch.device("cuda:0")
-    rng = np.random.default_rng(0x77140005)
+def tucker_items(torch, dev, count, seed=0x77140005):
+    """The Tucker batch: `count` first-order float32 tensors with extents
+    drawn from U{16..256}, factors with orthonormal rows (QR of a Gaussian
+    matrix) and ranks drawn from U{4..64}.  Returns the chain items
+    (A, [U1, U2, U3], C), the ranks, the algorithmic flops and the compulsory
+    bytes."""
+    rng = np.random.default_rng(seed)
     gen = torch.Generator(device=dev)
-    gen.manual_seed(0x77140005)
+    gen.manual_seed(seed)
     items = []
+    ranks = []
     flops = 0.0
     compulsory = 0.0
-    for _ in range(args.tucker_count):
+    for _ in range(count):
         na = [int(x) for x in rng.integers(16, 257, size=3)]
         # a factor with orthonormal rows needs r <= n: ranks drawn above the
         # extent are clamped to it
@@ -286,11 +356,22 @@ def tucker_batch(args):
             Us.append(torch.from_numpy(np.ascontiguousarray(Q[:, :r].T)).to(dev, torch.float32))
         C = torch.empty(rk[::-1], device=dev, dtype=torch.float32).permute(2, 1, 0)
         items.append((A, Us, C))
+        ranks.append(rk)
         cur = list(na)
         for k in range(3):  # stages in layout order 1, 2, 3
             flops += 2.0 * rk[k] * float(np.prod(cur))
             cur[k] = rk[k]
         compulsory += 4.0 * (np.prod(na) + sum(r * n for r, n in zip(rk, na)) + np.prod(rk))
+    return items, ranks, flops, compulsory
+
+
+def tucker_batch(args):
+    """Tucker batch (tucker_items) of 256 tensors as one plan, replayed as one
+    hipGraph, in FP32, BF16X3 and FP16X2."""
+    import torch
+    from ttm_amd.chain import ChainPlan
+    dev = torch.device("cuda:0")
+    items, _, flops, compulsory = tucker_items(torch, dev, args.tucker_count)
     out = {"workload": f"{args.tucker_count} tensors, extents U{{16..256}}^3, ranks U{{4..64}}, "
                        "A x1 U1 x2 U2 x3 U3, orthonormal rows, one hipGraph replay per step",
            "gflop": round(flops / 1e9, 2), "compulsory_gb": round(compulsory / 1e9, 3)}
@@ -317,6 +398,161 @@ def tucker_batch(args):
     del items
     torch.cuda.empty_cache()
     return out
+
+
+def _sampled_err(torch, A, na, wa, Bm, C, nc, wc, q, rng, samples=4096):
+    """Normwise relative error of sampled entries of C against the float64
+    definition (Bm is the m x n_q matrix)."""
+    p = len(na)
+    idx = [rng.integers(0, nc[r], size=samples) for r in range(p)]
+    offc = sum(idx[r] * wc[r] for r in range(p))
+    base = sum(idx[r] * wa[r] for r in range(p) if r != q - 1)
+    nq = na[q - 1]
+    fib = torch.from_numpy((base[:, None] + np.arange(nq)[None, :] * wa[q - 1]).astype(np.int64))
+    a_f = A[fib.to(A.device)].double().cpu().numpy()
+    b_r = Bm.double().cpu().numpy()[idx[q - 1]]
+    expect = np.sum(a_f * b_r, axis=1)
+    got = C[torch.from_numpy(offc.astype(np.int64)).to(A.device)].double().cpu().numpy()
+    return float(np.linalg.norm(got - expect) / np.linalg.norm(expect))
+
+
+def _product(torch, dev, na, pia, q, m, colmajor, dtype, gen):
+    """Device buffers and arguments of one product with uniform[-1,1] data."""
+    import ttm_amd
+    tdt = torch.float32 if dtype == "f32" else torch.float64
+    nq = na[q - 1]
+    A = torch.empty(int(np.prod(na)), dtype=tdt, device=dev).uniform_(-1, 1, generator=gen)
+    Bm = torch.empty((m, nq), dtype=tdt, device=dev).uniform_(-1, 1, generator=gen)
+    B = Bm.t().contiguous() if colmajor else Bm
+    nc = list(na)
+    nc[q - 1] = m
+    C = torch.empty(int(np.prod(nc)), dtype=tdt, device=dev)
+    wa, wc = ttm_amd.dense_strides(na, pia), ttm_amd.dense_strides(nc, pia)
+    args = (q, len(na), A, na, wa, pia, B, [m, nq], [1, 2] if colmajor else [2, 1], C, nc, wc)
+    return A, Bm, C, args
+
+
+def _timed_product(torch, args_, na, pia, q, m, colmajor, dtype, gen, flush, precision=None, err=False, rng=None):
+    import ttm_amd
+    A, Bm, C, pargs = _product(torch, torch.device("cuda:0"), na, pia, q, m, colmajor, dtype, gen)
+    stream = torch.cuda.current_stream()
+    run = lambda: ttm_amd.ttm_raw(*pargs, precision=precision)
+    ms, ms_min = timed_ms(torch, run, args_.steps, stream, flush)
+    sym, splits = ttm_amd.last_kernel()
+    flops, bytes_ = work(na, q, m, 4 if dtype == "f32" else 8)
+    row = {"na": na, "pi": pia, "q": q, "m": m, "b": "col" if colmajor else "row", "dtype": dtype,
+           "precision": precision or "default", "symbol": sym, "splits": splits, "ms": round(ms, 4),
+           "ms_min": round(ms_min, 4), **roof(flops, bytes_, ms, dtype)}
+    if err:
+        row["normwise_err_sampled"] = _sampled_err(torch, A, na, pargs[4], Bm, C, pargs[10], pargs[11], q, rng)
+    del A, Bm, C, pargs
+    return row
+
+
+def config1_latency(args):
+    """A 64x48x32 first-order, q = 2, B 40x48 in both orientations, float32
+    and float64: microseconds per call (HIP events around one launch, warm and
+    after a cache flush; and the host clock around 200 back-to-back calls of
+    the device entry) and the normwise error against the float64 definition."""
+    import torch
+    import ttm_amd
+    dev = torch.device("cuda:0")
+    flush = Flusher(torch, dev)
+    stream = torch.cuda.current_stream()
+    na, pia, q, m = [64, 48, 32], [1, 2, 3], 2, 40
+    rows = []
+    for dtype, npdt in (("f32", np.float32), ("f64", np.float64)):
+        rng = np.random.default_rng(0x77140001)
+        a = rng.uniform(-1, 1, int(np.prod(na))).astype(npdt)
+        Bm = rng.uniform(-1, 1, (m, na[q - 1])).astype(npdt)
+        A3 = a.reshape(na[2], na[1], na[0]).astype(np.float64)
+        ref = np.einsum("kli,jl->kji", A3, Bm.astype(np.float64))
+        for colmajor in (True, False):
+            A = torch.from_numpy(a).to(dev)
+            B = torch.from_numpy(np.ascontiguousarray(Bm.T if colmajor else Bm).reshape(-1)).to(dev)
+            nc = [64, m, 32]
+            C = torch.empty(int(np.prod(nc)), dtype=A.dtype, device=dev)
+            wa, wc = ttm_amd.dense_strides(na, pia), ttm_amd.dense_strides(nc, pia)
+            pargs = (q, 3, A, na, wa, pia, B, [m, na[q - 1]], [1, 2] if colmajor else [2, 1], C, nc, wc)
+            run = lambda: ttm_amd.ttm_raw(*pargs)
+            warm, warm_min = timed_ms(torch, run, 200, stream, None, warmup=20)
+            cold, _ = timed_ms(torch, run, 50, stream, flush, warmup=2)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(200):
+                run()
+            torch.cuda.synchronize()
+            host_us = (time.perf_counter() - t0) / 200 * 1e6
+            got = C.double().cpu().numpy().reshape(32, m, 64)
+            den = np.linalg.norm(a.astype(np.float64)) * np.linalg.norm(Bm.astype(np.float64))
+            rows.append({"dtype": dtype, "b": "col" if colmajor else "row", "symbol": ttm_amd.last_kernel()[0],
+                         "device_us_warm": round(warm * 1e3, 2), "device_us_warm_min": round(warm_min * 1e3, 2),
+                         "device_us_after_flush": round(cold * 1e3, 2), "host_us_per_call": round(host_us, 2),
+                         "normwise_err": float(np.linalg.norm(got - ref) / den),
+                         "normwise_bound_4nq_eps": 4 * na[q - 1] * float(np.finfo(npdt).eps)})
+    return rows
+
+
+def config4_order5(args):
+    """Order-5 float32 A 32x24x40x16x48 in a seeded random layout, q = 1..5,
+    B 64 x n_q: FP32 and BF16X3, each product after a cache flush (HBM-bound)."""
+    import torch
+    dev = torch.device("cuda:0")
+    flush = Flusher(torch, dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0x77140004)
+    rng = np.random.default_rng(0x77140004)
+    na = [32, 24, 40, 16, 48]
+    pia = [int(x) + 1 for x in rng.permutation(5)]
+    rows = []
+    for q in range(1, 6):
+        for prec in (None, "bf16x3"):
+            rows.append(_timed_product(torch, args, na, pia, q, 64, False, "f32", gen, flush, precision=prec,
+                                       err=True, rng=rng))
+    return {"pi": pia, "rows": rows,
+            "fp32_min_frac": min(r["frac"] for r in rows if r["precision"] == "default"),
+            "bf16x3_min_frac": min(r["frac"] for r in rows if r["precision"] == "bf16x3")}
+
+
+def split_and_gemv(args):
+    """Products whose blocks alone would not fill the chip: the p = 1 GEMV
+    (n_q = 2^24, m in {1, 16}), A [2^20, 4] with q = 1 at m = 1024, and
+    A [16, 16, 2^20] with q = 3 at m = 1024 -- the skinny and split kernels."""
+    import torch
+    dev = torch.device("cuda:0")
+    flush = Flusher(torch, dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0x77140006)
+    rows = []
+    for dtype in ("f32", "f64"):
+        for m in (1, 16):
+            rows.append(_timed_product(torch, args, [1 << 24], [1], 1, m, False, dtype, gen, flush))
+        rows.append(_timed_product(torch, args, [1 << 20, 4], [1, 2], 1, 1024, True, dtype, gen, flush))
+        rows.append(_timed_product(torch, args, [16, 16, 1 << 20], [1, 2, 3], 3, 1024, True, dtype, gen, flush))
+    torch.cuda.empty_cache()
+    return rows
+
+
+def odd_extents(args):
+    """Extents that are not multiples of the vector width beside aligned
+    neighbours of the same size class (float32, B column-major)."""
+    import torch
+    dev = torch.device("cuda:0")
+    flush = Flusher(torch, dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0x77140007)
+    pairs = [(([999, 1000, 500], [1, 2, 3], 2, 16), ([1000, 1000, 500], [1, 2, 3], 2, 16)),
+             (([999, 1000, 500], [1, 2, 3], 2, 1024), ([1000, 1000, 500], [1, 2, 3], 2, 1024)),
+             (([1023, 1024, 512], [1, 2, 3], 1, 16), ([1024, 1024, 512], [1, 2, 3], 1, 16)),
+             (([1023, 1024, 512], [1, 2, 3], 1, 1024), ([1024, 1024, 512], [1, 2, 3], 1, 1024)),
+             (([1024, 1024, 513], [3, 2, 1], 3, 16), ([1024, 1024, 512], [3, 2, 1], 3, 16))]
+    rows = []
+    for odd, even in pairs:
+        r_odd = _timed_product(torch, args, *odd, True, "f32", gen, flush)
+        r_even = _timed_product(torch, args, *even, True, "f32", gen, flush)
+        rows.append({"odd": r_odd, "aligned": r_even, "frac_ratio": round(r_odd["frac"] / r_even["frac"], 3)})
+        torch.cuda.empty_cache()
+    return rows
 
 
 def host_path():
@@ -432,7 +668,7 @@ def main():
     if args.only:
         keep = set(args.only.split(","))
         cases = [c for c in cases if c["label"] in keep]
-    step_s, split = gpu_bench(args, cases)
+    step_s, split, colb = gpu_bench(args, cases)
     total_flops = sum(c["flops"] for c in cases)
     total_bytes = sum(c["bytes"] for c in cases)
     kern_time = sum(c["ms"] for c in cases) * 1e-3
@@ -469,6 +705,10 @@ def main():
 
     tucker = tucker_batch(args) if args.tucker_count > 0 else None
     host = host_path() if args.host_path else None
+    extras = None
+    if args.extras:
+        extras = {"config1_latency": config1_latency(args), "config4_order5": config4_order5(args),
+                  "colmajor_b_m16": colb, "split_and_gemv": split_and_gemv(args), "odd_extents": odd_extents(args)}
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -498,6 +738,7 @@ def main():
         "bf16x3_vs_fp32": split,
         "tucker_batch": tucker,
         "host_path": host,
+        "extras": extras,
         "cases": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in c.items()
                    if k in ("label", "symbol", "ms", "ms_min", "tflops", "tflops_reference_count", "gbps", "bound",
                            "frac")}

@@ -160,3 +160,53 @@ def test_tucker_batch_reduced(gpu, prec):
     plan.run_graph()
     check(items, refs, torch.float32, f"tucker {prec}", prec=prec)
     plan.close()
+
+
+def _tensordot_chain(A, Us):
+    """A (natural mode order) x_k U_k for k = 1, 2, 3 in float64 through BLAS."""
+    X = A.astype(np.float64)
+    for k, U in enumerate(Us):
+        X = np.moveaxis(np.tensordot(U.astype(np.float64), X, axes=([1], [k])), 0, k)
+    return X
+
+
+@pytest.mark.parametrize("prec", [None, "bf16x3", "fp16x2"])
+def test_tucker_batch_config5_full_size(gpu, prec):
+    """The Tucker batch of the benchmark at its stated size: 256 first-order
+    float32 tensors, extents U{16..256}^3, ranks U{4..64}, orthonormal-row
+    factors (bench.tucker_items, the same seed), as one hipGraph replay of a
+    plan (tuned tiles for FP32), every core against a float64 tensordot
+    chain on the host."""
+    import torch
+    import bench
+    from ttm_amd.chain import ChainPlan
+    items, ranks, flops, _ = bench.tucker_items(torch, gpu, 256)
+    _say(f"config 5 {prec or 'fp32'}: {len(items)} tensors, {flops / 1e9:.1f} GFLOP")
+    plan = ChainPlan(items, precision=prec)
+    if prec is None:
+        plan.tune()
+    for it in items:
+        it[2].fill_(float("nan"))
+    plan.run_graph()
+    torch.cuda.synchronize()
+    _say("stage kernels", plan.stage_kernels)
+    eps = np.finfo(np.float32).eps
+    worst = 0.0
+    for t, (A, Us, C) in enumerate(items):
+        a = A.double().cpu().numpy()
+        us = [U.double().cpu().numpy() for U in Us]
+        ref = _tensordot_chain(a, us)
+        got = C.double().cpu().numpy()
+        err = float(np.linalg.norm(got - ref))
+        if prec is None:
+            nqs = sum(a.shape) + sum(a.shape)
+            scale = np.linalg.norm(a) * np.prod([np.linalg.norm(U) for U in us])
+            bound = 4 * nqs * eps * scale
+        else:
+            bound = 1e-6 * max(np.linalg.norm(ref), 1e-30)
+        worst = max(worst, err / bound)
+        assert np.isfinite(got).all() and err <= bound, (t, a.shape, ranks[t], err, bound)
+    _say(f"config 5 {prec or 'fp32'}: worst error / bound {worst:.3e}")
+    plan.close()
+    del items
+    torch.cuda.empty_cache()
