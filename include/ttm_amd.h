@@ -105,6 +105,28 @@ ttm_status_t ttm_sttm_ex(unsigned q, unsigned p,
                          float *c, const size_t *nc, const size_t *wc,
                          ttm_precision_t precision, ttm_stream_t stream);
 
+/* --- allocation-free variant -------------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The same product as ttm_sttm_ex, on the same kernels,
+ * with every device scratch buffer the call needs (the partial products of a
+ * contraction split, the converted or quantized operands and the scales of
+ * the reduced-precision kernels) taken from a fixed array of
+ * TTM_NOALLOC_SCRATCH_BYTES bytes that the library holds in each GPU's
+ * memory until the process ends, instead of from its memory pool; the call
+ * allocates nothing.  A product whose scratch would exceed the array is
+ * rejected with TTM_ERR_ALLOC before any launch (ttm_last_error() gives the
+ * bytes needed); contraction splits are limited so that their partials fit.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other threads, other streams) race and must be serialised by the
+ * caller. */
+#define TTM_NOALLOC_SCRATCH_BYTES (64u << 20)
+ttm_status_t ttm_sttm_ex_noalloc(unsigned q, unsigned p,
+                                 const float *a, const size_t *na, const size_t *wa, const size_t *pia,
+                                 const float *b, const size_t *nb, const size_t *pib,
+                                 float *c, const size_t *nc, const size_t *wc,
+                                 ttm_precision_t precision, ttm_stream_t stream);
+
 /* --- host pointers, synchronous ---------------------------------------- */
 /* Same contract as include/tlib/ttm.h:54-93 with caller-owned host buffers:
  * C is complete when the call returns. */

@@ -69,6 +69,7 @@ SIGNATURES = [
     ("ttm_sttm", ctypes.c_int, _TTM_ARGS + [_vp]),
     ("ttm_dttm", ctypes.c_int, _TTM_ARGS + [_vp]),
     ("ttm_sttm_ex", ctypes.c_int, _TTM_ARGS + [ctypes.c_int, _vp]),
+    ("ttm_sttm_ex_noalloc", ctypes.c_int, _TTM_ARGS + [ctypes.c_int, _vp]),
     ("ttm_sttm_host", ctypes.c_int, _TTM_ARGS),
     ("ttm_dttm_host", ctypes.c_int, _TTM_ARGS),
     ("ttm_sttm_host_ex", ctypes.c_int, _TTM_ARGS + [ctypes.c_int]),

@@ -382,6 +382,17 @@ ttm_status_t run_host(unsigned q, unsigned p, const T *a, const size_t *na, cons
 
 } // namespace
 
+namespace ttm_amd
+{
+// The float32 device entry, for the allocation-free variant (ttm_noalloc.hip).
+ttm_status_t sttm_device(unsigned q, unsigned p, const float *a, const size_t *na, const size_t *wa,
+                         const size_t *pia, const float *b, const size_t *nb, const size_t *pib, float *c,
+                         const size_t *nc, const size_t *wc, ttm_precision_t prec, hipStream_t s)
+{
+    return run_device<float>(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, prec, s);
+}
+} // namespace ttm_amd
+
 extern "C" {
 
 ttm_status_t ttm_sttm(unsigned q, unsigned p, const float *a, const size_t *na, const size_t *wa,

@@ -775,7 +775,8 @@ Geom geom(const Fold &f, bool is_double)
 // contraction is cut into `splits` ranges of kc indices (a multiple of 64,
 // so that every range starts 16-byte aligned and on a whole 128-byte line of
 // a float32 row) whose partial products are summed by a second pass.  Ranges
-// keep at least 512 indices, and the partials stay within 256 MiB.
+// keep at least 512 indices, and the partials stay within 256 MiB (within the
+// fixed scratch region when one is bound).
 constexpr uint64_t kSplitMinChunk = 512;
 constexpr uint64_t kSplitMaxBytes = 256ull << 20;
 
@@ -790,7 +791,8 @@ Split split_for(const Fold &f, bool is_double, const Geom &g, unsigned want)
         s = tiles >= target ? 1 : (target + tiles - 1) / tiles;
     }
     uint64_t const by_len = f.nq / kSplitMinChunk;
-    uint64_t const by_bytes = kSplitMaxBytes / (f.numel_c() * (is_double ? 8 : 4) + 1);
+    uint64_t const room = std::min<uint64_t>(kSplitMaxBytes, scratch_available());
+    uint64_t const by_bytes = room / (f.numel_c() * (is_double ? 8 : 4) + 1);
     s = std::min<uint64_t>({s, by_len, by_bytes, 65535});
     if (s <= 1) return sp;
     uint64_t kc = (f.nq + s - 1) / s;

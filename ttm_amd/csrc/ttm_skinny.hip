@@ -317,7 +317,7 @@ SkinnyGeom skinny_geom(const Fold &f, bool is_double)
 }
 
 // Splits: about eight blocks per compute unit, ranges of at least 2048
-// indices (a multiple of 64), partials within 256 MiB.
+// indices (a multiple of 64), partials within 256 MiB (or the bound scratch).
 Split skinny_split(const Fold &f, bool is_double, const SkinnyGeom &g, unsigned want)
 {
     Split sp;
@@ -328,7 +328,8 @@ Split skinny_split(const Fold &f, bool is_double, const SkinnyGeom &g, unsigned 
         s = g.nrb >= target ? 1 : (target + g.nrb - 1) / g.nrb;
     }
     uint64_t const by_len = f.nq / 2048;
-    uint64_t const by_bytes = (256ull << 20) / (f.numel_c() * (is_double ? 8 : 4) + 1);
+    uint64_t const room = std::min<uint64_t>(256ull << 20, scratch_available());
+    uint64_t const by_bytes = room / (f.numel_c() * (is_double ? 8 : 4) + 1);
     s = std::min<uint64_t>({s, by_len, by_bytes, 65535});
     if (s <= 1) return sp;
     uint64_t kc = (f.nq + s - 1) / s;

@@ -4,6 +4,7 @@
 
 #include <atomic>
 #include <cstdint>
+#include <cstdio>
 #include <mutex>
 #include <vector>
 
@@ -51,7 +52,10 @@ ttm_status_t ws_acquire(size_t bytes, hipStream_t s, void **out, const char *wha
     size_t const need = (bytes + 255) / 256 * 256;
     if (t_base) {
         if (need > t_cap - t_used) {
-            set_error(TTM_ERR_ALLOC, what);
+            char msg[256];
+            std::snprintf(msg, sizeof msg, "%s: %zu bytes of scratch needed, %zu of the fixed %zu-byte region left",
+                          what, need, t_cap - t_used, t_cap);
+            set_error(TTM_ERR_ALLOC, msg);
             return TTM_ERR_ALLOC;
         }
         *out = static_cast<char *>(t_base) + t_used;
@@ -89,6 +93,8 @@ ScratchScope::~ScratchScope()
 }
 
 bool scratch_bound() { return t_base != nullptr; }
+
+size_t scratch_available() { return t_base ? t_cap - t_used : SIZE_MAX; }
 
 // Queried once per device; concurrent first calls store the same value.
 int device_cus()

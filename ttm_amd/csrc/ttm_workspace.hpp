@@ -46,6 +46,9 @@ class ScratchScope
 
 // True while a ScratchScope is bound on this thread.
 bool scratch_bound();
+// Bytes a ws_acquire can still hand out: the rest of the bound region, or
+// SIZE_MAX for the pool.
+size_t scratch_available();
 
 // Compute units of the current device (256 when it cannot be queried).
 int device_cus();
