@@ -131,7 +131,7 @@ def test_skinny_2p20x4_q1_m1024(gpu, dtype):
         C, wa, nc, wc = _run(torch, ttm_amd, q, A, na, pia, Bm, colmajor, dtype)
         sym, splits = ttm_amd.last_kernel()
         _say(f"{label}: ran {sym} with {splits} splits")
-        assert sym.startswith("k_skinny_") and splits > 8, (sym, splits)
+        assert sym.startswith("k_skinny_") and splits > 1, (sym, splits)
         _sampled(torch, A, na, wa, Bm, C, nc, wc, q, dtype, rng, label)
         del Bm, C
     torch.cuda.empty_cache()

@@ -8,9 +8,10 @@ must cover ceil(n_q / vec) pieces rounded to 32, plus one, and B plus the waves'
 8 KB buffers must fit the 160 KB of LDS of a gfx950 CU.  A forced family whose
 preconditions a problem does not meet is ignored for it.  Invalid metadata
 (the reference's validation failures, include/tlib/ttm.h:65-90) plans nothing.
-The one-thread-per-element kernel runs only when forced: every valid dense
-problem gets a streaming, skinny or tile kernel, the NN tile padding its slabs
-exactly when w % vec != 0.
+The one-thread-per-element kernel runs only when forced or for an A of fewer
+elements than one 16-byte vector: every other valid dense problem gets a
+streaming, skinny or tile kernel, the NN tile padding its slabs exactly when
+w % vec != 0.
 """
 import re
 
@@ -54,8 +55,8 @@ def _check(q, na, pia, nb, pib, dbl, prec, k, sym, forced=False):
         assert k in ("lowp", "lowp_generic"), case
         return
     assert k not in ("lowp", "lowp_generic", "none"), case
-    if not forced:
-        assert k != "generic", case
+    if not forced and int(np.prod(na, dtype=np.int64)) >= vec:
+        assert k != "generic", case  # only an A of fewer elements than a vector
     if k == "skinny":
         assert w == 1 and outer <= 8 and nq >= 512, case
     if k == "gemm":

@@ -181,12 +181,13 @@ struct TileShape
 
 // One BM x BN block of D starting at (j0, n0); sA / sB are the block's two
 // LDS buffers per operand.  Begins and ends with all waves synchronised.
-// PAD (NN view only): slabs of any width w and element-aligned A and C; off,
-// the NN view requires w % VEC == 0 and 16-byte aligned A and C (wp == w).
+// PAD (NN view only): slabs of any width w and element-aligned A and C (a_len:
+// elements of A from a to its end; A holds at least VEC); off, the NN view requires
+// w % VEC == 0 and 16-byte aligned A and C (wp == w).
 template <class T, int JT, int NT, int WAVES_M, int BK, int VIEW, bool BCOL, bool PAD = false>
 __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__restrict__ b, T *__restrict__ c,
                                           uint64_t ncols, uint64_t nq, uint64_t ld, uint64_t w, uint64_t wp,
-                                          uint64_t m, uint64_t j0, uint64_t n0, unsigned flags,
+                                          uint64_t m, uint64_t j0, uint64_t n0, unsigned flags, uint64_t a_len,
                                           T (*sA)[BK * TileShape<T, JT, NT, WAVES_M, BK>::LDA],
                                           T (*sB)[BK * TileShape<T, JT, NT, WAVES_M, BK>::LDB])
 {
@@ -298,9 +299,24 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     int ra_dm[PF][A_PER], rb_dm[PF][B_PER];
     constexpr int DM_WHOLE = VEC << 8; // no shift, all VEC elements valid
 
-    // Interior tile (every row and column in range, whole slices, 16-byte
-    // vectors throughout): unguarded loads, all in flight together.
-    bool const full = n0 + BN <= ncols && j0 + BM <= m && nq % BK == 0 && (a_vec || (!TN && !PAD)) && b_vec_ok;
+    // Interior tile (every row and column in range, 16-byte vectors
+    // throughout): the whole slices run with unguarded loads, all in flight
+    // together, and only a ragged last slice (nq % BK != 0) is guarded.  With
+    // PAD the A vectors are element-aligned; the tile is interior when every
+    // one of them is whole (no slab row end inside a vector).
+    // TN rows and B at element alignment take element-aligned 16-byte loads
+    // (every vector of a whole slice lies inside its row or column); the TT
+    // tiles of the grouped kernels keep the aligned-only rule.
+    bool full = n0 + BN <= ncols && j0 + BM <= m &&
+                (VIEW == VIEW_TT ? (a_vec && b_vec_ok) : (TN || a_vec || !PAD));
+    if constexpr (PAD) {
+        if (!a_vec) {
+            bool whole = n0 + BN <= ncols && j0 + BM <= m;
+#pragma unroll
+            for (int e = 0; e < A_PER; ++e) whole = whole && a_nv[e] == VEC;
+            full = __syncthreads_and(whole ? 1 : 0) != 0;
+        }
+    }
     uint64_t a_fix[A_PER]; // A offset of the thread's vector in slice 0
 #pragma unroll
     for (int e = 0; e < A_PER; ++e) a_fix[e] = TN ? a_base[e] + a_row[e] : a_base[e] + (uint64_t)a_row[e] * w;
@@ -314,23 +330,26 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
             ra_dm[S][e] = DM_WHOLE;
             if constexpr (!TN) {
                 uint64_t const l = l0 + a_row[e];
-                if (a_ok[e] && l < nq) {
-                    if (!PAD || a_vec) {
-                        ra[S][e] = load_a(a + a_base[e] + l * w);
-                    } else if (w >= (uint64_t)VEC) {
-                        // element-aligned vector; a partial last vector of the
-                        // slab row is read as the row's last VEC elements and
-                        // shifted into place when stored
-                        int const d = VEC - a_nv[e];
-                        V const v = ldg<typename Gt::vec_u>(a + a_base[e] + l * w - d);
-                        ra[S][e] = v;
-                        ra_dm[S][e] = d | a_nv[e] << 8;
-                    } else {
-                        // rows shorter than a vector
-#pragma unroll
-                        for (int x = 0; x < VEC; ++x)
-                            if (x < a_nv[e]) ra[S][e][x] = ldg<T>(a + a_base[e] + l * w + x);
-                    }
+                if (!PAD || a_vec) {
+                    if (a_ok[e] && l < nq) ra[S][e] = load_a(a + a_base[e] + l * w);
+                } else {
+                    // Branch-free: the VEC elements from the vector's first
+                    // (element-aligned) address; rows of a slab are
+                    // contiguous, so elements past the slab row's end belong
+                    // to the next row and are masked.  Only at the very end of
+                    // A is the read moved back and shifted into place (d).
+                    // Invalid vectors and rows read in-bounds addresses and
+                    // keep no element (nvalid 0).
+                    // (a_len may be below VEC for the last rows of a split:
+                    // the read then starts before `a`, still inside A, which
+                    // holds at least VEC elements)
+                    uint64_t const lc = l < nq ? l : nq - 1;
+                    int64_t const addr = (int64_t)(a_base[e] + lc * w);
+                    int64_t const lim = (int64_t)a_len - VEC;
+                    int64_t const ac = addr < lim ? addr : lim;
+                    V const v = ldg<typename Gt::vec_u>(a + ac);
+                    ra[S][e] = v;
+                    ra_dm[S][e] = (int)(addr - ac) | (l < nq ? a_nv[e] : 0) << 8;
                 }
             } else if (nq >= (uint64_t)VEC) {
                 // One in-bounds vector per thread and no divergent branch (a
@@ -423,13 +442,30 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     auto load_full = [&](uint64_t l0, auto set) {
         constexpr int S = decltype(set)::value;
 #pragma unroll
-        for (int e = 0; e < A_PER; ++e) ra[S][e] = load_a(a + a_fix[e] + l0 * a_step);
+        for (int e = 0; e < A_PER; ++e) {
+            if constexpr (PAD) {
+                if (!a_vec) {
+                    // element-aligned whole vectors
+                    using VU = typename Gt::vec_u;
+                    ra[S][e] = nt_a ? (V)ldg_nt<VU>(a + a_fix[e] + l0 * a_step) : (V)ldg<VU>(a + a_fix[e] + l0 * a_step);
+                    continue;
+                }
+            }
+            if constexpr (TN && VIEW != VIEW_TT) {
+                if (!a_vec) {
+                    ra[S][e] = ldg<typename Gt::vec_u>(a + a_fix[e] + l0 * a_step);
+                    continue;
+                }
+            }
+            ra[S][e] = load_a(a + a_fix[e] + l0 * a_step);
+        }
 #pragma unroll
         for (int e = 0; e < B_PER; ++e) {
             uint64_t const jj = b_in[e] ? (uint64_t)b_col[e] : 0;
             uint64_t const ll = b_in[e] ? (uint64_t)b_row[e] : 0;
-            if constexpr (BCOL) rb[S][e] = ldg<V>(b + j0 + jj + (l0 + ll) * m);
-            else rb[S][e] = ldg<V>(b + (j0 + jj) * ld + l0 + ll);
+            T const *bp = BCOL ? b + j0 + jj + (l0 + ll) * m : b + (j0 + jj) * ld + l0 + ll;
+            if (VIEW == VIEW_TT || b_vec_ok) rb[S][e] = ldg<V>(bp);
+            else rb[S][e] = ldg<typename Gt::vec_u>(bp);
         }
     };
 
@@ -500,48 +536,59 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     };
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, PF - 1>;
-    auto kloop = [&](auto full_tag) {
+    // Slices [sb, se) of the contraction; starts and ends with all waves
+    // synchronised and both LDS buffers free.
+    auto kloop = [&](auto full_tag, uint64_t sb, uint64_t se) {
         constexpr bool FULL = decltype(full_tag)::value;
-        auto load = [&](uint64_t l0, auto set) {
-            if constexpr (FULL) load_full(l0, set);
-            else load_tiles(l0, set);
+        auto load = [&](uint64_t s, auto set) {
+            if constexpr (FULL) load_full(s * BK, set);
+            else load_tiles(s * BK, set);
         };
+        uint64_t const n = se - sb;
         if constexpr (PF == 1) {
-            load(0, I0());
+            load(sb, I0());
             store_tiles(0, I0(), full_tag);
             __syncthreads();
-            for (uint64_t s = 0; s < nslices; ++s) {
+            for (uint64_t s = 0; s < n; ++s) {
                 int const buf = (int)(s & 1);
-                bool const more = s + 1 < nslices;
-                if (more) load((s + 1) * BK, I0());
+                bool const more = s + 1 < n;
+                if (more) load(sb + s + 1, I0());
                 multiply(buf);
                 if (more) store_tiles(buf ^ 1, I0(), full_tag);
                 __syncthreads();
             }
         } else {
-            // Slice s sits in LDS buffer s & 1 and arrives in register set
-            // s & 1.  A set is reloaded only after its slice was stored, a
+            // Slice sb + s sits in LDS buffer s & 1 and arrives in register
+            // set s & 1.  A set is reloaded only after its slice was stored, a
             // buffer only after the barrier that ended its slice's products.
-            load(0, I0());
-            if (nslices > 1) load(BK, I1());
+            load(sb, I0());
+            if (n > 1) load(sb + 1, I1());
             store_tiles(0, I0(), full_tag);
             __syncthreads();
-            for (uint64_t s = 0; s < nslices; s += 2) {
-                if (s + 2 < nslices) load((s + 2) * BK, I0());
+            for (uint64_t s = 0; s < n; s += 2) {
+                if (s + 2 < n) load(sb + s + 2, I0());
                 multiply(0);
-                if (s + 1 < nslices) store_tiles(1, I1(), full_tag);
+                if (s + 1 < n) store_tiles(1, I1(), full_tag);
                 __syncthreads();
-                if (s + 1 < nslices) {
-                    if (s + 3 < nslices) load((s + 3) * BK, I1());
+                if (s + 1 < n) {
+                    if (s + 3 < n) load(sb + s + 3, I1());
                     multiply(1);
-                    if (s + 2 < nslices) store_tiles(0, I0(), full_tag);
+                    if (s + 2 < n) store_tiles(0, I0(), full_tag);
                     __syncthreads();
                 }
             }
         }
     };
-    if (full) kloop(std::true_type());
-    else kloop(std::false_type());
+    if constexpr (VIEW == VIEW_TT) {
+        // grouped chain stages (contractions of 16..256): one loop per tile,
+        // unguarded only when every slice is whole
+        if (full && nq % BK == 0) kloop(std::true_type(), 0, nslices);
+        else kloop(std::false_type(), 0, nslices);
+    } else {
+        uint64_t const nwhole = full ? nq / BK : 0; // slices with unguarded loads
+        if (nwhole > 0) kloop(std::true_type(), 0, nwhole);
+        if (nwhole < nslices) kloop(std::false_type(), nwhole, nslices);
+    }
 
     // ---- epilogue ---------------------------------------------------------
     uint64_t const ncol0 = n0 + a_off; // first of this lane's NT adjacent columns
@@ -647,13 +694,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const T *__restrict__ a, const 
     block_coords(nj, total, jb, nb);
     uint64_t const l0 = (uint64_t)blockIdx.y * kc;
     uint64_t const len = nq - l0 < kc ? nq - l0 : kc;
+    // elements of A from the split's first row to the end of A (PAD reads)
+    uint64_t const a_len = TN ? 0 : (ncols / wp) * nq * w - l0 * w;
     gemm_tile<T, JT, NT, WAVES_M, BK, TN ? VIEW_TN : VIEW_NN, BCOL, PAD>(
         a + l0 * (TN ? 1 : w), b + l0 * (BCOL ? m : 1), c + (uint64_t)blockIdx.y * c_split, ncols, len, nq, w, wp,
-        m, (uint64_t)jb * S::BM, (uint64_t)nb * S::BN, flags, sA, sB);
+        m, (uint64_t)jb * S::BM, (uint64_t)nb * S::BN, flags, a_len, sA, sB);
 }
 
 // C[i] = sum_{s < S} P[s * n + i] in the order s = 0, 1, ... (deterministic).
 // VEC-wide when n % VEC == 0 and C is 16-byte aligned (the partials are).
+// One thread per element (vector): for many elements and few splits.
 template <class T>
 __global__ __launch_bounds__(256) void k_sum_splits(const T *__restrict__ p, uint64_t n, unsigned nsplit,
                                                     T *__restrict__ c, int vec)
@@ -675,6 +725,36 @@ __global__ __launch_bounds__(256) void k_sum_splits(const T *__restrict__ p, uin
             for (unsigned s = 1; s < nsplit; ++s) acc += p[(uint64_t)s * n + i];
             c[i] = acc;
         }
+    }
+}
+
+// The same sum with a wave per element (vector): lane l adds the splits
+// l, l + 64, ... in order, then the 64 lane sums are added by a fixed
+// butterfly (the result does not depend on scheduling).
+template <class T>
+__global__ __launch_bounds__(256) void k_sum_splits_wave(const T *__restrict__ p, uint64_t n, unsigned nsplit,
+                                                         T *__restrict__ c, int vec)
+{
+    constexpr int VEC = G<T>::VEC;
+    using V = typename G<T>::vec;
+    int const lane = threadIdx.x & 63;
+    uint64_t const v = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (vec) {
+        if (v >= n / VEC) return;
+        V acc = vzero<V>();
+        for (unsigned s = lane; s < nsplit; s += 64) acc += ldg<V>(p + (uint64_t)s * n + v * VEC);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) acc[e] += __shfl_xor(acc[e], o, 64);
+        if (lane == 0) stg<V>(c + v * VEC, acc);
+    } else {
+        if (v >= n) return;
+        T acc = 0;
+        for (unsigned s = lane; s < nsplit; s += 64) acc += p[(uint64_t)s * n + v];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (lane == 0) c[v] = acc;
     }
 }
 
@@ -711,8 +791,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_grouped(const GroupProblem *__r
         unsigned const jb = local % cur.nj, nb = local / cur.nj;
         gemm_tile<T, JT, NT, WAVES_M, BK, VIEW_TT, false>(
             static_cast<const T *>(cur.a), static_cast<const T *>(cur.b), static_cast<T *>(cur.c), cur.ncols,
-            cur.nq, cur.nq, cur.ncols, cur.ncols, cur.m, (uint64_t)jb * S::BM, (uint64_t)nb * S::BN, cur.flags, sA,
-            sB);
+            cur.nq, cur.nq, cur.ncols, cur.ncols, cur.m, (uint64_t)jb * S::BM, (uint64_t)nb * S::BN, cur.flags, 0,
+            sA, sB);
         if (!has_next) break;
         t = tn;
         cur = nxt;
@@ -811,12 +891,15 @@ void launch_cfg(const Fold &f, const Geom &g, const Split &sp, unsigned flags, c
 {
     unsigned const total = g.nj * g.nn;
     uint64_t const c_split = sp.splits > 1 ? f.numel_c() : 0;
-    if (!TN && g.pad)
-        hipLaunchKernelGGL((k_gemm<T, JT, NT, WM, BK, TN, BCOL, true>), dim3(total, sp.splits), dim3(256), 0, s, a,
-                           b, c, g.ncols, f.nq, f.w, g.wp, f.m, g.nj, total, sp.kc, c_split, flags);
-    else
-        hipLaunchKernelGGL((k_gemm<T, JT, NT, WM, BK, TN, BCOL>), dim3(total, sp.splits), dim3(256), 0, s, a, b, c,
-                           g.ncols, f.nq, f.w, g.wp, f.m, g.nj, total, sp.kc, c_split, flags);
+    if constexpr (!TN) {
+        if (g.pad) {
+            hipLaunchKernelGGL((k_gemm<T, JT, NT, WM, BK, TN, BCOL, true>), dim3(total, sp.splits), dim3(256), 0, s,
+                               a, b, c, g.ncols, f.nq, f.w, g.wp, f.m, g.nj, total, sp.kc, c_split, flags);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_gemm<T, JT, NT, WM, BK, TN, BCOL>), dim3(total, sp.splits), dim3(256), 0, s, a, b, c,
+                       g.ncols, f.nq, f.w, g.wp, f.m, g.nj, total, sp.kc, c_split, flags);
 }
 
 template <class T, bool TN, bool BCOL>
@@ -858,6 +941,8 @@ std::string gemm_symbol(const Fold &f, bool is_double, bool aligned)
 
 bool gemm_supports(const Fold &f, bool is_double)
 {
+    // the element-aligned reads take whole vectors from inside A
+    if (f.numel_a() < (is_double ? 2u : 4u)) return false;
     Geom const g = geom(f, is_double);
     // one-dimensional grid of blocks (the splits go along y)
     return (uint64_t)g.nj * g.nn < (1ull << 31) && g.nj > 0 && g.nn > 0;
@@ -921,8 +1006,17 @@ void launch_sum_splits(const T *partials, uint64_t n, unsigned splits, T *c, hip
     constexpr uint64_t VEC = 16 / sizeof(T);
     int const vec = (n % VEC == 0 && aligned16(c) && aligned16(partials)) ? 1 : 0;
     uint64_t const work = vec ? n / VEC : n;
+    uint64_t const cus = (uint64_t)device_cus();
+    if (splits >= 16 && work < cus * 1024) {
+        // few elements, many splits: a wave per element (vector), lanes over
+        // the splits (the per-thread loops would be long chains of loads)
+        uint64_t const blocks = (work + 3) / 4;
+        hipLaunchKernelGGL(k_sum_splits_wave<T>, dim3((unsigned)blocks), dim3(256), 0, s, partials, n, splits, c,
+                           vec);
+        return;
+    }
     uint64_t blocks = (work + 255) / 256;
-    uint64_t const cap = (uint64_t)device_cus() * 8;
+    uint64_t const cap = cus * 8;
     if (blocks > cap) blocks = cap;
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(k_sum_splits<T>, dim3((unsigned)blocks), dim3(256), 0, s, partials, n, splits, c, vec);

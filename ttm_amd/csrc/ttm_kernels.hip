@@ -102,7 +102,10 @@ __global__ __launch_bounds__(256) void k_generic(const T *__restrict__ a, const 
         T const *ap = a + k * nq * w + i;
         T const *bp = b + j * bsj;
         T s = 0;
-        for (uint64_t l = 0; l < nq; ++l) s = __builtin_fma(bp[l * bsl], ap[l * w], s);
+        for (uint64_t l = 0; l < nq; ++l) {
+            if constexpr (sizeof(T) == 4) s = __builtin_fmaf(bp[l * bsl], ap[l * w], s);
+            else s = __builtin_fma(bp[l * bsl], ap[l * w], s);
+        }
         c[idx] = s;
     }
 }
@@ -137,6 +140,19 @@ __device__ __forceinline__ typename Traits<T>::vec load_bvec(const T *bj, uint64
         for (int e = 0; e < Traits<T>::VEC; ++e) r[e] = bj[(l + e) * bsl];
         return r;
     }
+}
+
+// Column-major B (B(j, l) at b[j + l*m]) copied row-major (bt[j*nq + l]) for
+// the streams, which read 4 consecutive l of a row per lane.  One thread per
+// element, writes coalesced (B is small: m <= 64 rows).
+template <class T>
+__global__ __launch_bounds__(256) void k_b_rows(const T *__restrict__ b, T *__restrict__ bt, uint64_t m,
+                                                uint64_t nq)
+{
+    uint64_t const idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= m * nq) return;
+    uint64_t const j = idx / nq, l = idx - j * nq;
+    bt[idx] = b[j + l * m];
 }
 
 // nn_stream: w % VEC == 0.  Fused column index n = k*w + i over all slabs.
@@ -568,8 +584,21 @@ int pick_jt(uint64_t m) { return m <= 16 ? 1 : (m <= 32 ? 2 : 4); }
 // 0.76 / 0.72, depth 7 0.77 / 0.59 and depth 8 0.62 / 0.60 (occupancy falls).
 constexpr int stream_depth(int jt, bool tn) { return jt == 1 ? (tn ? 3 : 6) : (jt == 2 ? (tn ? 2 : 3) : 2); }
 
-// 16-byte vectors of B rows: row-major B with nq % VEC == 0.
-bool stream_bvec(const Fold &f, int vec) { return !f.b_colmajor && f.nq % (uint64_t)vec == 0; }
+// A column-major B of at most this many bytes is copied row-major for the
+// NN stream (one small kernel; reading it in place takes VEC strided loads
+// per lane and group: 0.60-0.73 of 8 TB/s at m = 16 against 0.79-0.86).
+constexpr uint64_t kStreamBRowsMax = 16ull << 20;
+bool stream_b_copy(const Fold &f, bool tn, int vec)
+{
+    return !tn && f.b_colmajor && f.m > 1 && f.nq % (uint64_t)vec == 0 &&
+           f.m * f.nq * (vec == 4 ? 4 : 8) <= kStreamBRowsMax;
+}
+// 16-byte vectors of B rows: row-major B (or a single row, or the row-major
+// copy) with nq % VEC == 0.
+bool stream_bvec(const Fold &f, int vec, bool tn = false)
+{
+    return f.nq % (uint64_t)vec == 0 && (!f.b_colmajor || f.m == 1 || stream_b_copy(f, tn, vec));
+}
 
 // tn_lds instances: B rows of at most SPB_MAX pieces (nq <= VEC * (SPB_MAX - 1)),
 // eight waves per workgroup while B and the eight 8 KB wave buffers fit the
@@ -657,10 +686,26 @@ ttm_status_t launch_plain(const Fold &f, const T *a, const T *b, T *c, hipStream
         // the planner only picks these for m <= 64 (one row block)
         bool const tn = kern == Kernel::tn_stream;
         int const jt = pick_jt(f.m);
-        bool const bv = stream_bvec(f, VEC);
-        if (jt == 1) bv ? launch_stream<T, 1, true>(f, a, b, c, s, tn) : launch_stream<T, 1, false>(f, a, b, c, s, tn);
-        else if (jt == 2) bv ? launch_stream<T, 2, true>(f, a, b, c, s, tn) : launch_stream<T, 2, false>(f, a, b, c, s, tn);
-        else bv ? launch_stream<T, 4, true>(f, a, b, c, s, tn) : launch_stream<T, 4, false>(f, a, b, c, s, tn);
+        bool bv = stream_bvec(f, VEC, tn);
+        Fold g = f;
+        void *ws = nullptr;
+        if (stream_b_copy(f, tn, VEC)) {
+            if (ws_acquire(f.m * f.nq * sizeof(T), s, &ws, "row-major copy of B") == TTM_SUCCESS) {
+                uint64_t const n = f.m * f.nq;
+                hipLaunchKernelGGL(k_b_rows<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b,
+                                   static_cast<T *>(ws), f.m, f.nq);
+                b = static_cast<const T *>(ws);
+                g.b_colmajor = false;
+            } else {
+                bv = false; // no room (the allocation-free entry): read B in place
+            }
+        } else if (f.b_colmajor && f.m == 1) {
+            g.b_colmajor = false; // one row: B(0, l) at b[l] either way
+        }
+        if (jt == 1) bv ? launch_stream<T, 1, true>(g, a, b, c, s, tn) : launch_stream<T, 1, false>(g, a, b, c, s, tn);
+        else if (jt == 2) bv ? launch_stream<T, 2, true>(g, a, b, c, s, tn) : launch_stream<T, 2, false>(g, a, b, c, s, tn);
+        else bv ? launch_stream<T, 4, true>(g, a, b, c, s, tn) : launch_stream<T, 4, false>(g, a, b, c, s, tn);
+        ws_release(ws, s);
         break;
     }
     case Kernel::gemm: return launch_gemm<T>(f, a, b, c, s, splits);
@@ -703,8 +748,8 @@ std::string kernel_symbol(Kernel k, const Fold &f, bool is_double, ttm_precision
                    ", " + std::to_string(smax) + ">";
         }
         return std::string(tn ? "k_tn_stream<" : "k_nn_stream<") + t + ", " + jt + ", " +
-               std::to_string(stream_depth(j, tn)) + ", " + (stream_bvec(f, is_double ? 2 : 4) ? "true" : "false") +
-               ">";
+               std::to_string(stream_depth(j, tn)) + ", " +
+               (stream_bvec(f, is_double ? 2 : 4, tn) ? "true" : "false") + ">";
     }
     case Kernel::gemm: return gemm_symbol(f, is_double, aligned);
     case Kernel::skinny: return skinny_symbol(f, is_double);

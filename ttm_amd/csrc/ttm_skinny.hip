@@ -59,6 +59,10 @@ __device__ __forceinline__ V gld_nt(const void *p)
     return __builtin_nontemporal_load((const __attribute__((address_space(1))) V *)p);
 }
 
+// a * b + c in T's own precision (__builtin_fma is the double one)
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T v)
 {
@@ -111,7 +115,7 @@ __global__ __launch_bounds__(256) void k_skinny_rows(const T *__restrict__ a, co
 #pragma unroll
             for (int n = 0; n < NC; ++n)
 #pragma unroll
-                for (int e = 0; e < VEC; ++e) acc[r][n] = __builtin_fma(bv[r][e], av[n][e], acc[r][n]);
+                for (int e = 0; e < VEC; ++e) acc[r][n] = fmadd(bv[r][e], av[n][e], acc[r][n]);
     };
 
     constexpr uint64_t STEP = 256 * VEC;
@@ -128,6 +132,9 @@ __global__ __launch_bounds__(256) void k_skinny_rows(const T *__restrict__ a, co
 #pragma unroll
             for (int n = 0; n < NC; ++n) av[u][n] = gld<VU>(arow[n] + l + u * STEP);
         }
+        // every load of the batch is issued before the first product (left
+        // alone, the scheduler interleaves them and waits after two)
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < U; ++u) fma_step(bv[u], av[u]);
     }
@@ -138,6 +145,7 @@ __global__ __launch_bounds__(256) void k_skinny_rows(const T *__restrict__ a, co
         for (int r = 0; r < R; ++r) bv[r] = gld_nt<VU>(brow[r] + l);
 #pragma unroll
         for (int n = 0; n < NC; ++n) av[n] = gld<VU>(arow[n] + l);
+        __builtin_amdgcn_sched_barrier(0);
         fma_step(bv, av);
     }
     // the last nq % VEC indices of the product (last range only)
@@ -146,7 +154,7 @@ __global__ __launch_bounds__(256) void k_skinny_rows(const T *__restrict__ a, co
 #pragma unroll
         for (int r = 0; r < R; ++r)
 #pragma unroll
-            for (int n = 0; n < NC; ++n) acc[r][n] = __builtin_fma(brow[r][lt], arow[n][lt], acc[r][n]);
+            for (int n = 0; n < NC; ++n) acc[r][n] = fmadd(brow[r][lt], arow[n][lt], acc[r][n]);
     }
 
     int const lane = tid & 63, wave = tid >> 6;
@@ -219,7 +227,7 @@ __global__ __launch_bounds__(256) void k_skinny_cols(const T *__restrict__ a, co
 #pragma unroll
         for (int v = 0; v < VEC; ++v)
 #pragma unroll
-            for (int n = 0; n < NC; ++n) acc[v][n] = __builtin_fma(bv[v], av[n], acc[v][n]);
+            for (int n = 0; n < NC; ++n) acc[v][n] = fmadd(bv[v], av[n], acc[v][n]);
     };
 
     uint64_t l = lb + tl;
@@ -232,6 +240,7 @@ __global__ __launch_bounds__(256) void k_skinny_cols(const T *__restrict__ a, co
 #pragma unroll
             for (int n = 0; n < NC; ++n) av[u][n] = arow[n][l + (uint64_t)u * LT];
         }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < U; ++u) fma_step(bv[u], av[u]);
     }
@@ -316,18 +325,27 @@ SkinnyGeom skinny_geom(const Fold &f, bool is_double)
     return g;
 }
 
-// Splits: about eight blocks per compute unit, ranges of at least 2048
-// indices (a multiple of 64), partials within 256 MiB (or the bound scratch).
+// Indices a block of the row-major kernel covers per step of its loop (all
+// 256 threads, U vectors each), times the steps a range should at least take.
+uint64_t rows_min_range(const SkinnyGeom &g, int vec)
+{
+    int const u = g.r >= 8 ? 1 : (g.r >= 4 ? 2 : 8);
+    return std::max<uint64_t>(2048, (uint64_t)256 * vec * u * 2);
+}
+
+// Splits: about four blocks per compute unit, ranges long enough for a few
+// rounds of the load pipeline (a multiple of 64), partials within 256 MiB
+// (or the bound scratch).
 Split skinny_split(const Fold &f, bool is_double, const SkinnyGeom &g, unsigned want)
 {
     Split sp;
     sp.kc = f.nq;
     uint64_t s = want;
     if (s == 0) {
-        uint64_t const target = (uint64_t)device_cus() * 8;
+        uint64_t const target = (uint64_t)device_cus() * 4;
         s = g.nrb >= target ? 1 : (target + g.nrb - 1) / g.nrb;
     }
-    uint64_t const by_len = f.nq / 2048;
+    uint64_t const by_len = f.nq / (g.rows ? rows_min_range(g, is_double ? 2 : 4) : 2048);
     uint64_t const room = std::min<uint64_t>(256ull << 20, scratch_available());
     uint64_t const by_bytes = room / (f.numel_c() * (is_double ? 8 : 4) + 1);
     s = std::min<uint64_t>({s, by_len, by_bytes, 65535});
