@@ -65,7 +65,9 @@ typedef enum ttm_status_t {
     TTM_ERR_PRECISION = 22,       /* precision not available for this element type */
     TTM_ERR_HIP = 23,             /* a HIP runtime call failed */
     TTM_ERR_ALLOC = 24,           /* device allocation failed */
-    TTM_ERR_ARGUMENT = 25         /* other invalid argument (batched descriptors) */
+    TTM_ERR_ARGUMENT = 25,        /* other invalid argument (batched descriptors) */
+    TTM_ERR_POLICY = 26           /* a policy tag triple the reference does not define
+                                     (include/tlib/detail/ttm.h:391-400) */
 } ttm_status_t;
 
 /* Arithmetic used for a float32 product.  TTM_PREC_DEFAULT is the plain
@@ -170,6 +172,54 @@ ttm_status_t ttm_dtune(unsigned q, unsigned p,
                        ttm_stream_t stream, unsigned *splits);
 /* Forget every tuned split of this process (the file is left alone). */
 void ttm_tune_clear(void);
+
+/* --- host executor (no GPU) ---------------------------------------------- */
+/* The reference's policy tag triples (include/tlib/detail/tags.h:22-69), for
+ * the host executor: how the work is cut (slices of the fastest mode's extent
+ * or whole subtensors below q) and scheduled over threads (sequential, one
+ * contiguous range per thread, chunks or single items on demand, inside each
+ * product, both, one flat batch, or the reference's `combined` choice). */
+typedef enum ttm_cpu_parallel_t {
+    TTM_CPU_SEQUENTIAL = 0,          /* parallel_policy::sequential */
+    TTM_CPU_PARALLEL_BLAS = 1,       /* parallel_policy::parallel_blas */
+    TTM_CPU_PARALLEL_LOOP = 2,       /* parallel_policy::parallel_loop */
+    TTM_CPU_PARALLEL_TASKLOOP = 3,   /* parallel_policy::parallel_taskloop */
+    TTM_CPU_PARALLEL_TASK = 4,       /* parallel_policy::parallel_task */
+    TTM_CPU_PARALLEL_LOOP_BLAS = 5,  /* parallel_policy::parallel_loop_blas */
+    TTM_CPU_BATCHED_GEMM = 6,        /* parallel_policy::batched_gemm */
+    TTM_CPU_COMBINED = 7             /* parallel_policy::combined */
+} ttm_cpu_parallel_t;
+typedef enum ttm_cpu_slicing_t {
+    TTM_CPU_SLICE = 0,               /* slicing_policy::slice */
+    TTM_CPU_SUBTENSOR = 1,           /* slicing_policy::subtensor */
+    TTM_CPU_SLICING_COMBINED = 2     /* slicing_policy::combined */
+} ttm_cpu_slicing_t;
+typedef enum ttm_cpu_fusion_t {
+    TTM_CPU_FUSION_NONE = 0,         /* fusion_policy::none */
+    TTM_CPU_FUSION_OUTER = 1,        /* fusion_policy::outer */
+    TTM_CPU_FUSION_ALL = 2           /* fusion_policy::all */
+} ttm_cpu_fusion_t;
+
+/* Replaces the reference's host executors, tlib::ttm::ttm(ep, sp, fp, ...)
+ * for each of the 19 tag triples it defines (include/tlib/detail/ttm.h:405-1474),
+ * for callers without a GPU: host pointers, synchronous, `threads` worker
+ * threads (0 = the hardware's), `ratio` the share of the threads on the outer
+ * loop for parallel_loop_blas (ignored otherwise; 0 = half).  Same validation
+ * and statuses as the GPU entries; an undefined triple returns
+ * TTM_ERR_POLICY.  Every triple and thread count gives the same bits: each
+ * element of C is one in-order sum over the contraction. */
+ttm_status_t ttm_sttm_cpu(unsigned q, unsigned p,
+                          const float *a, const size_t *na, const size_t *wa, const size_t *pia,
+                          const float *b, const size_t *nb, const size_t *pib,
+                          float *c, const size_t *nc, const size_t *wc,
+                          ttm_cpu_parallel_t parallel, ttm_cpu_slicing_t slicing, ttm_cpu_fusion_t fusion,
+                          unsigned threads, double ratio);
+ttm_status_t ttm_dttm_cpu(unsigned q, unsigned p,
+                          const double *a, const size_t *na, const size_t *wa, const size_t *pia,
+                          const double *b, const size_t *nb, const size_t *pib,
+                          double *c, const size_t *nc, const size_t *wc,
+                          ttm_cpu_parallel_t parallel, ttm_cpu_slicing_t slicing, ttm_cpu_fusion_t fusion,
+                          unsigned threads, double ratio);
 
 /* --- batched multi-TTM chains (Tucker-style) ------------------------------ */
 /* No single reference counterpart: generalises ttmpy.ttms (the sequential

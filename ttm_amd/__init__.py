@@ -10,12 +10,12 @@ arrays and ROCm torch tensors:
 """
 from ._lib import (LIB_PATH, Precision, Status, TTMError, force_kernel, last_kernel, lib, plan_kernel,
                    plan_kernel_symbol, plan_splits, tune_clear, version, workspace_bytes)
-from .api import dense_strides, layout_from_strides, ttm, ttm_raw, tune_raw
+from .api import dense_strides, layout_from_strides, ttm, ttm_cpu_raw, ttm_raw, tune_raw
 from . import ttmpy
 
 __all__ = ["LIB_PATH", "Precision", "Status", "TTMError", "force_kernel", "last_kernel", "lib", "plan_kernel",
            "plan_kernel_symbol", "plan_splits", "tune_clear", "tune_raw", "version", "workspace_bytes",
-           "dense_strides", "layout_from_strides", "ttm", "ttm_raw", "ttmpy"]
+           "dense_strides", "layout_from_strides", "ttm", "ttm_cpu_raw", "ttm_raw", "ttmpy"]
 from . import chain  # noqa: E402  (batched multi-TTM chains)
 
 __all__.append("chain")

@@ -43,6 +43,7 @@ class Status(enum.IntEnum):
     HIP = 23
     ALLOC = 24
     ARGUMENT = 25
+    POLICY = 26
 
 
 class Precision(enum.IntEnum):
@@ -53,6 +54,13 @@ class Precision(enum.IntEnum):
     FP8_E4M3 = 4
     FP16X2 = 5
 
+
+# The reference's policy tags (include/tlib/detail/tags.h:22-69) as the host
+# executor's enums.
+CPU_PARALLEL = {"sequential": 0, "parallel_blas": 1, "parallel_loop": 2, "parallel_taskloop": 3,
+                "parallel_task": 4, "parallel_loop_blas": 5, "batched_gemm": 6, "combined": 7}
+CPU_SLICING = {"slice": 0, "subtensor": 1, "combined": 2}
+CPU_FUSION = {"none": 0, "outer": 1, "all": 2}
 
 PRECISION_NAMES = {
     "default": Precision.DEFAULT, "fp32": Precision.DEFAULT, "fp64": Precision.DEFAULT,
@@ -87,6 +95,10 @@ SIGNATURES = [
     ("ttm_device_copy", ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     ("ttm_device_zero", ctypes.c_int, [_vp, ctypes.c_size_t]),
     ("ttm_device_synchronize", ctypes.c_int, []),
+    ("ttm_sttm_cpu", ctypes.c_int, _TTM_ARGS + [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint,
+                                                 ctypes.c_double]),
+    ("ttm_dttm_cpu", ctypes.c_int, _TTM_ARGS + [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint,
+                                                 ctypes.c_double]),
     ("ttm_stune", ctypes.c_int, _TTM_ARGS + [_vp, ctypes.POINTER(ctypes.c_uint)]),
     ("ttm_dtune", ctypes.c_int, _TTM_ARGS + [_vp, ctypes.POINTER(ctypes.c_uint)]),
     ("ttm_tune_clear", None, []),

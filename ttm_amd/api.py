@@ -10,7 +10,8 @@ import ctypes
 
 import numpy as np
 
-from ._lib import PRECISION_NAMES, Precision, Status, TTMError, check, lib, sizes
+from ._lib import (CPU_FUSION, CPU_PARALLEL, CPU_SLICING, PRECISION_NAMES, Precision, Status, TTMError, check, lib,
+                   sizes)
 
 try:  # torch is optional for host-only use
     import torch
@@ -166,6 +167,27 @@ def ttm_raw(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, *, precision=None, stre
     else:
         st = L.ttm_dttm_host(*args)
     check(st)
+    return c
+
+
+def ttm_cpu_raw(q, p, a, na, wa, pia, b, nb, pib, c, nc, wc, *, parallel="combined", slicing="combined",
+                fusion="all", threads=0, ratio=0.0):
+    """The reference's low-level call on the host executor (no GPU): NumPy
+    arrays (or CPU torch tensors), the policy tag triple by name
+    (parallel: sequential, parallel_blas, parallel_loop, parallel_taskloop,
+    parallel_task, parallel_loop_blas, batched_gemm, combined; slicing:
+    slice, subtensor, combined; fusion: none, outer, all).  A triple the
+    reference does not define raises TTMError with status POLICY."""
+    device, dt = _check_buffers(p, a, na, b, nb, c, nc)
+    if device:
+        raise TypeError("the host executor takes host arrays")
+    L = lib()
+    args = [ctypes.c_uint(int(q)), ctypes.c_uint(int(p)),
+            _pointer(a), sizes(na), sizes(wa), sizes(pia),
+            _pointer(b), sizes(nb), sizes(pib),
+            _pointer(c), sizes(nc), sizes(wc),
+            CPU_PARALLEL[parallel], CPU_SLICING[slicing], CPU_FUSION[fusion], int(threads), float(ratio)]
+    check((L.ttm_sttm_cpu if dt == np.float32 else L.ttm_dttm_cpu)(*args))
     return c
 
 

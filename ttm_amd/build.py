@@ -29,6 +29,11 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
           f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 
 
+# Per-file flags: the host executor sums without contraction into FMAs, so
+# its results do not depend on the host CPU's instruction set.
+EXTRA = {"ttm_cpu.cpp": ["-ffp-contract=off"]}
+
+
 def _hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if cand and os.path.exists(cand):
@@ -38,7 +43,7 @@ def _hipcc() -> str:
 
 def _compile(hipcc: str, src: pathlib.Path) -> pathlib.Path:
     obj = BUILD / (src.name + ".o")
-    cmd = [hipcc, *COMMON, *DEVICE_FLAGS, "-c", str(src), "-o", str(obj)]
+    cmd = [hipcc, *COMMON, *DEVICE_FLAGS, *EXTRA.get(src.name, []), "-c", str(src), "-o", str(obj)]
     if src.suffix == ".hip":
         cmd[1:1] = ["-x", "hip"]
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -43,6 +43,9 @@ const char *const k_messages[] = {
     "Error in tlib::tensor_times_matrix: HIP runtime error.",
     "Error in tlib::tensor_times_matrix: device memory allocation failed.",
     "Error in tlib::tensor_times_matrix: invalid argument.",
+    // the reference prints this text for a tag triple it does not define
+    // (include/tlib/detail/ttm.h:399) and does nothing
+    "Error in tlib::tensor_times_matrix: This type of ttm is not yet defined",
 };
 constexpr int k_num_messages = sizeof(k_messages) / sizeof(k_messages[0]);
 
