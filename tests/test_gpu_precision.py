@@ -126,6 +126,45 @@ def test_variant_matches_its_emulation(gpu, prec):
                                        sliced=fam == "lowp" and m <= 64)
 
 
+def test_variants_random_shapes_match_their_emulation(gpu):
+    """Seeded random orders, extents (one of about 300 in half the cases:
+    several 128-column blocks, partial slices), layouts, q, m (one or two row
+    blocks, ragged) and B orientations for every reduced precision, each
+    against the emulation of its own arithmetic (the bound of
+    check_vs_emulation); plus FP8 on the NN view with n_q = 300 (partial
+    128-wide scale slices) in both B layouts."""
+    import torch
+    import ttm_amd
+    from ttm_amd._lib import PRECISION_NAMES
+    rng = np.random.default_rng(31)
+    cases = [([12, 300, 8], [1, 2, 3], 2, 70), ([12, 300, 8], [1, 2, 3], 2, 200)]
+    while len(cases) < 26:
+        p = int(rng.integers(2, 5))
+        na = [int(rng.integers(2, 14)) for _ in range(p)]
+        if rng.random() < 0.5:
+            na[int(rng.integers(0, p))] = int(rng.integers(250, 330))
+        if np.prod(na) > 400_000:
+            continue
+        pia = [int(x) + 1 for x in rng.permutation(p)]
+        q = int(rng.integers(1, p + 1))
+        m = int(rng.choice([1, 5, 17, 64, 65, 130, 200]))
+        cases.append((na, pia, q, m))
+    checked = 0
+    for i, (na, pia, q, m) in enumerate(cases):
+        for prec in PRECS:
+            colmajor = bool((i + len(prec)) % 2)
+            for cm in ((False, True) if i < 2 else (colmajor,)):
+                a, wa, Bm, b, nb, pib, nc, wc = case(rng, na, pia, q, m, cm, "uniform")
+                c = run(torch, gpu, q, a, na, wa, pia, b, nb, pib, nc, wc, prec)
+                fam = ttm_amd.plan_kernel(q, na, pia, nb, pib, precision=int(PRECISION_NAMES[prec]))
+                c3 = folded(c, nc, pia, q)
+                label = f"{prec} {na} {pia} q={q} m={m} col={cm} {fam}"
+                _say(label)
+                check_vs_emulation(c3, a, na, pia, q, Bm, prec, label, sliced=fam == "lowp" and m <= 64)
+                checked += 1
+    assert checked >= 5 * 26
+
+
 def test_precision_error_table_64x48x32(gpu, oracle):
     """A 64x48x32 first-order, q=2, B 40x48, float32 inputs, U[-1,1] and log-uniform."""
     import torch

@@ -423,6 +423,79 @@ def test_k_lowp_accesses_in_bounds(BM, view, parts):
                                    a_vec=a_vec, ldb=ldb, plane=plane, parts=parts)
 
 
+def lds_swz(P):
+    return ((P >> 1) & 3) ^ ((P >> 3) & 1)
+
+
+def lowp_dma_b_accesses(chk, BM, *, j0, nq, ldb, plane, parts):
+    """B's planes by LDS DMA in the pipelined BF16X3 tile (dma_b of
+    ttm_lowp.hip), for every slice the loop requests (slices past the end
+    repeat the last one), with each lane's LDS destination checked against
+    its part of the tile: lane L of the 1 KB piece at physical row p0 writes
+    physical row p0 + L / 4, position L % 4."""
+    BK = 32
+    BH = BM // 64
+    nslices = (nq + BK - 1) // BK
+    last = (nslices - 1) * BK
+    for t in range(nslices + 2):
+        l0 = min(t * BK, last)
+        for wv in range(4):
+            for p in range(parts):
+                for h in range(BH):
+                    p0 = 64 * h + 16 * wv
+                    for lane in range(64):
+                        pr = p0 + (lane >> 2)
+                        assert pr < BM and (lane & 3) < 4
+                        off = (lds_row_inv(pr) & 63) * ldb + 8 * ((lane & 3) ^ lds_swz(pr))
+                        # the logical row the register path stores at physical row pr
+                        assert lds_row_inv(pr) // 64 == h
+                        chk("bs", p * plane + (j0 + 64 * h) * ldb + l0 + off, 8)
+
+
+@pytest.mark.parametrize("view", ["NN", "TN"])
+def test_k_lowp_pipelined_accesses_in_bounds(view):
+    """k_lowp<2, TN, 128> (BF16X3, pipelined): A's loads as in the other
+    tiles (two slices ahead, clamped to the last slice), B's planes by LDS
+    DMA; the planes are padded to whole blocks and slices."""
+    BM, parts = 128, 3
+    for nq, m in itertools.product((4, 32, 33, 100, 256), (65, 128, 131, 300)):
+        shapes = [(1, 1), (3, 1), (129, 1), (300, 1)] if view == "TN" else [(1, 128), (2, 4), (3, 132), (1, 256)]
+        for outer, w in shapes:
+            ncols = outer if view == "TN" else outer * w
+            nj = (m + BM - 1) // BM
+            ldb = (nq + 31) // 32 * 32
+            plane = nj * BM * ldb
+            sizes = {"a": outer * nq * w, "b": m * nq, "bs": parts * plane}
+            nn = (ncols + 127) // 128
+            for jb, nb in {(nj - 1, nn - 1), (0, 0)}:
+                chk = Bounds(sizes)
+                lowp_tile_accesses(chk, view, True, BM, ncols=ncols, nq=nq, w=w, m=m, j0=jb * BM, n0=nb * 128,
+                                   a_vec=(nq % 4 == 0) if view == "TN" else True, ldb=ldb, plane=plane,
+                                   parts=parts)
+                lowp_dma_b_accesses(chk, BM, j0=jb * BM, nq=nq, ldb=ldb, plane=plane, parts=parts)
+
+
+def test_dma_lane_order_matches_the_register_layout():
+    """Each (physical row, position) of a B part is written by exactly one
+    DMA lane, with the logical row and chunk the register path puts there
+    (lds_at(P, c) = P * 32 + 8 * (c ^ lds_swz(P)))."""
+    BM = 128
+    seen = {}
+    for wv in range(4):
+        for h in range(2):
+            p0 = 64 * h + 16 * wv
+            for lane in range(64):
+                pr, pos = p0 + (lane >> 2), lane & 3
+                c = pos ^ lds_swz(pr)
+                seen[(pr, pos)] = (lds_row_inv(pr), c)
+    assert len(seen) == BM * 4
+    for (pr, pos), (row, c) in seen.items():
+        # the register path: thread (k_p, k_c) holds logical row lds_row_inv(k_p) (+64), chunk k_c,
+        # and stores it at lds_at(k_p + 64 s, k_c)
+        assert pr * 32 + 8 * pos == pr * 32 + 8 * (c ^ lds_swz(pr))
+        assert row == lds_row_inv(pr)
+
+
 @pytest.mark.parametrize("BM", [64, 128])
 def test_k_lowp_grouped_accesses_in_bounds(BM):
     """k_lowp_grouped: TT tiles converting A and the row-major B themselves."""

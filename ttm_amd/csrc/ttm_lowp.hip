@@ -229,13 +229,19 @@ __device__ __forceinline__ int lds_at(int P, int c) { return P * LP_LDK + 8 * (c
 // planes of `op` at bs + p * plane, row j at j * ldb, zero padded to
 // multiples of 32 columns and 128 rows (k_lowp_split_b); otherwise B is the
 // row-major float32 m x nq matrix b, converted here.
-template <int MODE, int VIEW, bool BSPLIT, int BM = LP_BM>
+// PIPE (with BSPLIT): A's global loads are issued two slices ahead (two
+// register sets), and B's planes go straight from memory into two LDS
+// buffers sB / sB1 by LDS DMA, one slice ahead, so no slice waits for the
+// previous one's loads; the loads that miss the L2 (A's, the first time a
+// column block is read) get two slices of matrix-core work to arrive.
+template <int MODE, int VIEW, bool BSPLIT, int BM = LP_BM, bool PIPE = false>
 __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const float *__restrict__ b,
                                           const typename Lp<MODE>::op *__restrict__ bs, uint64_t ldb, uint64_t plane,
                                           float *__restrict__ c, uint64_t ncols, uint64_t nq, uint64_t w, uint64_t m,
                                           uint64_t j0, uint64_t n0, unsigned flags,
                                           typename Lp<MODE>::op (*sA)[LP_BN * LP_LDK],
-                                          typename Lp<MODE>::op (*sB)[BM * LP_LDK], int ea_in = 0, int ce = 0)
+                                          typename Lp<MODE>::op (*sB)[BM * LP_LDK], int ea_in = 0, int ce = 0,
+                                          typename Lp<MODE>::op (*sB1)[BM * LP_LDK] = nullptr)
 {
     // BM = 128: wave wm owns the 64-row group wm (MFMA tiles t = 0..3);
     // BM = 64 (products of at most 64 rows): both waves along m share the one
@@ -314,13 +320,14 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     // Registers of the next slice's global loads (one set; the index keeps
     // the loaders generic over register sets).
     using I0 = std::integral_constant<int, 0>;
-    f32x4 ra[1][4];
+    static_assert(!PIPE || BSPLIT, "LDS DMA copies B's planes");
+    f32x4 ra[PIPE ? 2 : 1][4];
     f32x4 rb[1][4];
     i32x4 rbs[1][BSPLIT ? P : 1][2];
     auto load_a_vec = [&](const float *p) -> f32x4 {
         return stream ? ldg_nt<f32x4>(p) : ldg<f32x4>(p);
     };
-    auto load = [&](uint64_t l0, auto full_tag, auto set_tag) {
+    auto load_a = [&](uint64_t l0, auto full_tag, auto set_tag) {
         constexpr int S = decltype(set_tag)::value;
         if constexpr (decltype(full_tag)::value) {
             if constexpr (!TN) {
@@ -360,6 +367,9 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                 }
             }
         }
+    };
+    auto load_b = [&](uint64_t l0, auto set_tag) {
+        constexpr int S = decltype(set_tag)::value;
         if constexpr (BSPLIT) {
             // padded planes: every load in range
 #pragma unroll
@@ -386,18 +396,24 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
             }
         }
     };
+    auto load = [&](uint64_t l0, auto full_tag, auto set_tag) {
+        load_a(l0, full_tag, set_tag);
+        load_b(l0, set_tag);
+    };
 
     // Convert and store, in four pieces: piece t of the A tile is row x = t
     // of the MN micro-block (r[e][x] = logical row mn_c4 + x, k mn_k4 + e),
     // or, for K rows, half h = t of physical rows k_p + 64h (r[2h], r[2h+1] =
     // its 8 k; pieces 2 and 3 empty); the B tile goes with piece 3.
-    auto store_piece = [&](int t, auto set_tag, op (*sa)[BN * LP_LDK], op (*sb)[BM * LP_LDK]) {
+    // (A from register set SA, B from set S)
+    auto store_piece = [&](int t, auto set_tag, op (*sa)[BN * LP_LDK], op (*sb)[BM * LP_LDK], auto a_set_tag) {
         constexpr int S = decltype(set_tag)::value;
+        constexpr int SA = decltype(a_set_tag)::value;
         if constexpr (!TN) {
             int const x = t;
             op2 lo[P], hi[P];
-            f32x2 x01 = {ra[S][0][x], ra[S][1][x]};
-            f32x2 x23 = {ra[S][2][x], ra[S][3][x]};
+            f32x2 x01 = {ra[SA][0][x], ra[SA][1][x]};
+            f32x2 x23 = {ra[SA][2][x], ra[SA][3][x]};
             if constexpr (MODE == LP_FP16X2) {
                 x01 = scale2(x01, a_exp());
                 x23 = scale2(x23, a_exp());
@@ -413,7 +429,7 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
             op2 q[4][P];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                f32x2 x = {ra[S][2 * h + u / 2][2 * (u % 2)], ra[S][2 * h + u / 2][2 * (u % 2) + 1]};
+                f32x2 x = {ra[SA][2 * h + u / 2][2 * (u % 2)], ra[SA][2 * h + u / 2][2 * (u % 2) + 1]};
                 if constexpr (MODE == LP_FP16X2) x = scale2(x, a_exp());
                 L::split2(x, q[u]);
             }
@@ -425,7 +441,7 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                 *reinterpret_cast<v8 *>(&sa[p][off]) = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
             }
         }
-        if (t == 3) {
+        if (t == 3 && !PIPE) {
             if constexpr (BSPLIT) {
 #pragma unroll
                 for (int p = 0; p < P; ++p)
@@ -453,9 +469,9 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
             }
         }
     };
-    auto store_all = [&](auto set_tag, op (*sa)[BN * LP_LDK], op (*sb)[BM * LP_LDK]) {
+    auto store_all = [&](auto set_tag, op (*sa)[BN * LP_LDK], op (*sb)[BM * LP_LDK], auto a_set_tag) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) store_piece(t, set_tag, sa, sb);
+        for (int t = 0; t < 4; ++t) store_piece(t, set_tag, sa, sb, a_set_tag);
     };
 
     f32x4 acc[JTL][4];
@@ -483,20 +499,24 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
             v8 fb[P];
 #pragma unroll
             for (int p = 0; p < P; ++p) fb[p] = *reinterpret_cast<const v8 *>(&sb[p][lds_at(brow + 16 * jt, lg)]);
+            // per accumulator the products go smallest first; the four
+            // accumulators of a row tile take turns (independent MFMAs
+            // back to back)
+            auto step = [&](int pb, int pa) {
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) {
-                if constexpr (P == 3) {
-                    acc[jt][nt] = L::mfma(fb[1], fa[1][nt], acc[jt][nt]);
-                    acc[jt][nt] = L::mfma(fb[0], fa[2][nt], acc[jt][nt]);
-                    acc[jt][nt] = L::mfma(fb[2], fa[0][nt], acc[jt][nt]);
-                    acc[jt][nt] = L::mfma(fb[0], fa[1][nt], acc[jt][nt]);
-                    acc[jt][nt] = L::mfma(fb[1], fa[0][nt], acc[jt][nt]);
-                } else if constexpr (P == 2) {
-                    acc[jt][nt] = L::mfma(fb[0], fa[1][nt], acc[jt][nt]);
-                    acc[jt][nt] = L::mfma(fb[1], fa[0][nt], acc[jt][nt]);
-                }
-                acc[jt][nt] = L::mfma(fb[0], fa[0][nt], acc[jt][nt]);
+                for (int nt = 0; nt < 4; ++nt) acc[jt][nt] = L::mfma(fb[pb], fa[pa][nt], acc[jt][nt]);
+            };
+            if constexpr (P == 3) {
+                step(1, 1);
+                step(0, 2);
+                step(2, 0);
+                step(0, 1);
+                step(1, 0);
+            } else if constexpr (P == 2) {
+                step(0, 1);
+                step(1, 0);
             }
+            step(0, 0);
         }
     };
 
@@ -569,7 +589,7 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
             update_scale();
         }
         for (uint64_t s = 0; s < nslices; ++s) {
-            store_all(I0(), sA, sB);
+            store_all(I0(), sA, sB, I0());
             __syncthreads();
             if (s + 1 < nslices) load((s + 1) * BK, full_tag, I0());
             multiply(sA, sB);
@@ -580,8 +600,121 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                 if (s + 1 < nslices) update_scale();
         }
     };
-    if (full) kloop(std::true_type());
-    else kloop(std::false_type());
+    // PIPE.  B's planes by LDS DMA: each lane moves 16 bytes and a wave's
+    // 64 lanes fill 1 KB of LDS in lane order, i.e. 16 physical rows x 4
+    // chunks; lane L of the piece at physical row P0 takes physical row
+    // P = P0 + L / 4, position L % 4, so it loads logical row lds_row_inv(P),
+    // chunk (L % 4) ^ lds_swz(P) (the layout the register path stores).
+    // Wave w moves rows 64h + 16w .. + 15 of each part p: P * BH pieces.
+    // The planes are padded to whole blocks and slices: no load is guarded.
+    auto dma_b = [&](uint64_t l0, op (*sb)[BM * LP_LDK]) {
+        int const wv = __builtin_amdgcn_readfirstlane(wave);
+        int const cp = lane & 3;
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int h = 0; h < BH; ++h) {
+                int const p0 = 64 * h + 16 * wv;
+                int const pr = p0 + (lane >> 2);
+                uint32_t const off = (uint32_t)(lds_row_inv(pr) & 63) * (uint32_t)ldb + 8u * (uint32_t)(cp ^ lds_swz(pr));
+                const op *src = bs + p * plane + (j0 + 64 * h) * ldb + l0 + off;
+                __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)src,
+                                                 (void __attribute__((address_space(3))) *)&sb[p][p0 * LP_LDK], 16,
+                                                 0, 0);
+            }
+    };
+    // Barrier that waits for this wave's LDS operations and for all but the
+    // last `N` of its vector-memory operations (the LDS DMA of the slice
+    // about to be read was issued before the A loads of the next slice).
+    // No fence: the LDS DMA of the next slice stays in flight across it.
+    // The waits go through the builtin (s_waitcnt immediate: vmcnt in bits
+    // 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8), so the compiler counts them
+    // and adds none of its own for the DMA they cover; the barrier is a
+    // compiler memory barrier as well.
+    auto barrier = [&]() { asm volatile("s_barrier" ::: "memory"); };
+    auto sync_vm4 = [&]() {
+        __builtin_amdgcn_s_waitcnt(0x0074); // vmcnt(4) lgkmcnt(0)
+        barrier();
+    };
+    auto sync_vm0 = [&]() {
+        __builtin_amdgcn_s_waitcnt(0x0070); // vmcnt(0) lgkmcnt(0)
+        barrier();
+    };
+    auto sync_lds = [&]() {
+        __builtin_amdgcn_s_waitcnt(0xC07F); // lgkmcnt(0)
+        barrier();
+    };
+    using I1 = std::integral_constant<int, 1>;
+    static_assert(!PIPE || !SLICE_SCALE, "per-slice scales read register set 0");
+    // Slice s: A in register set s % 2 (loaded during slice s - 2), B in
+    // LDS buffer s % 2 (DMA issued during slice s - 1).  Every slice issues
+    // the same loads -- past the end they repeat the last slice, whose data
+    // is never used -- so the compiler's waits count exactly: before A(s) is
+    // stored, DMA(s) and A(s + 1) may stay in flight (10 instructions), and
+    // the wait for this wave's DMA(s) before the barrier leaves A(s + 1) (4
+    // instructions).  The guarded loop issues a varying number: it waits
+    // for all.
+    auto kloop_pipe = [&](auto full_tag) {
+        constexpr bool FULL = decltype(full_tag)::value;
+        uint64_t const last = (nslices - 1) * BK;
+        auto at = [&](uint64_t t) { return t * BK < last ? t * BK : last; };
+        auto wait_b = [&]() {
+            if constexpr (FULL) sync_vm4();
+            else sync_vm0();
+        };
+        load_a(0, full_tag, I0());
+        __builtin_amdgcn_sched_barrier(0);
+        dma_b(0, sB);
+        __builtin_amdgcn_sched_barrier(0);
+        load_a(at(1), full_tag, I1());
+        // The compiler drains every load at the loop head (its count of
+        // loads still in flight does not survive the back edge), so one
+        // trip runs four slices: the drain waits only for the loads issued
+        // during the trip's last slice.  The scheduling barriers keep each
+        // slice's loads ahead of its multiply.
+        auto half0 = [&](uint64_t s) {
+            store_all(I0(), sA, sB, I0());
+            wait_b();
+            dma_b(at(s + 1), sB1);
+            __builtin_amdgcn_sched_barrier(0);
+            load_a(at(s + 2), full_tag, I0());
+            __builtin_amdgcn_sched_barrier(0);
+            multiply(sA, sB);
+            sync_lds();
+        };
+        auto half1 = [&](uint64_t s) {
+            store_all(I0(), sA, sB1, I1());
+            wait_b();
+            dma_b(at(s + 1), sB);
+            __builtin_amdgcn_sched_barrier(0);
+            load_a(at(s + 2), full_tag, I1());
+            __builtin_amdgcn_sched_barrier(0);
+            multiply(sA, sB1);
+            sync_lds();
+        };
+        uint64_t s = 0;
+        for (; s + 3 < nslices; s += 4) {
+            half0(s);
+            half1(s + 1);
+            half0(s + 2);
+            half1(s + 3);
+        }
+        if (s + 1 < nslices) {
+            half0(s);
+            half1(s + 1);
+            s += 2;
+        }
+        if (s < nslices) half0(s);
+        // no LDS DMA may still be writing when the workgroup ends
+        __builtin_amdgcn_s_waitcnt(0x0F70); // vmcnt(0)
+    };
+    if constexpr (PIPE) {
+        if (full) kloop_pipe(std::true_type());
+        else kloop_pipe(std::false_type());
+    } else {
+        if (full) kloop(std::true_type());
+        else kloop(std::false_type());
+    }
     if constexpr (MODE == LP_FP16X2) {
         // ce = ea_in + eb (B's exponent from the pass over B); with slice
         // scales ea_in is 0 and A's exponent is ea, and the grouped blocks
@@ -700,6 +833,8 @@ __global__ __launch_bounds__(256) void k_lowp_split_b(const float *__restrict__ 
 // Blocks per CU the register budget is set for: three (168 registers) where
 // the tile fits or nearly fits them, two (256) for the 128-row BF16X3 tile.
 constexpr int lp_blocks_per_cu(int mode, int bm) { return mode == LP_BF16X3 && bm == 128 ? 2 : 3; }
+// Single-product tiles with the pipelined slice loop (PIPE).
+constexpr bool lp_pipe(int mode, int bm) { return mode == LP_BF16X3 && bm == 128; }
 
 template <int MODE, bool TN, int BM>
 __global__ __launch_bounds__(256, lp_blocks_per_cu(MODE, BM)) void k_lowp(const float *__restrict__ a, const typename Lp<MODE>::op *__restrict__ bs,
@@ -710,13 +845,15 @@ __global__ __launch_bounds__(256, lp_blocks_per_cu(MODE, BM)) void k_lowp(const 
     using op = typename Lp<MODE>::op;
     __shared__ __attribute__((aligned(16))) op sA[Lp<MODE>::PARTS][LP_BN * LP_LDK];
     __shared__ __attribute__((aligned(16))) op sB[Lp<MODE>::PARTS][BM * LP_LDK];
+    // second B buffer of the pipelined loop (unreferenced, and so not
+    // allocated, otherwise)
+    __shared__ __attribute__((aligned(16))) op sB1[Lp<MODE>::PARTS][BM * LP_LDK];
     unsigned jb, nb;
     lp_block_coords(nj, total, jb, nb);
-    lowp_tile<MODE, TN ? LPV_TN : LPV_NN, true, BM>(a, nullptr, bs, ldb, plane, c, ncols, nq, w, m, (uint64_t)jb * BM,
-                                                    (uint64_t)nb * LP_BN,
-                                                    TILE_A_VEC | TILE_C_VEC | (nj == 1 ? TILE_STREAM : 0u), sA, sB,
-                                                    MODE == LP_FP16X2 ? exps[0] : 0,
-                                                    MODE == LP_FP16X2 ? exps[0] + exps[1] : 0);
+    lowp_tile<MODE, TN ? LPV_TN : LPV_NN, true, BM, lp_pipe(MODE, BM)>(
+        a, nullptr, bs, ldb, plane, c, ncols, nq, w, m, (uint64_t)jb * BM, (uint64_t)nb * LP_BN,
+        TILE_A_VEC | TILE_C_VEC | (nj == 1 ? TILE_STREAM : 0u), sA, sB, MODE == LP_FP16X2 ? exps[0] : 0,
+        MODE == LP_FP16X2 ? exps[0] + exps[1] : 0, sB1);
 }
 
 // FP16X2 scales: the largest magnitude of A (blocks [0, blocks_a)) and of B
