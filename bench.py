@@ -41,19 +41,19 @@ METRIC = "TTM TFLOP/s + HBM GB/s vs roofline; BF16x3 vs FP32 at equal FP64-oracl
 # WRITE_SIZE; KiB per dispatch, mean over the dispatches of the case), keyed by
 # kernel symbol and case.  Both counters are scaled by what they read for a
 # plain 1 GiB device copy (`bench.py --copy-calibration`): FETCH_SIZE reads
-# 524,305 KiB for 1,048,576 KiB fetched (scale 2.0), WRITE_SIZE 1,048,576 KiB
+# 524,303 KiB for 1,048,576 KiB fetched (scale 2.0), WRITE_SIZE 1,048,576 KiB
 # for 1,048,576 KiB written (scale 1.0).  Summaries and commands:
 # profiles/ttm_hbm_traffic_pmc.md.  A symbol or case without an entry
 # reports traffic null.
 PMC_FETCH_SCALE = 2.0
 PMC_WRITE_SCALE = 1.0
 PMC_TRAFFIC_KIB = {
-    "k_gemm<double, 4, 4, 2, 16, false, false, false>": {"f64-first-q2-m1024": (6264796.8, 4198676.5)},
-    "k_gemm<float, 4, 4, 2, 16, false, false, false>": {"f32-first-q2-m1024": (3084220.3, 2097152.0)},
-    "k_nn_stream<float, 1, 6, true>": {"f32-first-q2-m16": (1048888.5, 32768.0),
-                                        "f32-first-q3-m16": (1048761.0, 65536.0)},
-    "k_tn_lds<float, 8, 2, 257>": {"f32-first-q1-m16": (1053797.3, 32768.0)},
-    "k_tn_lds<double, 3, 2, 513>": {"f64-first-q1-m16": (2099067.8, 65538.5)},
+    "k_gemm<double, 4, 4, 2, 16, false, false, false>": {"f64-first-q2-m1024": (6264054.0, 4200535.7)},
+    "k_gemm<float, 4, 4, 2, 16, false, false, false>": {"f32-first-q2-m1024": (3081001.4, 2097152.0)},
+    "k_nn_stream<float, 1, 6, true>": {"f32-first-q2-m16": (1048885.0, 32768.0),
+                                        "f32-first-q3-m16": (1048757.6, 65536.0)},
+    "k_tn_lds<float, 8, 2, 257>": {"f32-first-q1-m16": (1053808.5, 32768.0)},
+    "k_tn_lds<double, 3, 2, 513>": {"f64-first-q1-m16": (2099106.4, 65537.5)},
 }
 
 
