@@ -249,7 +249,15 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     // order is the same (tile t of group h reads physical rows 64h + 16t + lc).
     static_assert(BM == 128 || BM == 64, "64- or 128-row blocks");
     constexpr int BH = BM / 64;  // 64-row groups of the block
-    constexpr int JTL = BM / 32; // MFMA row tiles per wave
+    constexpr bool TN = VIEW != LPV_NN;
+    constexpr bool TSTORE = VIEW == LPV_TN;
+    // COLSPLIT (64-row blocks storing C[n][j]): the four waves split the
+    // columns instead (32 each) and every wave takes all four row tiles, so a
+    // lane holds four consecutive rows j of a column of C and stores them as
+    // one 16-byte vector (with the waves split along m it held two).
+    constexpr bool COLSPLIT = BM == 64 && TSTORE;
+    constexpr int JTL = COLSPLIT ? 4 : BM / 32; // MFMA row tiles per wave
+    constexpr int NTL = COLSPLIT ? 2 : 4;       // MFMA column tiles per wave
     using L = Lp<MODE>;
     using op = typename L::op;
     using op2 = typename L::v2;
@@ -257,8 +265,6 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
     using v8 = typename L::v8;
     constexpr int P = L::PARTS;
     constexpr int BN = LP_BN, BK = LP_BK;
-    constexpr bool TN = VIEW != LPV_NN;
-    constexpr bool TSTORE = VIEW == LPV_TN;
     bool const a_vec = (flags & TILE_A_VEC) != 0;
     bool const c_vec = (flags & TILE_C_VEC) != 0;
     bool const stream = (flags & TILE_STREAM) != 0;
@@ -474,26 +480,26 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
         for (int t = 0; t < 4; ++t) store_piece(t, set_tag, sa, sb, a_set_tag);
     };
 
-    f32x4 acc[JTL][4];
+    f32x4 acc[JTL][NTL];
 #pragma unroll
     for (int jt = 0; jt < JTL; ++jt)
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) acc[jt][nt] = zero4();
+        for (int nt = 0; nt < NTL; ++nt) acc[jt][nt] = zero4();
 
     // Interleaved fragments: tile jt holds rows 4*x + jt (x < 16) of the
     // wave's 64 rows, tile nt columns 4*y + nt, so a lane's 4 results of a
     // row (column) are adjacent and leave with one 16-byte store.  Physical
     // rows: tile t, lane lc -> 64*half + 16*t + lc.
-    int const arow = wn * 64 + lc; // + 16 * nt
-    int const brow = wm * (BM / 2) + lc; // + 16 * jt
+    int const arow = COLSPLIT ? wave * 32 + lc : wn * 64 + lc; // + 16 * nt
+    int const brow = COLSPLIT ? lc : wm * (BM / 2) + lc;       // + 16 * jt
 
     // The products of one slice from the LDS tiles (sa, sb).
     auto multiply = [&](op (*sa)[BN * LP_LDK], op (*sb)[BM * LP_LDK]) {
-        v8 fa[P][4];
+        v8 fa[P][NTL];
 #pragma unroll
         for (int p = 0; p < P; ++p)
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) fa[p][nt] = *reinterpret_cast<const v8 *>(&sa[p][lds_at(arow + 16 * nt, lg)]);
+            for (int nt = 0; nt < NTL; ++nt) fa[p][nt] = *reinterpret_cast<const v8 *>(&sa[p][lds_at(arow + 16 * nt, lg)]);
 #pragma unroll
         for (int jt = 0; jt < JTL; ++jt) {
             v8 fb[P];
@@ -504,7 +510,7 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
             // back to back)
             auto step = [&](int pb, int pa) {
 #pragma unroll
-                for (int nt = 0; nt < 4; ++nt) acc[jt][nt] = L::mfma(fb[pb], fa[pa][nt], acc[jt][nt]);
+                for (int nt = 0; nt < NTL; ++nt) acc[jt][nt] = L::mfma(fb[pb], fa[pa][nt], acc[jt][nt]);
             };
             if constexpr (P == 3) {
                 step(1, 1);
@@ -573,7 +579,7 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
 #pragma unroll
             for (int jt = 0; jt < JTL; ++jt)
 #pragma unroll
-                for (int nt = 0; nt < 4; ++nt)
+                for (int nt = 0; nt < NTL; ++nt)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) acc[jt][nt][r] = ldexpf(acc[jt][nt][r], d);
         }
@@ -723,7 +729,7 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
 #pragma unroll
         for (int jt = 0; jt < JTL; ++jt)
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt)
+            for (int nt = 0; nt < NTL; ++nt)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) acc[jt][nt][r] = ldexpf(acc[jt][nt][r], -cfin);
     }
@@ -751,21 +757,28 @@ __device__ __forceinline__ void lowp_tile(const float *__restrict__ a, const flo
                         if (ncol0 + nt < ncols) stg<float>(cb + j * w + nt, acc[jt][nt][r]);
                 }
             }
-    } else if constexpr (BM == 64) {
-        // C[n][j]: the lane's two tiles are the adjacent rows 2 wm, 2 wm + 1 of
-        // each group of four
+    } else if constexpr (COLSPLIT) {
+        // C[n][j]: MFMA tile nt of wave w reads physical A rows 32 w + 16 nt + lc,
+        // i.e. column 64 (w / 2) + 4 lc + 2 (w % 2) + nt; its row tile jt reads
+        // physical B rows 16 jt + lc, so accumulator element r of tile jt is
+        // row 4 (4 lg + r) + jt: the four tiles give four consecutive rows
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            uint64_t const k = ncol0 + nt;
+        for (int nt = 0; nt < NTL; ++nt) {
+            uint64_t const k = n0 + 64 * (wave >> 1) + 4 * lc + 2 * (wave & 1) + nt;
             if (k >= ncols) continue;
             float *crow = c + k * m;
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+            for (int r = 0; r < 4; ++r) {
+                uint64_t const jr = j0 + 4 * (4 * lg + r);
+                if ((m & 3) == 0 && jr + 4 <= m) {
+                    f32x4 o = {acc[0][nt][r], acc[1][nt][r], acc[2][nt][r], acc[3][nt][r]};
+                    stg<f32x4>(crow + jr, o);
+                } else {
 #pragma unroll
-                for (int jt = 0; jt < JTL; ++jt) {
-                    uint64_t const j = j0 + 4 * (4 * lg + r) + 2 * wm + jt;
-                    if (j < m) stg<float>(crow + j, acc[jt][nt][r]);
+                    for (int jt = 0; jt < 4; ++jt)
+                        if (jr + jt < m) stg<float>(crow + jr + jt, acc[jt][nt][r]);
                 }
+            }
         }
     } else {
 #pragma unroll
