@@ -227,7 +227,8 @@ ttm_status_t ttm_dttm_cpu(unsigned q, unsigned p,
  * uneven extents in one plan.  Tensor t of a plan computes
  *     C = A x_{k1} U_{k1} x_{k2} U_{k2} ...   over every mode k with u[k-1] != NULL
  * where U_k is the row-major r[k-1] x n[k-1] factor of mode k; modes with
- * u[k-1] == NULL keep their extent.  A and C are dense in layout pi, in device
+ * u[k-1] == NULL keep their extent (their stage is a copy into the next
+ * stage's layout, not a product).  A and C are dense in layout pi, in device
  * memory.  Modes are contracted in layout order; every stage reads rows that
  * are contiguous along the contracted mode and writes that mode slowest, so the
  * result lands back in layout pi without any transposition. */

@@ -94,6 +94,43 @@ def test_random_batch(gpu, dt):
     plan.close()
 
 
+def test_modes_without_factor_are_copies(gpu):
+    """A mode whose factor is None keeps its extent through a copy stage
+    (k_transpose_grouped), exactly: a long unfactored mode (60000, whose
+    identity would take 14 GB), every mode unfactored (the chain is a
+    layout round trip), and unfactored modes between factored ones, in both
+    element types; the copy stages are bit-exact, so a chain of copies
+    returns A unchanged."""
+    import torch
+    from ttm_amd.chain import ChainPlan
+    rng = np.random.default_rng(21)
+    for dtype in (torch.float32, torch.float64):
+        specs = [([4, 3, 60000], [2, 1, 3], [True, True, False]),
+                 ([7, 130, 5, 9], [3, 1, 4, 2], [False, False, False, False]),
+                 ([33, 65, 17], [1, 2, 3], [False, True, False]),
+                 ([70, 3, 66], [3, 2, 1], [True, False, True])]
+        items, refs = [], []
+        for na, pi, has in specs:
+            A = rng.uniform(-1, 1, size=na)
+            Us = [rng.uniform(-1, 1, size=(int(rng.integers(1, 9)), n)) if h else None for n, h in zip(na, has)]
+            nc = [U.shape[0] if U is not None else n for U, n in zip(Us, na)]
+            At = to_layout(torch, gpu, A, pi, dtype)
+            Ut = [torch.from_numpy(U).to(gpu, dtype).contiguous() if U is not None else None for U in Us]
+            Ct = to_layout(torch, gpu, np.full(nc, np.nan), pi, dtype)
+            items.append((At, Ut, Ct, pi))
+            refs.append((A, Us, einsum_chain(A, Us)))
+        plan = ChainPlan(items)
+        kernels = plan.stage_kernels
+        _say(dtype, "stage kernels", kernels)
+        assert any(k.startswith("k_transpose_grouped<") for k in kernels)
+        plan.run()
+        torch.cuda.synchronize()
+        # all modes unfactored: C is A, bit for bit, in the same layout
+        assert torch.equal(items[1][2], items[1][0])
+        check(items, refs, dtype, f"copy stages {dtype}")
+        plan.close()
+
+
 def test_graph_replay_matches_eager(gpu):
     """Replays of the captured chain, with eager work in between, equal the eager run."""
     import torch

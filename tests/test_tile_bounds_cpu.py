@@ -17,6 +17,8 @@ import itertools
 import pathlib
 import re
 
+import numpy as np
+
 import pytest
 
 ROOT = pathlib.Path(__file__).resolve().parents[1]
@@ -526,3 +528,44 @@ def test_the_model_catches_the_fault_it_was_written_for():
             jj = bc  # the old fallback kept the thread's own column for b_in == False
             ll = brow if b_in else 0
             chk("b", (0 + jj) * nq + (nq - BK) + ll, vec)
+
+
+def test_transpose_grouped_tiles_cover_every_element_once():
+    """k_transpose_grouped (chain stages over modes without a factor): the
+    tile enumeration of the stage (nj = ceil(nq / 64) l-blocks per 64-column
+    block, l-blocks fastest, problems concatenated) reads only inside each
+    A, writes only inside each C, and writes every C element exactly once
+    with A's element at the transposed place."""
+    rng = np.random.default_rng(5)
+    probs = [(int(rng.integers(1, 200)), int(rng.integers(1, 150))) for _ in range(12)] + [(64, 64), (1, 1), (65, 63)]
+    tiles = []
+    for pid, (ncols, nq) in enumerate(probs):
+        nj, nn = (nq + 63) // 64, (ncols + 63) // 64
+        begin = len(tiles)
+        tiles += [(pid, begin, nj)] * (nj * nn)
+    srcs = [rng.standard_normal(ncols * nq) for ncols, nq in probs]
+    outs = [np.full(ncols * nq, np.nan) for ncols, nq in probs]
+    writes = [np.zeros(ncols * nq, dtype=int) for ncols, nq in probs]
+    for t, (pid, begin, nj) in enumerate(tiles):
+        ncols, nq = probs[pid]
+        local = t - begin
+        l0, n0 = (local % nj) * 64, (local // nj) * 64
+        tile = np.zeros((64, 65))
+        for ty in range(4):
+            for tx in range(64):
+                for r in range(ty, 64, 4):
+                    n, l = n0 + r, l0 + tx
+                    if n < ncols and l < nq:
+                        assert 0 <= n * nq + l < ncols * nq
+                        tile[r][tx] = srcs[pid][n * nq + l]
+        for ty in range(4):
+            for tx in range(64):
+                for r in range(ty, 64, 4):
+                    l, n = l0 + r, n0 + tx
+                    if l < nq and n < ncols:
+                        assert 0 <= l * ncols + n < ncols * nq
+                        outs[pid][l * ncols + n] = tile[tx][r]
+                        writes[pid][l * ncols + n] += 1
+    for (ncols, nq), a, c, w in zip(probs, srcs, outs, writes):
+        assert (w == 1).all()
+        assert np.array_equal(c.reshape(nq, ncols), a.reshape(ncols, nq).T)

@@ -3,6 +3,9 @@
 ``ChainPlan(items)`` prepares one plan for many tensors of uneven extents; each
 item is ``(A, factors, C)`` with ROCm torch tensors: A and C dense in the same
 layout, ``factors[k]`` the row-major ``r_k x n_k`` factor of mode k+1 or None.
+A mode whose factor is None keeps its extent: its stage only moves the data
+into the next stage's layout (a batched transpose, one read and one write of
+the tensor), no product.
 ``plan.run()`` launches one persistent grouped kernel per stage,
 ``plan.run_graph()`` replays the whole chain as one hipGraph, ``plan.tune()``
 picks tile shapes by timing them.

@@ -6,7 +6,9 @@
 // pi_s (it is the fastest mode), multiplies them by U_{pi_s} and writes the
 // result with that mode slowest (the TT view of ttm_gemm.hip).  The layout
 // therefore rotates by one position per stage and after p stages is pi again.
-// A mode without a factor is carried through by the identity.
+// A mode without a factor is carried through by that rotation alone: its
+// stage copies the current tensor into the rotated layout (a batched
+// transpose, k_transpose_grouped) instead of multiplying by an identity.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,6 +38,7 @@ struct Stage
     std::vector<GroupProblem> probs;   // tile_begin / nj filled for `cfg`
     std::vector<unsigned> tile_prob;
     GroupConfig cfg{4, 4, 1, 16};
+    bool transpose = false;            // modes without a factor: a copy, 64 x 64 tiles
     unsigned total = 0;
     unsigned grid = 0;
     GroupProblem *d_probs = nullptr;   // inside the plan's tile allocation
@@ -85,7 +88,7 @@ struct ttm_chain_plan_s
     bool is_double = false;
     ttm_precision_t prec = TTM_PREC_DEFAULT;
     std::vector<Stage> stages;
-    void *d_static = nullptr;   // identities + intermediates
+    void *d_static = nullptr;   // intermediates
     void *d_tiles = nullptr;    // descriptors + tile maps of all stages
     size_t tiles_bytes = 0;
     int num_cu = 0;
@@ -135,7 +138,8 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
 {
     size_t bytes = 0;
     for (Stage &st : plan->stages) {
-        unsigned const bm = (unsigned)st.cfg.bm(), bn = (unsigned)st.cfg.bn();
+        unsigned const bm = st.transpose ? 64u : (unsigned)st.cfg.bm();
+        unsigned const bn = st.transpose ? 64u : (unsigned)st.cfg.bn();
         st.tile_prob.clear();
         unsigned begin = 0;
         // Workgroup b takes tiles b, b + grid, b + 2 grid, ...; a tile's cost
@@ -160,8 +164,9 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
             begin += (unsigned)tiles;
         }
         st.total = begin;
-        int occ = lowp_group_precision(plan->prec) ? lowp_group_occupancy(plan->prec, st.cfg.bm())
-                                                : group_occupancy(st.cfg, plan->is_double);
+        int occ = st.transpose                     ? 8
+                  : lowp_group_precision(plan->prec) ? lowp_group_occupancy(plan->prec, st.cfg.bm())
+                                                     : group_occupancy(st.cfg, plan->is_double);
         if (occ <= 0) occ = 1;
         uint64_t const resident = (uint64_t)occ * (uint64_t)std::max(plan->num_cu, 1);
         st.grid = (unsigned)std::min<uint64_t>(st.total, resident);
@@ -202,6 +207,9 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
 ttm_status_t launch_stage(ttm_chain_plan_t plan, const Stage &st, hipStream_t s)
 {
     if (st.total == 0) return TTM_SUCCESS;
+    if (st.transpose)
+        return plan->is_double ? launch_transpose_grouped<double>(st.d_probs, st.d_tile_prob, st.total, st.grid, s)
+                               : launch_transpose_grouped<float>(st.d_probs, st.d_tile_prob, st.total, st.grid, s);
     if (lowp_group_precision(plan->prec))
         return launch_lowp_grouped(plan->prec, st.cfg.bm(), st.d_probs, st.d_tile_prob, st.total, st.grid, s);
     if (plan->is_double)
@@ -235,7 +243,6 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
     // ---- validation and sizes ------------------------------------------------
     size_t max_p = 0, ws_total = 0;
     std::vector<size_t> ws_of(count, 0);
-    std::vector<uint64_t> ident_sizes; // distinct extents that need an identity factor
     for (size_t t = 0; t < count; ++t) {
         ttm_chain_desc_t const &d = descs[t];
         char msg[96];
@@ -256,8 +263,6 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
         uint64_t biggest = 0;
         for (unsigned s = 0; s < d.p; ++s) {
             unsigned const mode = (unsigned)d.pi[s] - 1;
-            if (!d.u[mode] && std::find(ident_sizes.begin(), ident_sizes.end(), d.n[mode]) == ident_sizes.end())
-                ident_sizes.push_back(d.n[mode]);
             cur[mode] = d.u[mode] ? d.r[mode] : d.n[mode];
             if (s + 1 < d.p) {
                 uint64_t sz = 1;
@@ -281,47 +286,29 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
     }
     plan->num_cu = prop.multiProcessorCount;
 
-    // ---- static allocation: identities, then ping-pong intermediates --------
-    size_t ident_bytes = 0;
-    for (uint64_t n : ident_sizes) ident_bytes = align_up(ident_bytes + (size_t)(n * n) * esz, 256);
-    size_t const static_bytes = std::max<size_t>(ident_bytes + ws_total, 256);
+    // ---- static allocation: ping-pong intermediates ----------------------------
+    size_t const static_bytes = std::max<size_t>(ws_total, 256);
     if (hipMalloc(&plan->d_static, static_bytes) != hipSuccess) {
         (void)hipGetLastError();
         delete plan;
         return fail(TTM_ERR_ALLOC, "chain intermediates");
     }
     char *sbase = static_cast<char *>(plan->d_static);
-    std::map<uint64_t, const void *> ident_ptr;
-    {
-        size_t off = 0;
-        for (uint64_t n : ident_sizes) {
-            std::vector<char> host((size_t)(n * n) * esz, 0);
-            for (uint64_t i = 0; i < n; ++i) {
-                if (is_double) reinterpret_cast<double *>(host.data())[i * n + i] = 1.0;
-                else reinterpret_cast<float *>(host.data())[i * n + i] = 1.0f;
-            }
-            if (hipMemcpy(sbase + off, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
-                (void)hipGetLastError();
-                ttm_chain_plan_destroy(plan);
-                return fail(TTM_ERR_HIP, "identity upload");
-            }
-            ident_ptr[n] = sbase + off;
-            off = align_up(off + host.size(), 256);
-        }
-    }
 
     // ---- stages ---------------------------------------------------------------
     // Stage s of every chain, split by row class of m (<= 16, <= 32, <= 64,
-    // larger) so that each launch gets a tile height that fits its ranks;
-    // launches of one stage are independent, stages run in order.
+    // larger) so that each launch gets a tile height that fits its ranks,
+    // plus one copy launch for the modes without a factor; launches of one
+    // stage are independent, stages run in order.
     int const classes = lowp_group_precision(precision) ? 2 : 4;
+    int const groups_per_stage = classes + 1;
     auto m_class = [&](uint64_t m) {
         if (classes == 2) return m <= 64 ? 0 : 1; // the two reduced-precision tile heights
         return m <= 16 ? 0 : m <= 32 ? 1 : m <= 64 ? 2 : 3;
     };
-    std::vector<Stage> groups(max_p * classes);
+    std::vector<Stage> groups(max_p * groups_per_stage);
     std::vector<uint64_t> stage_max_m(groups.size(), 0), stage_cols(groups.size(), 0), stage_nq(groups.size(), 0);
-    size_t ws_off = ident_bytes;
+    size_t ws_off = 0;
     for (size_t t = 0; t < count; ++t) {
         ttm_chain_desc_t const &d = descs[t];
         char *buf[2] = {sbase + ws_off, sbase + ws_off + ws_of[t]};
@@ -335,7 +322,7 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
             P.a = in;
             P.nq = cur[mode];
             P.m = d.u[mode] ? d.r[mode] : d.n[mode];
-            P.b = d.u[mode] ? d.u[mode] : ident_ptr[d.n[mode]];
+            P.b = d.u[mode]; // null: the copy stage
             uint64_t ncols = 1;
             for (unsigned r = 0; r < d.p; ++r)
                 if (r != mode) ncols *= cur[r];
@@ -347,7 +334,8 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
             bool const b_al = (reinterpret_cast<uintptr_t>(P.b) & 15u) == 0;
             P.flags = ((a_al && P.nq % vec == 0) ? TILE_A_VEC : 0u) | ((c_al && ncols % vec == 0) ? TILE_C_VEC : 0u) |
                       (b_al ? TILE_B_VEC : 0u);
-            size_t const g = (size_t)s * classes + m_class(P.m);
+            size_t const g = (size_t)s * groups_per_stage + (d.u[mode] ? m_class(P.m) : classes);
+            groups[g].transpose = !d.u[mode];
             groups[g].probs.push_back(P);
             stage_max_m[g] = std::max(stage_max_m[g], P.m);
             stage_cols[g] += ncols;
@@ -362,6 +350,10 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
         for (size_t g = 0; g < groups.size(); ++g) {
             Stage &st = groups[g];
             if (st.probs.empty()) continue;
+            if (st.transpose) {
+                plan->stages.push_back(std::move(st));
+                continue;
+            }
             std::ostringstream key;
             key << (is_double ? "f64" : "f32") << "/p" << (int)precision << "/m" << log2_class(stage_max_m[g])
                 << "/nq" << log2_class(stage_nq[g]) << "/cols" << log2_class(stage_cols[g]) << "/n"
@@ -444,6 +436,7 @@ ttm_status_t ttm_chain_tune(ttm_chain_plan_t plan, ttm_stream_t stream)
     for (size_t si = 0; si < plan->stages.size() && !r; ++si) {
         float best_ms = 1e30f;
         int best = -1;
+        if (plan->stages[si].transpose) continue; // one copy kernel
         for (int ci = 0; ci < group_config_count() && !r; ++ci) {
             GroupConfig const cfg = group_config(ci);
             if (!config_allowed(cfg, plan->is_double)) continue;
@@ -484,8 +477,10 @@ unsigned ttm_chain_stages(ttm_chain_plan_t plan) { return plan ? (unsigned)plan-
 const char *ttm_chain_stage_kernel(ttm_chain_plan_t plan, unsigned s)
 {
     if (!plan || s >= plan->stages.size()) return "";
-    plan->symbol = lowp_group_precision(plan->prec) ? lowp_group_symbol(plan->prec, plan->stages[s].cfg.bm())
-                                                 : group_symbol(plan->stages[s].cfg, plan->is_double);
+    plan->symbol = plan->stages[s].transpose          ? (plan->is_double ? "k_transpose_grouped<double>"
+                                                                         : "k_transpose_grouped<float>")
+                   : lowp_group_precision(plan->prec) ? lowp_group_symbol(plan->prec, plan->stages[s].cfg.bm())
+                                                      : group_symbol(plan->stages[s].cfg, plan->is_double);
     return plan->symbol.c_str();
 }
 

@@ -1030,6 +1030,58 @@ template ttm_status_t launch_gemm<double>(const Fold &, const double *, const do
                                           unsigned);
 
 // ---- grouped launches -------------------------------------------------------
+// A chain stage over a mode without a factor: the layout rotation that the
+// product with the identity would do, as a copy, C[l * ncols + n] =
+// A[n * nq + l] (an ncols x nq matrix transposed).  Persistent like the
+// grouped products: workgroup g takes 64 x 64 tiles g, g + gridDim.x, ...
+// of the concatenated tile lists (nj = ceil(nq / 64) l-blocks per column
+// block, l-blocks fastest).  A tile goes through LDS so that the reads (along
+// l) and the writes (along n) are both contiguous across the lanes.
+template <class T>
+__global__ __launch_bounds__(256) void k_transpose_grouped(const GroupProblem *__restrict__ probs,
+                                                        const unsigned *__restrict__ tile_prob, unsigned total)
+{
+    __shared__ T tile[64][65];
+    int const tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
+        GroupProblem const P = probs[tile_prob[t]];
+        unsigned const local = t - P.tile_begin;
+        uint64_t const l0 = (uint64_t)(local % P.nj) * 64, n0 = (uint64_t)(local / P.nj) * 64;
+        const T *a = static_cast<const T *>(P.a);
+        T *c = static_cast<T *>(P.c);
+        __syncthreads(); // every read of the previous tile is done
+#pragma unroll 4
+        for (int r = ty; r < 64; r += 4) {
+            uint64_t const n = n0 + r, l = l0 + tx;
+            if (n < P.ncols && l < P.nq) tile[r][tx] = a[n * P.nq + l];
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int r = ty; r < 64; r += 4) {
+            uint64_t const l = l0 + r, n = n0 + tx;
+            if (l < P.nq && n < P.ncols) c[l * P.ncols + n] = tile[tx][r];
+        }
+    }
+}
+
+template <class T>
+ttm_status_t launch_transpose_grouped(const GroupProblem *d_probs, const unsigned *d_tile_prob, unsigned total,
+                                      unsigned grid, hipStream_t s)
+{
+    if (total == 0) return TTM_SUCCESS;
+    hipLaunchKernelGGL(k_transpose_grouped<T>, dim3(grid), dim3(256), 0, s, d_probs, d_tile_prob, total);
+    hipError_t const e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(TTM_ERR_HIP, hipGetErrorString(e));
+        return TTM_ERR_HIP;
+    }
+    return TTM_SUCCESS;
+}
+template ttm_status_t launch_transpose_grouped<float>(const GroupProblem *, const unsigned *, unsigned, unsigned,
+                                                      hipStream_t);
+template ttm_status_t launch_transpose_grouped<double>(const GroupProblem *, const unsigned *, unsigned, unsigned,
+                                                       hipStream_t);
+
 namespace
 {
 // (jt, nt, waves_m, bk): rows 16/32/64 with 256 columns, 128 x 128, and 64 or 32

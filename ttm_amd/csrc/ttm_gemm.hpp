@@ -86,6 +86,12 @@ struct GroupConfig
 };
 
 // The configurations the grouped launcher is compiled for.
+// Chain stages over modes without a factor: the TT view's layout rotation
+// as a copy, C[l * ncols + n] = A[n * nq + l], in 64 x 64 tiles (nj =
+// ceil(nq / 64) per column block); `grid` persistent workgroups.
+template <class T>
+ttm_status_t launch_transpose_grouped(const GroupProblem *d_probs, const unsigned *d_tile_prob, unsigned total,
+                                      unsigned grid, hipStream_t s);
 int group_config_count();
 GroupConfig group_config(int index);
 std::string group_symbol(const GroupConfig &cfg, bool is_double);
