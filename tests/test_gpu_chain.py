@@ -131,6 +131,70 @@ def test_modes_without_factor_are_copies(gpu):
         plan.close()
 
 
+def _short_row_items(torch, dev, rng, dtype):
+    """Chains whose stages the short-row kernel can take (contractions
+    4..256, ranks 1..64): orders 2 and 3, every layout, contraction lengths
+    at 4, 5, 15..17, 255, 256 and random, rank classes 16 / 32 / 64 at their
+    edges, single columns, and 200 small tensors (more problems than
+    workgroups, so workgroup ranges cross many problem boundaries)."""
+    specs = [([4, 256], 1), ([256, 5], 64), ([17, 15, 16], 33), ([255, 7, 9], 16), ([64, 65, 4], 17),
+             ([16, 31, 33], 32), ([200, 130], 48), ([4, 4, 4], 4)]
+    for _ in range(200):
+        p = int(rng.integers(2, 4))
+        specs.append(([int(x) for x in rng.integers(4, 41 if p == 3 else 257, size=p)], int(rng.integers(1, 65))))
+    items, refs = [], []
+    for na, rmax in specs:
+        p = len(na)
+        pi = [int(x) + 1 for x in rng.permutation(p)]
+        A = rng.uniform(-1, 1, size=na)
+        Us = [rng.uniform(-1, 1, size=(int(rng.integers(1, rmax + 1)) if k else rmax, n)) for k, n in enumerate(na)]
+        nc = [U.shape[0] for U in Us]
+        At = to_layout(torch, dev, A, pi, dtype)
+        Ut = [torch.from_numpy(U).to(dev, dtype).contiguous() for U in Us]
+        Ct = to_layout(torch, dev, np.full(nc, np.nan), pi, dtype)
+        items.append((At, Ut, Ct, pi))
+        refs.append((A, Us, einsum_chain(A, Us)))
+    return items, refs
+
+
+def test_short_row_kernel_matches_the_definition(gpu, monkeypatch):
+    """Stages with contractions of 4..256 and ranks up to 64 run the
+    short-row kernel (k_rows_grouped) by default in float32; with
+    TTM_AMD_CHAIN_ROWS=0 the same plan runs the grouped tiles.  Both are
+    checked against the float64 definition (every output written: C starts
+    as NaN), eagerly and as a graph replay, and against each other."""
+    import torch
+    from ttm_amd.chain import ChainPlan
+    rng = np.random.default_rng(31)
+    items, refs = _short_row_items(torch, gpu, rng, torch.float32)
+    plan = ChainPlan(items)
+    kernels = plan.stage_kernels
+    _say("short-row stage kernels", kernels)
+    assert sum(k.startswith("k_rows_grouped<") for k in kernels) >= 3
+    plan.run()
+    check(items, refs, torch.float32, "short rows")
+    got_rows = [it[2].clone() for it in items]
+    for it in items:
+        it[2].fill_(float("nan"))
+    plan.run_graph()
+    check(items, refs, torch.float32, "short rows graph")
+    plan.close()
+
+    monkeypatch.setenv("TTM_AMD_CHAIN_ROWS", "0")
+    for it in items:
+        it[2].fill_(float("nan"))
+    plan = ChainPlan(items)
+    assert not any(k.startswith("k_rows_grouped<") for k in plan.stage_kernels)
+    plan.run()
+    check(items, refs, torch.float32, "grouped tiles")
+    eps = np.finfo(np.float32).eps
+    for a, it, (A, Us, _) in zip(got_rows, items, refs):
+        scale = np.linalg.norm(A) * np.prod([np.linalg.norm(U) for U in Us])
+        diff = float(torch.linalg.vector_norm((a - it[2]).double()))
+        assert diff <= 8 * sum(A.shape) * eps * scale, (A.shape, diff / scale)
+    plan.close()
+
+
 def test_graph_replay_matches_eager(gpu):
     """Replays of the captured chain, with eager work in between, equal the eager run."""
     import torch

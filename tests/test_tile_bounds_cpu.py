@@ -569,3 +569,81 @@ def test_transpose_grouped_tiles_cover_every_element_once():
     for (ncols, nq), a, c, w in zip(probs, srcs, outs, writes):
         assert (w == 1).all()
         assert np.array_equal(c.reshape(nq, ncols), a.reshape(ncols, nq).T)
+
+
+def rows_unit(chk, A, B, nq, m, ncols, u, MB):
+    """k_rows_grouped (ttm_rows.hip), one unit of 16 columns: every global
+    load of A and B and every LDS read of the staged B, and the unit's
+    16 x MB block of C from the model of the lanes' fragments and the
+    16x16x4 MFMA (D[i][c] += sum_q src0(lane i + 16 q) * src1(lane c + 16 q))."""
+    S = (nq + 15) // 16 * 16 + 4
+    ng = (nq + 15) // 16
+    width = 16 * ng
+    total = ncols * nq
+    # B staged: rows past m and columns past nq zero
+    sB = np.full(MB * S, np.nan)
+    for idx in range(MB * width):
+        j, l = idx // width, idx % width
+        assert j * S + l < MB * (256 + 4)
+        if j < m and l < nq:
+            chk("b", j * nq + l)
+            sB[j * S + l] = B[j * nq + l]
+        else:
+            sB[j * S + l] = 0.0
+    col0 = u * 16
+    D = np.zeros((MB, 16))
+    for lane in range(64):
+        lc, lq = lane % 16, lane // 16
+        n = col0 + lc if col0 + lc < ncols else ncols - 1
+        frags = []
+        for g in range(ng - 1):
+            s = n * nq + 4 * lq + 16 * g
+            chk("a", s, 4)
+            assert 16 * g + 4 * lq + 3 < nq  # inside row n
+            frags.append(A[s:s + 4])
+        lim = nq - (16 * (ng - 1) + 4 * lq)
+        e = n * nq + 16 * (ng - 1) + 4 * lq
+        ec = e if e + 4 <= total else total - 4
+        d = e - ec
+        chk("a", ec, 4)
+        rl = A[ec:ec + 4]
+        af = np.zeros(4)
+        for x in range(4):
+            t = rl[x] if d == 0 else (rl[x + d] if x + d < 4 else 0.0)
+            af[x] = t if x < lim else 0.0
+            if x < lim:
+                assert x + d < 4 and ec + x + d == e + x
+        frags.append(af)
+        for g, fr in enumerate(frags):
+            for x in range(4):
+                for i in range(MB):
+                    addr = (i % 16 + 16 * (i // 16)) * S + 4 * lq + 16 * g + x
+                    assert 0 <= addr < MB * S and 16 * g + 4 * lq + x < width
+                    D[i, lc] += sB[addr] * fr[x]
+    for lc in range(16):
+        if col0 + lc < ncols:
+            for j in range(m):
+                chk("c", j * ncols + col0 + lc)
+    return D
+
+
+@pytest.mark.parametrize("MB", [16, 32, 64])
+def test_k_rows_grouped_unit_model(MB):
+    """The short-row kernel's lane arithmetic on the CPU: loads stay inside
+    A (the last group of the last row clamps to A's end and shifts), the
+    staged B covers every fragment read, and the MFMA model of a unit
+    equals B A^T for contraction lengths around every multiple of 4 and 16,
+    the extremes 4 and 256, partial last units and a single column."""
+    rng = np.random.default_rng(MB)
+    cases = [(4, 1), (5, 3), (7, 17), (15, 16), (16, 33), (17, 40), (31, 2), (33, 47), (100, 5), (255, 18),
+             (256, 16), (129, 1)]
+    for nq, ncols in cases:
+        m = int(rng.integers(MB // 2 + 1, MB + 1)) if MB > 16 else int(rng.integers(1, 17))
+        A = rng.standard_normal(ncols * nq)
+        B = rng.standard_normal(m * nq)
+        chk = Bounds({"a": ncols * nq, "b": m * nq, "c": m * ncols})
+        want = B.reshape(m, nq) @ A.reshape(ncols, nq).T
+        for u in range((ncols + 15) // 16):
+            D = rows_unit(chk, A, B, nq, m, ncols, u, MB)
+            cols = min(16, ncols - 16 * u)
+            np.testing.assert_allclose(D[:m, :cols], want[:, 16 * u:16 * u + cols], rtol=1e-12, atol=1e-12)

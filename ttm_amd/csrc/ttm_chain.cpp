@@ -27,6 +27,7 @@
 #include "ttm_gemm.hpp"
 #include "ttm_lowp.hpp"
 #include "ttm_plan.hpp"
+#include "ttm_rows.hpp"
 
 using namespace ttm_amd;
 
@@ -39,12 +40,23 @@ struct Stage
     std::vector<unsigned> tile_prob;
     GroupConfig cfg{4, 4, 1, 16};
     bool transpose = false;            // modes without a factor: a copy, 64 x 64 tiles
+    // float32 stages whose contractions are 4..256 long and whose ranks are
+    // at most 64 may run the short-row kernel (rows_height rows of B)
+    int rows_height = 0;               // 0: not eligible
+    bool rows = false;                 // the short-row kernel is chosen
+    std::vector<RowsProblem> rprobs;
+    std::vector<unsigned> wg_units, wg_prob;
     unsigned total = 0;
     unsigned grid = 0;
     GroupProblem *d_probs = nullptr;   // inside the plan's tile allocation
     unsigned *d_tile_prob = nullptr;
+    RowsProblem *d_rprobs = nullptr;
+    unsigned *d_wg_units = nullptr, *d_wg_prob = nullptr;
     std::string key;                   // shape class for the tuning cache
 };
+
+// Index of the short-row kernel in the tuning cache (after the tile shapes).
+int rows_choice() { return group_config_count(); }
 
 // ---- tuning cache ---------------------------------------------------------
 std::mutex g_tune_mutex;
@@ -63,7 +75,7 @@ void tune_load_locked()
     std::string key;
     int idx;
     while (in >> key >> idx)
-        if (idx >= 0 && idx < group_config_count()) g_tune_cache[key] = idx;
+        if (idx >= 0 && idx <= rows_choice()) g_tune_cache[key] = idx;
 }
 
 void tune_store_locked()
@@ -72,6 +84,13 @@ void tune_store_locked()
     if (!path) return;
     std::ofstream out(path, std::ios::trunc);
     for (auto const &kv : g_tune_cache) out << kv.first << ' ' << kv.second << '\n';
+}
+
+// TTM_AMD_CHAIN_ROWS=0 keeps every stage on the grouped tiles.
+bool rows_enabled()
+{
+    const char *v = std::getenv("TTM_AMD_CHAIN_ROWS");
+    return !(v && v[0] == '0');
 }
 
 int log2_class(uint64_t x)
@@ -138,6 +157,19 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
 {
     size_t bytes = 0;
     for (Stage &st : plan->stages) {
+        if (st.rows) {
+            st.rprobs.clear();
+            for (GroupProblem const &P : st.probs)
+                st.rprobs.push_back(RowsProblem{static_cast<const float *>(P.a), static_cast<const float *>(P.b),
+                                                static_cast<float *>(P.c), P.ncols, (unsigned)P.nq, (unsigned)P.m, 0, 0});
+            st.grid = (unsigned)std::max(plan->num_cu, 1) * rows_wg_per_cu(st.rows_height);
+            rows_layout(st.rprobs, st.rows_height, st.grid, st.wg_units, st.wg_prob);
+            if (st.wg_units.back() >= (1u << 31)) return fail(TTM_ERR_ARGUMENT, "too many columns in one stage");
+            st.total = st.wg_units.back();
+            bytes = align_up(bytes, 256) + st.rprobs.size() * sizeof(RowsProblem);
+            bytes = align_up(bytes, 256) + (st.wg_units.size() + st.wg_prob.size()) * sizeof(unsigned);
+            continue;
+        }
         unsigned const bm = st.transpose ? 64u : (unsigned)st.cfg.bm();
         unsigned const bn = st.transpose ? 64u : (unsigned)st.cfg.bn();
         st.tile_prob.clear();
@@ -188,6 +220,25 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
     size_t off = 0;
     char *base = static_cast<char *>(plan->d_tiles);
     for (Stage &st : plan->stages) {
+        if (st.rows) {
+            off = align_up(off, 256);
+            st.d_rprobs = reinterpret_cast<RowsProblem *>(base + off);
+            ttm_status_t s = hip_check(hipMemcpy(st.d_rprobs, st.rprobs.data(), st.rprobs.size() * sizeof(RowsProblem),
+                                                 hipMemcpyHostToDevice));
+            if (s) return s;
+            off += st.rprobs.size() * sizeof(RowsProblem);
+            off = align_up(off, 256);
+            st.d_wg_units = reinterpret_cast<unsigned *>(base + off);
+            st.d_wg_prob = st.d_wg_units + st.wg_units.size();
+            s = hip_check(hipMemcpy(st.d_wg_units, st.wg_units.data(), st.wg_units.size() * sizeof(unsigned),
+                                    hipMemcpyHostToDevice));
+            if (!s)
+                s = hip_check(hipMemcpy(st.d_wg_prob, st.wg_prob.data(), st.wg_prob.size() * sizeof(unsigned),
+                                        hipMemcpyHostToDevice));
+            if (s) return s;
+            off += (st.wg_units.size() + st.wg_prob.size()) * sizeof(unsigned);
+            continue;
+        }
         off = align_up(off, 256);
         st.d_probs = reinterpret_cast<GroupProblem *>(base + off);
         ttm_status_t s = hip_check(
@@ -207,6 +258,7 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
 ttm_status_t launch_stage(ttm_chain_plan_t plan, const Stage &st, hipStream_t s)
 {
     if (st.total == 0) return TTM_SUCCESS;
+    if (st.rows) return launch_rows_grouped(st.rows_height, st.d_rprobs, st.d_wg_units, st.d_wg_prob, st.grid, s);
     if (st.transpose)
         return plan->is_double ? launch_transpose_grouped<double>(st.d_probs, st.d_tile_prob, st.total, st.grid, s)
                                : launch_transpose_grouped<float>(st.d_probs, st.d_tile_prob, st.total, st.grid, s);
@@ -360,8 +412,21 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
                 << log2_class(st.probs.size());
             st.key = key.str();
             st.cfg = default_config(stage_max_m[g], lowp_group_precision(precision));
+            if (!is_double && precision == TTM_PREC_DEFAULT && rows_enabled()) {
+                bool ok = true;
+                for (GroupProblem const &P : st.probs) ok = ok && rows_eligible(P.nq, P.m, P.ncols);
+                if (ok) st.rows_height = rows_height(stage_max_m[g]);
+            }
+            st.rows = st.rows_height != 0; // the default where it applies
             auto it = g_tune_cache.find(st.key);
-            if (!lowp_group_precision(precision) && it != g_tune_cache.end()) st.cfg = group_config(it->second);
+            if (!lowp_group_precision(precision) && it != g_tune_cache.end()) {
+                if (it->second == rows_choice()) {
+                    st.rows = st.rows_height != 0;
+                } else {
+                    st.rows = false;
+                    st.cfg = group_config(it->second);
+                }
+            }
             plan->stages.push_back(std::move(st));
         }
     }
@@ -437,10 +502,16 @@ ttm_status_t ttm_chain_tune(ttm_chain_plan_t plan, ttm_stream_t stream)
         float best_ms = 1e30f;
         int best = -1;
         if (plan->stages[si].transpose) continue; // one copy kernel
-        for (int ci = 0; ci < group_config_count() && !r; ++ci) {
-            GroupConfig const cfg = group_config(ci);
-            if (!config_allowed(cfg, plan->is_double)) continue;
-            plan->stages[si].cfg = cfg;
+        for (int ci = 0; ci <= rows_choice() && !r; ++ci) {
+            if (ci == rows_choice()) {
+                if (plan->stages[si].rows_height == 0) continue;
+                plan->stages[si].rows = true;
+            } else {
+                GroupConfig const cfg = group_config(ci);
+                if (!config_allowed(cfg, plan->is_double)) continue;
+                plan->stages[si].rows = false;
+                plan->stages[si].cfg = cfg;
+            }
             r = layout_tiles(plan);
             if (r) break;
             Stage const &st = plan->stages[si];
@@ -458,7 +529,8 @@ ttm_status_t ttm_chain_tune(ttm_chain_plan_t plan, ttm_stream_t stream)
             }
         }
         if (!r && best >= 0) {
-            plan->stages[si].cfg = group_config(best);
+            plan->stages[si].rows = best == rows_choice();
+            if (best != rows_choice()) plan->stages[si].cfg = group_config(best);
             std::lock_guard<std::mutex> lock(g_tune_mutex);
             tune_load_locked();
             g_tune_cache[plan->stages[si].key] = best;
@@ -477,7 +549,8 @@ unsigned ttm_chain_stages(ttm_chain_plan_t plan) { return plan ? (unsigned)plan-
 const char *ttm_chain_stage_kernel(ttm_chain_plan_t plan, unsigned s)
 {
     if (!plan || s >= plan->stages.size()) return "";
-    plan->symbol = plan->stages[s].transpose          ? (plan->is_double ? "k_transpose_grouped<double>"
+    plan->symbol = plan->stages[s].rows               ? rows_symbol(plan->stages[s].rows_height)
+                   : plan->stages[s].transpose        ? (plan->is_double ? "k_transpose_grouped<double>"
                                                                          : "k_transpose_grouped<float>")
                    : lowp_group_precision(plan->prec) ? lowp_group_symbol(plan->prec, plan->stages[s].cfg.bm())
                                                       : group_symbol(plan->stages[s].cfg, plan->is_double);

@@ -1,0 +1,241 @@
+// Grouped TT products with short contractions (float32), for the stages of
+// batched multi-TTM chains:
+//   C[j * ncols + n] = sum_{l < nq} B[j * nq + l] * A[n * nq + l],
+// m <= 64 rows of B, 4 <= nq <= 256.
+//
+// A unit is 16 columns (16 rows of A, 16 * nq contiguous floats) and all m
+// rows of C.  One wave computes a unit on the 16x16x4 f32 MFMA with the
+// contraction taken in groups of 16: lane (c, q) = (lane % 16, lane / 16)
+// reads the 16-byte piece A[n0 + c][16 g + 4 q .. 16 g + 4 q + 3] of every
+// group g straight from global memory into registers (element-aligned when
+// nq % 4 != 0: one global_load_dwordx4 either way), and MFMA x of the group
+// takes element x of the piece, so the contraction index of MFMA x at lane
+// quarter q is 16 g + 4 q + x in both operands.  Nothing of A passes through
+// LDS: a wave issues the loads of its whole unit at once (at most 16 KB) and
+// sixteen waves per compute unit (m <= 16; eight for taller B, whose
+// accumulators and fragments need more registers) keep the HBM stream busy
+// between the matrix work.  B (m x nq, rows padded with zeros to the row
+// height and to a multiple of 16 columns) is staged in LDS once per problem
+// with a row stride of 16 k + 4 floats, so the 16-byte fragment reads of the
+// 16 rows of a fragment fall on distinct bank groups.  The last group of a
+// row reads past the row end into the next row (masked to zero), and at the
+// very end of A is read as A's last four elements and shifted into place.
+//
+// Workgroups are persistent over contiguous ranges of the launch's unit list
+// (ranges of nearly equal cost, computed on the host), so a workgroup stages
+// B only when its range crosses into the next problem; its waves take the
+// units of a range round-robin.  The staging is bracketed by barriers that
+// every wave of the workgroup reaches once per problem of the range.
+#include "ttm_rows.hpp"
+#include "ttm_plan.hpp"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace ttm_amd
+{
+namespace
+{
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+// 16-byte vectors at element alignment (a dword-aligned global_load_dwordx4)
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+
+// Global accesses through descriptor pointers: the explicit address space
+// keeps them global (not FLAT) instructions, which do not count against the
+// LDS wait counter.
+template <class V>
+__device__ __forceinline__ V ldg(const void *p)
+{
+    return *(const __attribute__((address_space(1))) V *)p;
+}
+template <class V>
+__device__ __forceinline__ void stg(void *p, V v)
+{
+    *(__attribute__((address_space(1))) V *)p = v;
+}
+
+constexpr int kGroups = (int)(kRowsMaxNq / 16); // contraction groups of 16 per row, at most
+constexpr int kStrideMax = (int)kRowsMaxNq + 4; // LDS row stride of B for nq = 256
+
+// LDS row stride of B for a contraction of nq: 16 * ceil(nq / 16) + 4 (a
+// multiple of 4 whose quarter is odd).
+__host__ __device__ constexpr unsigned rows_stride(unsigned nq) { return (nq + 15) / 16 * 16 + 4; }
+
+template <int MB, int W, int WPE>
+__global__ __launch_bounds__(64 * W, WPE) void k_rows_grouped(const RowsProblem *__restrict__ probs,
+                                                           const unsigned *__restrict__ wg_units,
+                                                           const unsigned *__restrict__ wg_prob)
+{
+    __shared__ __attribute__((aligned(16))) float sB[MB * kStrideMax];
+    constexpr int JT = MB / 16;
+    int const tid = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int const lc = lane & 15, lq = lane >> 4;
+    unsigned const u0 = wg_units[blockIdx.x], u1 = wg_units[blockIdx.x + 1];
+    if (u0 >= u1) return; // the whole workgroup: no barrier is skipped by part of it
+    unsigned p = wg_prob[blockIdx.x];
+    unsigned v = u0 + wave; // this wave's next unit
+    while (true) {
+        RowsProblem const P = probs[p];
+        unsigned const seg_end = P.unit_begin + P.nunits < u1 ? P.unit_begin + P.nunits : u1;
+        unsigned const nq = P.nq;
+        unsigned const S = rows_stride(nq);
+        unsigned const ng = (nq + 15) / 16;
+        unsigned const width = 16 * ng;
+        // B of this problem into LDS: rows past m and columns past nq are zero
+        __syncthreads(); // every wave is done with the previous problem's B
+        for (unsigned idx = tid; idx < MB * width; idx += 64 * W) {
+            unsigned const j = idx / width, l = idx - j * width;
+            sB[j * S + l] = (j < P.m && l < nq) ? ldg<float>(P.b + (uint64_t)j * nq + l) : 0.f;
+        }
+        __syncthreads();
+        uint64_t const total = P.ncols * nq;
+        const float *brow = sB + lc * S + 4 * lq;
+        int const lim = (int)nq - (int)(16 * (ng - 1) + 4 * lq); // valid elements of the last group's piece
+        for (; v < seg_end; v += W) {
+            uint64_t const col0 = (uint64_t)(v - P.unit_begin) * 16u;
+            // columns past the last one read the last one (not stored)
+            uint64_t const n = col0 + lc < P.ncols ? col0 + lc : P.ncols - 1;
+            const float *arow = P.a + n * nq + 4 * lq;
+            f32x4 ra[kGroups];
+#pragma unroll
+            for (int g = 0; g < kGroups; ++g)
+                if (g + 1 < (int)ng) ra[g] = ldg<f32x4u>(arow + 16 * g);
+            // the last group: clamped to A's last four elements (shift d)
+            uint64_t const e = n * nq + 16 * (ng - 1) + 4 * lq;
+            uint64_t const ec = e + 4 <= total ? e : total - 4;
+            int const d = (int)(e - ec);
+            f32x4 const rl = ldg<f32x4u>(P.a + ec);
+
+            f32x4 acc[JT];
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) acc[jt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            auto group = [&](unsigned g, f32x4 af) {
+                f32x4 bf[JT];
+#pragma unroll
+                for (int jt = 0; jt < JT; ++jt)
+                    bf[jt] = *reinterpret_cast<const f32x4 *>(brow + 16 * jt * S + 16 * g);
+#pragma unroll
+                for (int x = 0; x < 4; ++x)
+#pragma unroll
+                    for (int jt = 0; jt < JT; ++jt)
+                        acc[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[jt][x], af[x], acc[jt], 0, 0, 0);
+            };
+#pragma unroll
+            for (int g = 0; g < kGroups; ++g)
+                if (g + 1 < (int)ng) group(g, ra[g]);
+            {
+                // element x of the piece is rl[x + d]; elements past the row
+                // end (x >= lim, which covers everything past A's end) are zero
+                f32x4 af;
+#pragma unroll
+                for (int x = 0; x < 4; ++x) {
+                    float t = rl[x];
+                    if (d == 1) t = x + 1 < 4 ? rl[(x + 1) & 3] : 0.f;
+                    if (d == 2) t = x + 2 < 4 ? rl[(x + 2) & 3] : 0.f;
+                    if (d == 3) t = x + 3 < 4 ? rl[(x + 3) & 3] : 0.f;
+                    af[x] = x < lim ? t : 0.f;
+                }
+                group(ng - 1, af);
+            }
+            // lane (c, q) holds C rows 16 jt + 4 q + r of column col0 + c
+            if (col0 + lc < P.ncols) {
+#pragma unroll
+                for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        unsigned const j = 16 * jt + 4 * lq + r;
+                        if (j < P.m) stg<float>(P.c + (uint64_t)j * P.ncols + col0 + lc, acc[jt][r]);
+                    }
+            }
+        }
+        if (seg_end >= u1) break;
+        ++p;
+    }
+}
+
+// (row height, waves per workgroup, waves per SIMD): one workgroup per
+// compute unit; 16 waves for m <= 16, eight for the taller rows (150 and 206
+// registers).
+constexpr int kW16 = 16, kW32 = 8, kW64 = 8;
+
+} // namespace
+
+int rows_height(uint64_t m) { return m <= 16 ? 16 : m <= 32 ? 32 : m <= 64 ? 64 : 0; }
+
+bool rows_eligible(uint64_t nq, uint64_t m, uint64_t ncols)
+{
+    return nq >= kRowsMinNq && nq <= kRowsMaxNq && m >= 1 && m <= 64 && ncols >= 1;
+}
+
+unsigned rows_wg_per_cu(int) { return 1; }
+
+std::string rows_symbol(int height)
+{
+    char buf[64];
+    int const w = height == 16 ? kW16 : height == 32 ? kW32 : kW64;
+    int const wpe = height == 16 ? 4 : 2;
+    std::snprintf(buf, sizeof buf, "k_rows_grouped<%d, %d, %d>", height, w, wpe);
+    return buf;
+}
+
+void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, std::vector<unsigned> &wg_units,
+                 std::vector<unsigned> &wg_prob)
+{
+    // cost of a unit in bytes: HBM traffic or the padded matrix work at the
+    // f32 MFMA ridge (157.3 TFLOP/s / 8 TB/s = 19.7 flop per byte), whichever
+    // is larger, plus a fixed 2 KB for its instruction stream
+    std::vector<double> unit_cost;
+    unsigned begin = 0;
+    for (RowsProblem &P : probs) {
+        P.unit_begin = begin;
+        P.nunits = (unsigned)((P.ncols + 15) / 16);
+        begin += P.nunits;
+        double const padded = 16.0 * ((P.nq + 15) / 16);
+        for (unsigned u = 0; u < P.nunits; ++u) {
+            double const cols = (double)std::min<uint64_t>(16, P.ncols - (uint64_t)u * 16);
+            double const bytes = cols * 4.0 * (P.nq + P.m);
+            double const flops = 2.0 * 16.0 * padded * height;
+            unit_cost.push_back(std::max(bytes, flops / 19.66) + 2048.0);
+        }
+    }
+    double total = 0;
+    for (double c : unit_cost) total += c;
+    wg_units.assign(grid + 1, begin);
+    wg_prob.assign(grid, 0);
+    double acc = 0;
+    unsigned u = 0;
+    for (unsigned b = 0; b < grid; ++b) {
+        wg_units[b] = u;
+        double const target = total * (b + 1) / grid;
+        while (u < begin && acc + 0.5 * unit_cost[u] <= target) acc += unit_cost[u++];
+    }
+    wg_units[grid] = begin;
+    unsigned pi = 0;
+    for (unsigned b = 0; b < grid; ++b) {
+        while (pi + 1 < probs.size() && wg_units[b] >= probs[pi].unit_begin + probs[pi].nunits) ++pi;
+        wg_prob[b] = pi;
+    }
+}
+
+ttm_status_t launch_rows_grouped(int height, const RowsProblem *d_probs, const unsigned *d_wg_units,
+                                 const unsigned *d_wg_prob, unsigned grid, hipStream_t s)
+{
+    if (grid == 0) return TTM_SUCCESS;
+    switch (height) {
+    case 16: hipLaunchKernelGGL((k_rows_grouped<16, kW16, 4>), dim3(grid), dim3(64 * kW16), 0, s, d_probs, d_wg_units, d_wg_prob); break;
+    case 32: hipLaunchKernelGGL((k_rows_grouped<32, kW32, 2>), dim3(grid), dim3(64 * kW32), 0, s, d_probs, d_wg_units, d_wg_prob); break;
+    case 64: hipLaunchKernelGGL((k_rows_grouped<64, kW64, 2>), dim3(grid), dim3(64 * kW64), 0, s, d_probs, d_wg_units, d_wg_prob); break;
+    default: set_error(TTM_ERR_ARGUMENT, "short-row kernel: row height must be 16, 32 or 64"); return TTM_ERR_ARGUMENT;
+    }
+    hipError_t const e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(TTM_ERR_HIP, hipGetErrorString(e));
+        return TTM_ERR_HIP;
+    }
+    return TTM_SUCCESS;
+}
+
+} // namespace ttm_amd

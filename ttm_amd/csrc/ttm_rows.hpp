@@ -1,0 +1,57 @@
+// Grouped TT products with short contractions (float32): the kernel for
+// multi-TTM chain stages whose contracted extent is at most 256.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ttm_amd.h"
+
+namespace ttm_amd
+{
+
+// One product of a short-row launch, in the TT view:
+//   C[j * ncols + n] = sum_{l < nq} B[j * nq + l] * A[n * nq + l]
+// Its columns are cut into units of 16 (unit u covers columns 16 u ..
+// 16 u + 15); unit_begin is the problem's first unit in the launch's list.
+struct RowsProblem
+{
+    const float *a;
+    const float *b;
+    float *c;
+    uint64_t ncols;
+    unsigned nq;
+    unsigned m;
+    unsigned unit_begin;
+    unsigned nunits;
+};
+
+constexpr unsigned kRowsMinNq = 4;
+constexpr unsigned kRowsMaxNq = 256;
+
+// Row heights the kernel is compiled for (16, 32, 64); 0 when m > 64.
+int rows_height(uint64_t m);
+
+// A problem the kernel can run: 4 <= nq <= 256, 1 <= m <= 64.
+bool rows_eligible(uint64_t nq, uint64_t m, uint64_t ncols);
+
+// Workgroups per compute unit of the kernel for a row height.
+unsigned rows_wg_per_cu(int height);
+
+std::string rows_symbol(int height);
+
+// Units of the problems (unit_begin / nunits filled, in order) and the
+// launch's partition: workgroup b runs units [wg_units[b], wg_units[b + 1]),
+// which start inside problem wg_prob[b].  The ranges are contiguous and of
+// nearly equal estimated cost (HBM bytes or padded matrix work, whichever is
+// larger, plus a fixed cost per unit).
+void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, std::vector<unsigned> &wg_units,
+                 std::vector<unsigned> &wg_prob);
+
+ttm_status_t launch_rows_grouped(int height, const RowsProblem *d_probs, const unsigned *d_wg_units,
+                                 const unsigned *d_wg_prob, unsigned grid, hipStream_t s);
+
+} // namespace ttm_amd
