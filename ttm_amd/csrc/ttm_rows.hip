@@ -84,11 +84,29 @@ __global__ __launch_bounds__(64 * W, WPE) void k_rows_grouped(const RowsProblem 
         unsigned const S = rows_stride(nq);
         unsigned const ng = (nq + 15) / 16;
         unsigned const width = 16 * ng;
-        // B of this problem into LDS: rows past m and columns past nq are zero
+        // B of this problem into LDS: rows past m and columns past nq are
+        // zero.  Wave w stages rows w, w + W, ...; a lane's loads are all
+        // issued before its stores (one round trip per problem).
         __syncthreads(); // every wave is done with the previous problem's B
-        for (unsigned idx = tid; idx < MB * width; idx += 64 * W) {
-            unsigned const j = idx / width, l = idx - j * width;
-            sB[j * S + l] = (j < P.m && l < nq) ? ldg<float>(P.b + (uint64_t)j * nq + l) : 0.f;
+        {
+            static_assert(MB % W == 0, "whole rows per wave");
+            constexpr int RPW = MB / W;            // rows per wave
+            constexpr int CPL = kStrideMax / 64;   // columns per lane, at most
+            float t[RPW][CPL];
+#pragma unroll
+            for (int r = 0; r < RPW; ++r)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    unsigned const j = wave + W * r, l = lane + 64 * c;
+                    t[r][c] = (j < P.m && l < nq) ? ldg<float>(P.b + (uint64_t)j * nq + l) : 0.f;
+                }
+#pragma unroll
+            for (int r = 0; r < RPW; ++r)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    unsigned const j = wave + W * r, l = lane + 64 * c;
+                    if (l < width) sB[j * S + l] = t[r][c];
+                }
         }
         __syncthreads();
         uint64_t const total = P.ncols * nq;
