@@ -157,36 +157,39 @@ def _short_row_items(torch, dev, rng, dtype):
     return items, refs
 
 
-def test_short_row_kernel_matches_the_definition(gpu, monkeypatch):
+@pytest.mark.parametrize("prec", [None, "bf16x3"])
+def test_short_row_kernel_matches_the_definition(gpu, monkeypatch, prec):
     """Stages with contractions of 4..256 and ranks up to 64 run the
-    short-row kernel (k_rows_grouped) by default in float32; with
-    TTM_AMD_CHAIN_ROWS=0 the same plan runs the grouped tiles.  Both are
-    checked against the float64 definition (every output written: C starts
-    as NaN), eagerly and as a graph replay, and against each other."""
+    short-row kernels (k_rows_grouped, k_rows_bf16x3) by default in float32
+    and BF16X3; with TTM_AMD_CHAIN_ROWS=0 the same plan runs the grouped
+    tiles.  Both are checked against the float64 definition (every output
+    written: C starts as NaN), eagerly and as a graph replay, and against
+    each other."""
     import torch
     from ttm_amd.chain import ChainPlan
     rng = np.random.default_rng(31)
     items, refs = _short_row_items(torch, gpu, rng, torch.float32)
-    plan = ChainPlan(items)
+    name = "k_rows_bf16x3<" if prec else "k_rows_grouped<"
+    plan = ChainPlan(items, precision=prec)
     kernels = plan.stage_kernels
     _say("short-row stage kernels", kernels)
-    assert sum(k.startswith("k_rows_grouped<") for k in kernels) >= 3
+    assert sum(k.startswith(name) for k in kernels) >= 3
     plan.run()
-    check(items, refs, torch.float32, "short rows")
+    check(items, refs, torch.float32, "short rows", prec=prec)
     got_rows = [it[2].clone() for it in items]
     for it in items:
         it[2].fill_(float("nan"))
     plan.run_graph()
-    check(items, refs, torch.float32, "short rows graph")
+    check(items, refs, torch.float32, "short rows graph", prec=prec)
     plan.close()
 
     monkeypatch.setenv("TTM_AMD_CHAIN_ROWS", "0")
     for it in items:
         it[2].fill_(float("nan"))
-    plan = ChainPlan(items)
-    assert not any(k.startswith("k_rows_grouped<") for k in plan.stage_kernels)
+    plan = ChainPlan(items, precision=prec)
+    assert not any(k.startswith("k_rows_") for k in plan.stage_kernels)
     plan.run()
-    check(items, refs, torch.float32, "grouped tiles")
+    check(items, refs, torch.float32, "grouped tiles", prec=prec)
     eps = np.finfo(np.float32).eps
     for a, it, (A, Us, _) in zip(got_rows, items, refs):
         scale = np.linalg.norm(A) * np.prod([np.linalg.norm(U) for U in Us])

@@ -163,7 +163,7 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
                 st.rprobs.push_back(RowsProblem{static_cast<const float *>(P.a), static_cast<const float *>(P.b),
                                                 static_cast<float *>(P.c), P.ncols, (unsigned)P.nq, (unsigned)P.m, 0, 0});
             st.grid = (unsigned)std::max(plan->num_cu, 1) * rows_wg_per_cu(st.rows_height);
-            rows_layout(st.rprobs, st.rows_height, st.grid, st.wg_units, st.wg_prob);
+            rows_layout(st.rprobs, st.rows_height, st.grid, st.wg_units, st.wg_prob, plan->prec == TTM_PREC_BF16X3);
             if (st.wg_units.back() >= (1u << 31)) return fail(TTM_ERR_ARGUMENT, "too many columns in one stage");
             st.total = st.wg_units.back();
             bytes = align_up(bytes, 256) + st.rprobs.size() * sizeof(RowsProblem);
@@ -258,7 +258,9 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
 ttm_status_t launch_stage(ttm_chain_plan_t plan, const Stage &st, hipStream_t s)
 {
     if (st.total == 0) return TTM_SUCCESS;
-    if (st.rows) return launch_rows_grouped(st.rows_height, st.d_rprobs, st.d_wg_units, st.d_wg_prob, st.grid, s);
+    if (st.rows)
+        return launch_rows_grouped(st.rows_height, st.d_rprobs, st.d_wg_units, st.d_wg_prob, st.grid, s,
+                                   plan->prec == TTM_PREC_BF16X3);
     if (st.transpose)
         return plan->is_double ? launch_transpose_grouped<double>(st.d_probs, st.d_tile_prob, st.total, st.grid, s)
                                : launch_transpose_grouped<float>(st.d_probs, st.d_tile_prob, st.total, st.grid, s);
@@ -352,7 +354,9 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
     // larger) so that each launch gets a tile height that fits its ranks,
     // plus one copy launch for the modes without a factor; launches of one
     // stage are independent, stages run in order.
-    int const classes = lowp_group_precision(precision) ? 2 : 4;
+    // (FP16X2 keeps the two tile heights of its grouped kernel; BF16X3 and
+    // the plain precisions take the short-row kernels' classes)
+    int const classes = precision == TTM_PREC_FP16X2 || (precision == TTM_PREC_BF16X3 && !rows_enabled()) ? 2 : 4;
     int const groups_per_stage = classes + 1;
     auto m_class = [&](uint64_t m) {
         if (classes == 2) return m <= 64 ? 0 : 1; // the two reduced-precision tile heights
@@ -412,7 +416,7 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
                 << log2_class(st.probs.size());
             st.key = key.str();
             st.cfg = default_config(stage_max_m[g], lowp_group_precision(precision));
-            if (!is_double && precision == TTM_PREC_DEFAULT && rows_enabled()) {
+            if (!is_double && (precision == TTM_PREC_DEFAULT || precision == TTM_PREC_BF16X3) && rows_enabled()) {
                 bool ok = true;
                 for (GroupProblem const &P : st.probs) ok = ok && rows_eligible(P.nq, P.m, P.ncols);
                 if (ok) st.rows_height = rows_height(stage_max_m[g]);
@@ -549,7 +553,7 @@ unsigned ttm_chain_stages(ttm_chain_plan_t plan) { return plan ? (unsigned)plan-
 const char *ttm_chain_stage_kernel(ttm_chain_plan_t plan, unsigned s)
 {
     if (!plan || s >= plan->stages.size()) return "";
-    plan->symbol = plan->stages[s].rows               ? rows_symbol(plan->stages[s].rows_height)
+    plan->symbol = plan->stages[s].rows               ? rows_symbol(plan->stages[s].rows_height, plan->prec == TTM_PREC_BF16X3)
                    : plan->stages[s].transpose        ? (plan->is_double ? "k_transpose_grouped<double>"
                                                                          : "k_transpose_grouped<float>")
                    : lowp_group_precision(plan->prec) ? lowp_group_symbol(plan->prec, plan->stages[s].cfg.bm())

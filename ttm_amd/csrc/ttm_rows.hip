@@ -174,6 +174,179 @@ __global__ __launch_bounds__(64 * W, WPE) void k_rows_grouped(const RowsProblem 
     }
 }
 
+// ---------------------------------------------------------------------------
+// BF16X3: the same units with A and B split into three bfloat16 terms each
+// (x = x0 + x1 + x2 + O(2^-24 |x|), every difference exact in float32) and
+// the six largest cross products on the 16x16x32 bf16 MFMA, smallest first:
+// (1,1), (0,2), (2,0), (0,1), (1,0), (0,0) (A term, B term).  A group is 32
+// contraction indices: lane (c, q) reads A[n0 + c][32 g + 8 q .. + 7] as two
+// 16-byte pieces and splits them in registers; MFMA operand element e of
+// lane quarter q is contraction index 32 g + 8 q + e for both operands.  B's
+// three planes (bf16, row stride 32 k + 8) are split once per problem when
+// they are staged.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+constexpr int kGroups32 = (int)(kRowsMaxNq / 32);
+constexpr int kStrideMaxH = (int)kRowsMaxNq + 8; // bf16 elements
+__host__ __device__ constexpr unsigned rows_stride_h(unsigned nq) { return (nq + 31) / 32 * 32 + 8; }
+
+__device__ __forceinline__ void split3(f32x2 x, bf16x2 &p0, bf16x2 &p1, bf16x2 &p2)
+{
+    p0 = __builtin_convertvector(x, bf16x2);
+    f32x2 const r1 = x - __builtin_convertvector(p0, f32x2);
+    p1 = __builtin_convertvector(r1, bf16x2);
+    f32x2 const r2 = r1 - __builtin_convertvector(p1, f32x2);
+    p2 = __builtin_convertvector(r2, bf16x2);
+}
+
+// The three terms of eight floats (two 16-byte pieces) as bf16x8 operands.
+__device__ __forceinline__ void split3x8(f32x4 lo, f32x4 hi, bf16x8 (&t)[3])
+{
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        f32x2 const x = h < 2 ? f32x2{lo[2 * h], lo[2 * h + 1]} : f32x2{hi[2 * h - 4], hi[2 * h - 3]};
+        bf16x2 p0, p1, p2;
+        split3(x, p0, p1, p2);
+        t[0][2 * h] = p0[0];
+        t[0][2 * h + 1] = p0[1];
+        t[1][2 * h] = p1[0];
+        t[1][2 * h + 1] = p1[1];
+        t[2][2 * h] = p2[0];
+        t[2][2 * h + 1] = p2[1];
+    }
+}
+
+// Element x of the piece that starts at element e of A (clamped to A's last
+// four elements, shift d): elements past the row end (x >= lim) are zero.
+__device__ __forceinline__ f32x4 piece_last(f32x4 rl, int d, int lim)
+{
+    f32x4 af;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+        float t = rl[x];
+        if (d == 1) t = x + 1 < 4 ? rl[(x + 1) & 3] : 0.f;
+        if (d == 2) t = x + 2 < 4 ? rl[(x + 2) & 3] : 0.f;
+        if (d == 3) t = x + 3 < 4 ? rl[(x + 3) & 3] : 0.f;
+        af[x] = x < lim ? t : 0.f;
+    }
+    return af;
+}
+
+template <int MB, int W, int WPE>
+__global__ __launch_bounds__(64 * W, WPE) void k_rows_bf16x3(const RowsProblem *__restrict__ probs,
+                                                          const unsigned *__restrict__ wg_units,
+                                                          const unsigned *__restrict__ wg_prob)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 sB[3][MB * kStrideMaxH];
+    constexpr int JT = MB / 16;
+    int const tid = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int const lc = lane & 15, lq = lane >> 4;
+    unsigned const u0 = wg_units[blockIdx.x], u1 = wg_units[blockIdx.x + 1];
+    if (u0 >= u1) return;
+    unsigned p = wg_prob[blockIdx.x];
+    unsigned v = u0 + wave;
+    while (true) {
+        RowsProblem const P = probs[p];
+        unsigned const seg_end = P.unit_begin + P.nunits < u1 ? P.unit_begin + P.nunits : u1;
+        unsigned const nq = P.nq;
+        unsigned const S = rows_stride_h(nq);
+        unsigned const ng = (nq + 31) / 32;
+        unsigned const width = 32 * ng;
+        __syncthreads();
+        {
+            // rows w, w + W, ...; a lane takes pairs of columns 2 (lane + 64 c)
+            static_assert(MB % W == 0, "whole rows per wave");
+            constexpr int RPW = MB / W;
+            constexpr int CPL = (kRowsMaxNq / 2 + 63) / 64; // column pairs per lane, at most
+            f32x2 t[RPW][CPL];
+#pragma unroll
+            for (int r = 0; r < RPW; ++r)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    unsigned const j = wave + W * r, l = 2 * (lane + 64 * c);
+                    const float *bj = P.b + (uint64_t)j * nq;
+                    t[r][c][0] = (j < P.m && l < nq) ? ldg<float>(bj + l) : 0.f;
+                    t[r][c][1] = (j < P.m && l + 1 < nq) ? ldg<float>(bj + l + 1) : 0.f;
+                }
+#pragma unroll
+            for (int r = 0; r < RPW; ++r)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    unsigned const j = wave + W * r, l = 2 * (lane + 64 * c);
+                    if (l < width) {
+                        bf16x2 p0, p1, p2;
+                        split3(t[r][c], p0, p1, p2);
+                        *reinterpret_cast<bf16x2 *>(&sB[0][j * S + l]) = p0;
+                        *reinterpret_cast<bf16x2 *>(&sB[1][j * S + l]) = p1;
+                        *reinterpret_cast<bf16x2 *>(&sB[2][j * S + l]) = p2;
+                    }
+                }
+        }
+        __syncthreads();
+        uint64_t const total = P.ncols * nq;
+        unsigned const boff = lc * S + 8 * lq;
+        int const lim0 = (int)nq - (int)(32 * (ng - 1) + 8 * lq); // valid elements of the last group's first piece
+        for (; v < seg_end; v += W) {
+            uint64_t const col0 = (uint64_t)(v - P.unit_begin) * 16u;
+            uint64_t const n = col0 + lc < P.ncols ? col0 + lc : P.ncols - 1;
+            const float *arow = P.a + n * nq + 8 * lq;
+            f32x4 ra[kGroups32][2];
+#pragma unroll
+            for (int g = 0; g < kGroups32; ++g)
+                if (g + 1 < (int)ng) {
+                    ra[g][0] = ldg<f32x4u>(arow + 32 * g);
+                    ra[g][1] = ldg<f32x4u>(arow + 32 * g + 4);
+                }
+            uint64_t const e0 = n * nq + 32 * (ng - 1) + 8 * lq, e1 = e0 + 4;
+            uint64_t const ec0 = e0 + 4 <= total ? e0 : total - 4, ec1 = e1 + 4 <= total ? e1 : total - 4;
+            int const d0 = (int)(e0 - ec0), d1 = (int)(e1 - ec1);
+            f32x4 const rl0 = ldg<f32x4u>(P.a + ec0), rl1 = ldg<f32x4u>(P.a + ec1);
+
+            f32x4 acc[JT];
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) acc[jt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            auto group = [&](unsigned g, f32x4 lo, f32x4 hi) {
+                bf16x8 ta[3];
+                split3x8(lo, hi, ta);
+#pragma unroll
+                for (int jt = 0; jt < JT; ++jt) {
+                    unsigned const o = boff + 16 * jt * S + 32 * g;
+                    bf16x8 const b0 = *reinterpret_cast<const bf16x8 *>(&sB[0][o]);
+                    bf16x8 const b1 = *reinterpret_cast<const bf16x8 *>(&sB[1][o]);
+                    bf16x8 const b2 = *reinterpret_cast<const bf16x8 *>(&sB[2][o]);
+                    f32x4 a = acc[jt];
+                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b1, ta[1], a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b2, ta[0], a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0, ta[2], a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b1, ta[0], a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0, ta[1], a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0, ta[0], a, 0, 0, 0);
+                    acc[jt] = a;
+                }
+            };
+#pragma unroll
+            for (int g = 0; g < kGroups32; ++g)
+                if (g + 1 < (int)ng) group(g, ra[g][0], ra[g][1]);
+            group(ng - 1, piece_last(rl0, d0, lim0), piece_last(rl1, d1, lim0 - 4));
+            if (col0 + lc < P.ncols) {
+#pragma unroll
+                for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        unsigned const j = 16 * jt + 4 * lq + r;
+                        if (j < P.m) stg<float>(P.c + (uint64_t)j * P.ncols + col0 + lc, acc[jt][r]);
+                    }
+            }
+        }
+        if (seg_end >= u1) break;
+        ++p;
+    }
+}
+
 // (row height, waves per workgroup, waves per SIMD): one workgroup per
 // compute unit; 16 waves for m <= 16, eight for the taller rows (150 and 206
 // registers).
@@ -190,33 +363,37 @@ bool rows_eligible(uint64_t nq, uint64_t m, uint64_t ncols)
 
 unsigned rows_wg_per_cu(int) { return 1; }
 
-std::string rows_symbol(int height)
+std::string rows_symbol(int height, bool bf16x3)
 {
     char buf[64];
-    int const w = height == 16 ? kW16 : height == 32 ? kW32 : kW64;
-    int const wpe = height == 16 ? 4 : 2;
-    std::snprintf(buf, sizeof buf, "k_rows_grouped<%d, %d, %d>", height, w, wpe);
+    int const w = bf16x3 ? 8 : height == 16 ? kW16 : height == 32 ? kW32 : kW64;
+    int const wpe = !bf16x3 && height == 16 ? 4 : 2;
+    std::snprintf(buf, sizeof buf, "%s<%d, %d, %d>", bf16x3 ? "k_rows_bf16x3" : "k_rows_grouped", height, w, wpe);
     return buf;
 }
 
 void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, std::vector<unsigned> &wg_units,
-                 std::vector<unsigned> &wg_prob)
+                 std::vector<unsigned> &wg_prob, bool bf16x3)
 {
     // cost of a unit in bytes: HBM traffic or the padded matrix work at the
     // f32 MFMA ridge (157.3 TFLOP/s / 8 TB/s = 19.7 flop per byte), whichever
-    // is larger, plus a fixed 2 KB for its instruction stream
+    // is larger, plus a fixed 2 KB for its instruction stream.  BF16X3 takes
+    // six 16x16x32 bf16 products (16 cycles each) per 32 contraction indices
+    // where float32 takes eight 16x16x4 products (32 cycles each): 0.375 of
+    // the matrix time.
+    double const mfma_scale = bf16x3 ? 0.375 : 1.0;
     std::vector<double> unit_cost;
     unsigned begin = 0;
     for (RowsProblem &P : probs) {
         P.unit_begin = begin;
         P.nunits = (unsigned)((P.ncols + 15) / 16);
         begin += P.nunits;
-        double const padded = 16.0 * ((P.nq + 15) / 16);
+        double const padded = bf16x3 ? 32.0 * ((P.nq + 31) / 32) : 16.0 * ((P.nq + 15) / 16);
         for (unsigned u = 0; u < P.nunits; ++u) {
             double const cols = (double)std::min<uint64_t>(16, P.ncols - (uint64_t)u * 16);
             double const bytes = cols * 4.0 * (P.nq + P.m);
             double const flops = 2.0 * 16.0 * padded * height;
-            unit_cost.push_back(std::max(bytes, flops / 19.66) + 2048.0);
+            unit_cost.push_back(std::max(bytes, mfma_scale * flops / 19.66) + 2048.0);
         }
     }
     double total = 0;
@@ -239,13 +416,16 @@ void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, std
 }
 
 ttm_status_t launch_rows_grouped(int height, const RowsProblem *d_probs, const unsigned *d_wg_units,
-                                 const unsigned *d_wg_prob, unsigned grid, hipStream_t s)
+                                 const unsigned *d_wg_prob, unsigned grid, hipStream_t s, bool bf16x3)
 {
     if (grid == 0) return TTM_SUCCESS;
-    switch (height) {
+    switch (height * (bf16x3 ? -1 : 1)) {
     case 16: hipLaunchKernelGGL((k_rows_grouped<16, kW16, 4>), dim3(grid), dim3(64 * kW16), 0, s, d_probs, d_wg_units, d_wg_prob); break;
     case 32: hipLaunchKernelGGL((k_rows_grouped<32, kW32, 2>), dim3(grid), dim3(64 * kW32), 0, s, d_probs, d_wg_units, d_wg_prob); break;
     case 64: hipLaunchKernelGGL((k_rows_grouped<64, kW64, 2>), dim3(grid), dim3(64 * kW64), 0, s, d_probs, d_wg_units, d_wg_prob); break;
+    case -16: hipLaunchKernelGGL((k_rows_bf16x3<16, 8, 2>), dim3(grid), dim3(64 * 8), 0, s, d_probs, d_wg_units, d_wg_prob); break;
+    case -32: hipLaunchKernelGGL((k_rows_bf16x3<32, kW32, 2>), dim3(grid), dim3(64 * kW32), 0, s, d_probs, d_wg_units, d_wg_prob); break;
+    case -64: hipLaunchKernelGGL((k_rows_bf16x3<64, kW64, 2>), dim3(grid), dim3(64 * kW64), 0, s, d_probs, d_wg_units, d_wg_prob); break;
     default: set_error(TTM_ERR_ARGUMENT, "short-row kernel: row height must be 16, 32 or 64"); return TTM_ERR_ARGUMENT;
     }
     hipError_t const e = hipGetLastError();

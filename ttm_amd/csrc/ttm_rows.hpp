@@ -1,5 +1,5 @@
-// Grouped TT products with short contractions (float32): the kernel for
-// multi-TTM chain stages whose contracted extent is at most 256.
+// Grouped TT products with short contractions (float32, plain or BF16X3):
+// the kernels for multi-TTM chain stages whose contracted extent is at most 256.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -41,7 +41,9 @@ bool rows_eligible(uint64_t nq, uint64_t m, uint64_t ncols);
 // Workgroups per compute unit of the kernel for a row height.
 unsigned rows_wg_per_cu(int height);
 
-std::string rows_symbol(int height);
+// bf16x3: the BF16X3 form (k_rows_bf16x3), A and B split into three
+// bfloat16 terms, six products on the 16x16x32 bf16 MFMA.
+std::string rows_symbol(int height, bool bf16x3 = false);
 
 // Units of the problems (unit_begin / nunits filled, in order) and the
 // launch's partition: workgroup b runs units [wg_units[b], wg_units[b + 1]),
@@ -49,9 +51,9 @@ std::string rows_symbol(int height);
 // nearly equal estimated cost (HBM bytes or padded matrix work, whichever is
 // larger, plus a fixed cost per unit).
 void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, std::vector<unsigned> &wg_units,
-                 std::vector<unsigned> &wg_prob);
+                 std::vector<unsigned> &wg_prob, bool bf16x3 = false);
 
 ttm_status_t launch_rows_grouped(int height, const RowsProblem *d_probs, const unsigned *d_wg_units,
-                                 const unsigned *d_wg_prob, unsigned grid, hipStream_t s);
+                                 const unsigned *d_wg_prob, unsigned grid, hipStream_t s, bool bf16x3 = false);
 
 } // namespace ttm_amd
