@@ -2,7 +2,8 @@
 a CPU model of the address arithmetic of gemm_tile (ttm_gemm.hip: k_gemm in
 every compiled configuration, both views, both B orientations, padded slabs,
 contraction splits, and the grouped k_gemm_grouped tiles) and lowp_tile
-(ttm_lowp.hip: k_lowp with pre-converted B planes, k_lowp_grouped), run for
+(ttm_lowp.hip: k_lowp with pre-converted B planes, k_lowp_grouped) and the
+short-row chain kernels (ttm_rows.hip: k_rows_grouped, k_rows_bf16x3), run for
 every thread of a block on the boundary cases -- the minimal interior tile,
 edge tiles, the last (ragged) slice of the contraction, ragged m, rows
 shorter than a vector.
@@ -645,5 +646,84 @@ def test_k_rows_grouped_unit_model(MB):
         want = B.reshape(m, nq) @ A.reshape(ncols, nq).T
         for u in range((ncols + 15) // 16):
             D = rows_unit(chk, A, B, nq, m, ncols, u, MB)
+            cols = min(16, ncols - 16 * u)
+            np.testing.assert_allclose(D[:m, :cols], want[:, 16 * u:16 * u + cols], rtol=1e-12, atol=1e-12)
+
+
+def rows_unit_bf16x3(chk, A, B, nq, m, ncols, u, MB):
+    """k_rows_bf16x3 (ttm_rows.hip), one unit: groups of 32 contraction
+    indices, lane (c, q) reads two 16-byte pieces A[n][32 g + 8 q ..] (the
+    last group's pieces each clamped to A's end and shifted), and the staged
+    B planes (row stride 32 k + 8 bf16) are read at lc * S + 8 q + 32 g
+    (+ 16 jt rows).  The split itself is exact in sum (x0 + x1 + x2 = x up to
+    2^-24 |x|), so the model multiplies the unsplit values."""
+    S = (nq + 31) // 32 * 32 + 8
+    ng = (nq + 31) // 32
+    width = 32 * ng
+    total = ncols * nq
+    sB = np.zeros(MB * S)
+    for j in range(MB):
+        for l in range(0, width, 2):
+            for x in (0, 1):
+                if j < m and l + x < nq:
+                    chk("b", j * nq + l + x)
+                    sB[j * S + l + x] = B[j * nq + l + x]
+            assert j * S + l + 1 < MB * (256 + 8)
+    col0 = u * 16
+    D = np.zeros((MB, 16))
+
+    def last_piece(e, lim):
+        ec = e if e + 4 <= total else total - 4
+        d = e - ec
+        chk("a", ec, 4)
+        rl = A[ec:ec + 4]
+        out = np.zeros(4)
+        for x in range(4):
+            if x < lim:
+                assert x + d < 4 and ec + x + d == e + x
+                out[x] = rl[x + d]
+        return out
+
+    for lane in range(64):
+        lc, lq = lane % 16, lane // 16
+        n = col0 + lc if col0 + lc < ncols else ncols - 1
+        groups = []
+        for g in range(ng - 1):
+            s = n * nq + 32 * g + 8 * lq
+            chk("a", s, 4)
+            chk("a", s + 4, 4)
+            assert 32 * g + 8 * lq + 7 < nq
+            groups.append(np.concatenate([A[s:s + 4], A[s + 4:s + 8]]))
+        e0 = n * nq + 32 * (ng - 1) + 8 * lq
+        lim0 = nq - (32 * (ng - 1) + 8 * lq)
+        groups.append(np.concatenate([last_piece(e0, lim0), last_piece(e0 + 4, lim0 - 4)]))
+        for g, fr in enumerate(groups):
+            for e in range(8):
+                for i in range(MB):
+                    addr = i * S + 8 * lq + 32 * g + e
+                    assert 0 <= addr < MB * S and 32 * g + 8 * lq + e < width
+                    D[i, lc] += sB[addr] * fr[e]
+    for lc in range(16):
+        if col0 + lc < ncols:
+            for j in range(m):
+                chk("c", j * ncols + col0 + lc)
+    return D
+
+
+@pytest.mark.parametrize("MB", [16, 32, 64])
+def test_k_rows_bf16x3_unit_model(MB):
+    """The BF16X3 short-row kernel's lane arithmetic (32-index groups, two
+    clamped pieces in the last group, three B planes) on the CPU: in-bounds
+    loads and the unit's block of C equal to B A^T."""
+    rng = np.random.default_rng(100 + MB)
+    cases = [(4, 1), (5, 3), (31, 17), (32, 16), (33, 33), (63, 40), (100, 5), (255, 18), (256, 16), (129, 1)]
+    for nq, ncols in cases:
+        m = int(rng.integers(MB // 2 + 1, MB + 1)) if MB > 16 else int(rng.integers(1, 17))
+        A = rng.standard_normal(ncols * nq)
+        B = rng.standard_normal(m * nq)
+        chk = Bounds({"a": ncols * nq, "b": m * nq, "c": m * ncols})
+        want = B.reshape(m, nq) @ A.reshape(ncols, nq).T
+        for u in range((ncols + 15) // 16):
+            D = rows_unit_bf16x3(chk, A, B, nq, m, ncols, u, MB)
             cols = min(16, ncols - 16 * u)
             np.testing.assert_allclose(D[:m, :cols], want[:, 16 * u:16 * u + cols], rtol=1e-12, atol=1e-12)
