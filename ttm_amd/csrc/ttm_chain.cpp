@@ -45,13 +45,13 @@ struct Stage
     int rows_height = 0;               // 0: not eligible
     bool rows = false;                 // the short-row kernel is chosen
     std::vector<RowsProblem> rprobs;
-    std::vector<unsigned> wg_units, wg_prob;
+    RowsLayout rlayout;
     unsigned total = 0;
     unsigned grid = 0;
     GroupProblem *d_probs = nullptr;   // inside the plan's tile allocation
     unsigned *d_tile_prob = nullptr;
     RowsProblem *d_rprobs = nullptr;
-    unsigned *d_wg_units = nullptr, *d_wg_prob = nullptr;
+    RowsSched rsched{};                // device schedule (inside the plan's tile allocation)
     std::string key;                   // shape class for the tuning cache
 };
 
@@ -163,11 +163,13 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
                 st.rprobs.push_back(RowsProblem{static_cast<const float *>(P.a), static_cast<const float *>(P.b),
                                                 static_cast<float *>(P.c), P.ncols, (unsigned)P.nq, (unsigned)P.m, 0, 0});
             st.grid = (unsigned)std::max(plan->num_cu, 1) * rows_wg_per_cu(st.rows_height);
-            rows_layout(st.rprobs, st.rows_height, st.grid, st.wg_units, st.wg_prob, plan->prec == TTM_PREC_BF16X3);
-            if (st.wg_units.back() >= (1u << 31)) return fail(TTM_ERR_ARGUMENT, "too many columns in one stage");
-            st.total = st.wg_units.back();
+            uint64_t units = 0;
+            for (RowsProblem const &P : st.rprobs) units += (P.ncols + 15) / 16;
+            if (units >= (1ull << 31)) return fail(TTM_ERR_ARGUMENT, "too many columns in one stage");
+            rows_layout(st.rprobs, st.rows_height, st.grid, st.rlayout, plan->prec == TTM_PREC_BF16X3);
+            st.total = st.rlayout.units;
             bytes = align_up(bytes, 256) + st.rprobs.size() * sizeof(RowsProblem);
-            bytes = align_up(bytes, 256) + (st.wg_units.size() + st.wg_prob.size()) * sizeof(unsigned);
+            bytes = align_up(bytes, 256) + (st.rlayout.wg_units.size() + st.rlayout.wg_prob.size()) * sizeof(unsigned);
             continue;
         }
         unsigned const bm = st.transpose ? 64u : (unsigned)st.cfg.bm();
@@ -228,15 +230,16 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
             if (s) return s;
             off += st.rprobs.size() * sizeof(RowsProblem);
             off = align_up(off, 256);
-            st.d_wg_units = reinterpret_cast<unsigned *>(base + off);
-            st.d_wg_prob = st.d_wg_units + st.wg_units.size();
-            s = hip_check(hipMemcpy(st.d_wg_units, st.wg_units.data(), st.wg_units.size() * sizeof(unsigned),
-                                    hipMemcpyHostToDevice));
-            if (!s)
-                s = hip_check(hipMemcpy(st.d_wg_prob, st.wg_prob.data(), st.wg_prob.size() * sizeof(unsigned),
-                                        hipMemcpyHostToDevice));
+            // the ranges' first units and their problems
+            RowsLayout const &L = st.rlayout;
+            std::vector<unsigned> words(L.wg_units);
+            words.insert(words.end(), L.wg_prob.begin(), L.wg_prob.end());
+            unsigned *w = reinterpret_cast<unsigned *>(base + off);
+            s = hip_check(hipMemcpy(w, words.data(), words.size() * sizeof(unsigned), hipMemcpyHostToDevice));
             if (s) return s;
-            off += (st.wg_units.size() + st.wg_prob.size()) * sizeof(unsigned);
+            st.rsched.wg_units = w;
+            st.rsched.wg_prob = w + L.wg_units.size();
+            off += words.size() * sizeof(unsigned);
             continue;
         }
         off = align_up(off, 256);
@@ -259,8 +262,7 @@ ttm_status_t launch_stage(ttm_chain_plan_t plan, const Stage &st, hipStream_t s)
 {
     if (st.total == 0) return TTM_SUCCESS;
     if (st.rows)
-        return launch_rows_grouped(st.rows_height, st.d_rprobs, st.d_wg_units, st.d_wg_prob, st.grid, s,
-                                   plan->prec == TTM_PREC_BF16X3);
+        return launch_rows_grouped(st.rows_height, st.d_rprobs, st.rsched, st.grid, s, plan->prec == TTM_PREC_BF16X3);
     if (st.transpose)
         return plan->is_double ? launch_transpose_grouped<double>(st.d_probs, st.d_tile_prob, st.total, st.grid, s)
                                : launch_transpose_grouped<float>(st.d_probs, st.d_tile_prob, st.total, st.grid, s);

@@ -62,10 +62,51 @@ constexpr int kStrideMax = (int)kRowsMaxNq + 4; // LDS row stride of B for nq = 
 // multiple of 4 whose quarter is odd).
 __host__ __device__ constexpr unsigned rows_stride(unsigned nq) { return (nq + 15) / 16 * 16 + 4; }
 
+// Element x of the piece that starts at element e of A (clamped to A's last
+// four elements, shift d): elements past the row end (x >= lim) are zero.
+__device__ __forceinline__ f32x4 piece_last(f32x4 rl, int d, int lim)
+{
+    f32x4 af;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+        float t = rl[x];
+        if (d == 1) t = x + 1 < 4 ? rl[(x + 1) & 3] : 0.f;
+        if (d == 2) t = x + 2 < 4 ? rl[(x + 2) & 3] : 0.f;
+        if (d == 3) t = x + 3 < 4 ? rl[(x + 3) & 3] : 0.f;
+        af[x] = x < lim ? t : 0.f;
+    }
+    return af;
+}
+
+// The launch's schedule: workgroup b runs the units [wg_units[b],
+// wg_units[b + 1]) (contiguous, of nearly equal estimated cost, starting in
+// problem wg_prob[b]) as blocks cut at problem ends; B is staged at the start
+// of each block, and the waves take the block's units round-robin.  (A pool
+// of chunks handed out through an atomic counter after about 85 % of the
+// static work measured the same within run-to-run noise: not kept.)
+template <int W, class Stage, class Unit>
+__device__ __forceinline__ void rows_run(const RowsProblem *__restrict__ probs, const RowsSched &sc, Stage &&stage,
+                                         Unit &&unit)
+{
+    unsigned const wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    unsigned sb = sc.wg_units[blockIdx.x];           // cursor (uniform)
+    unsigned const u1 = sc.wg_units[blockIdx.x + 1];
+    unsigned q = sc.wg_prob[blockIdx.x];             // problem of the cursor
+    while (sb < u1) {
+        while (sb >= probs[q].unit_begin + probs[q].nunits) ++q;
+        RowsProblem const P = probs[q];
+        unsigned const pend = P.unit_begin + P.nunits;
+        unsigned const be = pend < u1 ? pend : u1;
+        __syncthreads(); // every wave is done with the previous problem's B
+        stage(P);
+        __syncthreads();
+        for (unsigned v = sb + wave; v < be; v += W) unit(P, v);
+        sb = be;
+    }
+}
+
 template <int MB, int W, int WPE>
-__global__ __launch_bounds__(64 * W, WPE) void k_rows_grouped(const RowsProblem *__restrict__ probs,
-                                                           const unsigned *__restrict__ wg_units,
-                                                           const unsigned *__restrict__ wg_prob)
+__global__ __launch_bounds__(64 * W, WPE) void k_rows_grouped(const RowsProblem *__restrict__ probs, RowsSched sc)
 {
     __shared__ __attribute__((aligned(16))) float sB[MB * kStrideMax];
     constexpr int JT = MB / 16;
@@ -73,105 +114,80 @@ __global__ __launch_bounds__(64 * W, WPE) void k_rows_grouped(const RowsProblem 
     int const lane = tid & 63;
     int const wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     int const lc = lane & 15, lq = lane >> 4;
-    unsigned const u0 = wg_units[blockIdx.x], u1 = wg_units[blockIdx.x + 1];
-    if (u0 >= u1) return; // the whole workgroup: no barrier is skipped by part of it
-    unsigned p = wg_prob[blockIdx.x];
-    unsigned v = u0 + wave; // this wave's next unit
-    while (true) {
-        RowsProblem const P = probs[p];
-        unsigned const seg_end = P.unit_begin + P.nunits < u1 ? P.unit_begin + P.nunits : u1;
-        unsigned const nq = P.nq;
-        unsigned const S = rows_stride(nq);
-        unsigned const ng = (nq + 15) / 16;
-        unsigned const width = 16 * ng;
-        // B of this problem into LDS: rows past m and columns past nq are
-        // zero.  Wave w stages rows w, w + W, ...; a lane's loads are all
-        // issued before its stores (one round trip per problem).
-        __syncthreads(); // every wave is done with the previous problem's B
-        {
-            static_assert(MB % W == 0, "whole rows per wave");
-            constexpr int RPW = MB / W;            // rows per wave
-            constexpr int CPL = kStrideMax / 64;   // columns per lane, at most
-            float t[RPW][CPL];
+    // B of a problem into LDS: rows past m and columns past nq are zero.
+    // Wave w stages rows w, w + W, ...; a lane's loads are all issued before
+    // its stores (one round trip per problem).
+    auto stage = [&](const RowsProblem &P) {
+        static_assert(MB % W == 0, "whole rows per wave");
+        constexpr int RPW = MB / W;          // rows per wave
+        constexpr int CPL = kStrideMax / 64; // columns per lane, at most
+        unsigned const nq = P.nq, S = rows_stride(nq), width = (nq + 15) / 16 * 16;
+        float t[RPW][CPL];
 #pragma unroll
-            for (int r = 0; r < RPW; ++r)
+        for (int r = 0; r < RPW; ++r)
 #pragma unroll
-                for (int c = 0; c < CPL; ++c) {
-                    unsigned const j = wave + W * r, l = lane + 64 * c;
-                    t[r][c] = (j < P.m && l < nq) ? ldg<float>(P.b + (uint64_t)j * nq + l) : 0.f;
-                }
+            for (int c = 0; c < CPL; ++c) {
+                unsigned const j = wave + W * r, l = lane + 64 * c;
+                t[r][c] = (j < P.m && l < nq) ? ldg<float>(P.b + (uint64_t)j * nq + l) : 0.f;
+            }
 #pragma unroll
-            for (int r = 0; r < RPW; ++r)
+        for (int r = 0; r < RPW; ++r)
 #pragma unroll
-                for (int c = 0; c < CPL; ++c) {
-                    unsigned const j = wave + W * r, l = lane + 64 * c;
-                    if (l < width) sB[j * S + l] = t[r][c];
-                }
-        }
-        __syncthreads();
+            for (int c = 0; c < CPL; ++c) {
+                unsigned const j = wave + W * r, l = lane + 64 * c;
+                if (l < width) sB[j * S + l] = t[r][c];
+            }
+    };
+    auto unit = [&](const RowsProblem &P, unsigned v) {
+        unsigned const nq = P.nq, S = rows_stride(nq), ng = (nq + 15) / 16;
         uint64_t const total = P.ncols * nq;
         const float *brow = sB + lc * S + 4 * lq;
         int const lim = (int)nq - (int)(16 * (ng - 1) + 4 * lq); // valid elements of the last group's piece
-        for (; v < seg_end; v += W) {
-            uint64_t const col0 = (uint64_t)(v - P.unit_begin) * 16u;
-            // columns past the last one read the last one (not stored)
-            uint64_t const n = col0 + lc < P.ncols ? col0 + lc : P.ncols - 1;
-            const float *arow = P.a + n * nq + 4 * lq;
-            f32x4 ra[kGroups];
+        uint64_t const col0 = (uint64_t)(v - P.unit_begin) * 16u;
+        // columns past the last one read the last one (not stored)
+        uint64_t const n = col0 + lc < P.ncols ? col0 + lc : P.ncols - 1;
+        const float *arow = P.a + n * nq + 4 * lq;
+        f32x4 ra[kGroups];
 #pragma unroll
-            for (int g = 0; g < kGroups; ++g)
-                if (g + 1 < (int)ng) ra[g] = ldg<f32x4u>(arow + 16 * g);
-            // the last group: clamped to A's last four elements (shift d)
-            uint64_t const e = n * nq + 16 * (ng - 1) + 4 * lq;
-            uint64_t const ec = e + 4 <= total ? e : total - 4;
-            int const d = (int)(e - ec);
-            f32x4 const rl = ldg<f32x4u>(P.a + ec);
+        for (int g = 0; g < kGroups; ++g)
+            if (g + 1 < (int)ng) ra[g] = ldg<f32x4u>(arow + 16 * g);
+        // the last group: clamped to A's last four elements (shift d)
+        uint64_t const e = n * nq + 16 * (ng - 1) + 4 * lq;
+        uint64_t const ec = e + 4 <= total ? e : total - 4;
+        int const d = (int)(e - ec);
+        f32x4 const rl = ldg<f32x4u>(P.a + ec);
 
-            f32x4 acc[JT];
+        f32x4 acc[JT];
 #pragma unroll
-            for (int jt = 0; jt < JT; ++jt) acc[jt] = f32x4{0.f, 0.f, 0.f, 0.f};
-            auto group = [&](unsigned g, f32x4 af) {
-                f32x4 bf[JT];
+        for (int jt = 0; jt < JT; ++jt) acc[jt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        auto group = [&](unsigned g, f32x4 af) {
+            f32x4 bf[JT];
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) bf[jt] = *reinterpret_cast<const f32x4 *>(brow + 16 * jt * S + 16 * g);
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
 #pragma unroll
                 for (int jt = 0; jt < JT; ++jt)
-                    bf[jt] = *reinterpret_cast<const f32x4 *>(brow + 16 * jt * S + 16 * g);
+                    acc[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[jt][x], af[x], acc[jt], 0, 0, 0);
+        };
 #pragma unroll
-                for (int x = 0; x < 4; ++x)
+        for (int g = 0; g < kGroups; ++g)
+            if (g + 1 < (int)ng) group(g, ra[g]);
+        // element x of the piece is rl[x + d]; elements past the row end
+        // (x >= lim, which covers everything past A's end) are zero
+        group(ng - 1, piece_last(rl, d, lim));
+        // lane (c, q) holds C rows 16 jt + 4 q + r of column col0 + c
+        if (col0 + lc < P.ncols) {
 #pragma unroll
-                    for (int jt = 0; jt < JT; ++jt)
-                        acc[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[jt][x], af[x], acc[jt], 0, 0, 0);
-            };
+            for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
-            for (int g = 0; g < kGroups; ++g)
-                if (g + 1 < (int)ng) group(g, ra[g]);
-            {
-                // element x of the piece is rl[x + d]; elements past the row
-                // end (x >= lim, which covers everything past A's end) are zero
-                f32x4 af;
-#pragma unroll
-                for (int x = 0; x < 4; ++x) {
-                    float t = rl[x];
-                    if (d == 1) t = x + 1 < 4 ? rl[(x + 1) & 3] : 0.f;
-                    if (d == 2) t = x + 2 < 4 ? rl[(x + 2) & 3] : 0.f;
-                    if (d == 3) t = x + 3 < 4 ? rl[(x + 3) & 3] : 0.f;
-                    af[x] = x < lim ? t : 0.f;
+                for (int r = 0; r < 4; ++r) {
+                    unsigned const j = 16 * jt + 4 * lq + r;
+                    if (j < P.m) stg<float>(P.c + (uint64_t)j * P.ncols + col0 + lc, acc[jt][r]);
                 }
-                group(ng - 1, af);
-            }
-            // lane (c, q) holds C rows 16 jt + 4 q + r of column col0 + c
-            if (col0 + lc < P.ncols) {
-#pragma unroll
-                for (int jt = 0; jt < JT; ++jt)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        unsigned const j = 16 * jt + 4 * lq + r;
-                        if (j < P.m) stg<float>(P.c + (uint64_t)j * P.ncols + col0 + lc, acc[jt][r]);
-                    }
-            }
         }
-        if (seg_end >= u1) break;
-        ++p;
-    }
+    };
+    rows_run<W>(probs, sc, stage, unit);
 }
 
 // ---------------------------------------------------------------------------
@@ -218,26 +234,8 @@ __device__ __forceinline__ void split3x8(f32x4 lo, f32x4 hi, bf16x8 (&t)[3])
     }
 }
 
-// Element x of the piece that starts at element e of A (clamped to A's last
-// four elements, shift d): elements past the row end (x >= lim) are zero.
-__device__ __forceinline__ f32x4 piece_last(f32x4 rl, int d, int lim)
-{
-    f32x4 af;
-#pragma unroll
-    for (int x = 0; x < 4; ++x) {
-        float t = rl[x];
-        if (d == 1) t = x + 1 < 4 ? rl[(x + 1) & 3] : 0.f;
-        if (d == 2) t = x + 2 < 4 ? rl[(x + 2) & 3] : 0.f;
-        if (d == 3) t = x + 3 < 4 ? rl[(x + 3) & 3] : 0.f;
-        af[x] = x < lim ? t : 0.f;
-    }
-    return af;
-}
-
 template <int MB, int W, int WPE>
-__global__ __launch_bounds__(64 * W, WPE) void k_rows_bf16x3(const RowsProblem *__restrict__ probs,
-                                                          const unsigned *__restrict__ wg_units,
-                                                          const unsigned *__restrict__ wg_prob)
+__global__ __launch_bounds__(64 * W, WPE) void k_rows_bf16x3(const RowsProblem *__restrict__ probs, RowsSched sc)
 {
     __shared__ __attribute__((aligned(16))) __bf16 sB[3][MB * kStrideMaxH];
     constexpr int JT = MB / 16;
@@ -245,106 +243,94 @@ __global__ __launch_bounds__(64 * W, WPE) void k_rows_bf16x3(const RowsProblem *
     int const lane = tid & 63;
     int const wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     int const lc = lane & 15, lq = lane >> 4;
-    unsigned const u0 = wg_units[blockIdx.x], u1 = wg_units[blockIdx.x + 1];
-    if (u0 >= u1) return;
-    unsigned p = wg_prob[blockIdx.x];
-    unsigned v = u0 + wave;
-    while (true) {
-        RowsProblem const P = probs[p];
-        unsigned const seg_end = P.unit_begin + P.nunits < u1 ? P.unit_begin + P.nunits : u1;
-        unsigned const nq = P.nq;
-        unsigned const S = rows_stride_h(nq);
-        unsigned const ng = (nq + 31) / 32;
-        unsigned const width = 32 * ng;
-        __syncthreads();
-        {
-            // rows w, w + W, ...; a lane takes pairs of columns 2 (lane + 64 c)
-            static_assert(MB % W == 0, "whole rows per wave");
-            constexpr int RPW = MB / W;
-            constexpr int CPL = (kRowsMaxNq / 2 + 63) / 64; // column pairs per lane, at most
-            f32x2 t[RPW][CPL];
+    // B's three planes: rows w, w + W, ... of wave w; a lane takes pairs of
+    // columns 2 (lane + 64 c), all loads before the stores
+    auto stage = [&](const RowsProblem &P) {
+        static_assert(MB % W == 0, "whole rows per wave");
+        constexpr int RPW = MB / W;
+        constexpr int CPL = (kRowsMaxNq / 2 + 63) / 64; // column pairs per lane, at most
+        unsigned const nq = P.nq, S = rows_stride_h(nq), width = (nq + 31) / 32 * 32;
+        f32x2 t[RPW][CPL];
 #pragma unroll
-            for (int r = 0; r < RPW; ++r)
+        for (int r = 0; r < RPW; ++r)
 #pragma unroll
-                for (int c = 0; c < CPL; ++c) {
-                    unsigned const j = wave + W * r, l = 2 * (lane + 64 * c);
-                    const float *bj = P.b + (uint64_t)j * nq;
-                    t[r][c][0] = (j < P.m && l < nq) ? ldg<float>(bj + l) : 0.f;
-                    t[r][c][1] = (j < P.m && l + 1 < nq) ? ldg<float>(bj + l + 1) : 0.f;
+            for (int c = 0; c < CPL; ++c) {
+                unsigned const j = wave + W * r, l = 2 * (lane + 64 * c);
+                const float *bj = P.b + (uint64_t)j * nq;
+                t[r][c][0] = (j < P.m && l < nq) ? ldg<float>(bj + l) : 0.f;
+                t[r][c][1] = (j < P.m && l + 1 < nq) ? ldg<float>(bj + l + 1) : 0.f;
+            }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                unsigned const j = wave + W * r, l = 2 * (lane + 64 * c);
+                if (l < width) {
+                    bf16x2 p0, p1, p2;
+                    split3(t[r][c], p0, p1, p2);
+                    *reinterpret_cast<bf16x2 *>(&sB[0][j * S + l]) = p0;
+                    *reinterpret_cast<bf16x2 *>(&sB[1][j * S + l]) = p1;
+                    *reinterpret_cast<bf16x2 *>(&sB[2][j * S + l]) = p2;
                 }
-#pragma unroll
-            for (int r = 0; r < RPW; ++r)
-#pragma unroll
-                for (int c = 0; c < CPL; ++c) {
-                    unsigned const j = wave + W * r, l = 2 * (lane + 64 * c);
-                    if (l < width) {
-                        bf16x2 p0, p1, p2;
-                        split3(t[r][c], p0, p1, p2);
-                        *reinterpret_cast<bf16x2 *>(&sB[0][j * S + l]) = p0;
-                        *reinterpret_cast<bf16x2 *>(&sB[1][j * S + l]) = p1;
-                        *reinterpret_cast<bf16x2 *>(&sB[2][j * S + l]) = p2;
-                    }
-                }
-        }
-        __syncthreads();
+            }
+    };
+    auto unit = [&](const RowsProblem &P, unsigned v) {
+        unsigned const nq = P.nq, S = rows_stride_h(nq), ng = (nq + 31) / 32;
         uint64_t const total = P.ncols * nq;
         unsigned const boff = lc * S + 8 * lq;
         int const lim0 = (int)nq - (int)(32 * (ng - 1) + 8 * lq); // valid elements of the last group's first piece
-        for (; v < seg_end; v += W) {
-            uint64_t const col0 = (uint64_t)(v - P.unit_begin) * 16u;
-            uint64_t const n = col0 + lc < P.ncols ? col0 + lc : P.ncols - 1;
-            const float *arow = P.a + n * nq + 8 * lq;
-            f32x4 ra[kGroups32][2];
+        uint64_t const col0 = (uint64_t)(v - P.unit_begin) * 16u;
+        uint64_t const n = col0 + lc < P.ncols ? col0 + lc : P.ncols - 1;
+        const float *arow = P.a + n * nq + 8 * lq;
+        f32x4 ra[kGroups32][2];
 #pragma unroll
-            for (int g = 0; g < kGroups32; ++g)
-                if (g + 1 < (int)ng) {
-                    ra[g][0] = ldg<f32x4u>(arow + 32 * g);
-                    ra[g][1] = ldg<f32x4u>(arow + 32 * g + 4);
-                }
-            uint64_t const e0 = n * nq + 32 * (ng - 1) + 8 * lq, e1 = e0 + 4;
-            uint64_t const ec0 = e0 + 4 <= total ? e0 : total - 4, ec1 = e1 + 4 <= total ? e1 : total - 4;
-            int const d0 = (int)(e0 - ec0), d1 = (int)(e1 - ec1);
-            f32x4 const rl0 = ldg<f32x4u>(P.a + ec0), rl1 = ldg<f32x4u>(P.a + ec1);
-
-            f32x4 acc[JT];
-#pragma unroll
-            for (int jt = 0; jt < JT; ++jt) acc[jt] = f32x4{0.f, 0.f, 0.f, 0.f};
-            auto group = [&](unsigned g, f32x4 lo, f32x4 hi) {
-                bf16x8 ta[3];
-                split3x8(lo, hi, ta);
-#pragma unroll
-                for (int jt = 0; jt < JT; ++jt) {
-                    unsigned const o = boff + 16 * jt * S + 32 * g;
-                    bf16x8 const b0 = *reinterpret_cast<const bf16x8 *>(&sB[0][o]);
-                    bf16x8 const b1 = *reinterpret_cast<const bf16x8 *>(&sB[1][o]);
-                    bf16x8 const b2 = *reinterpret_cast<const bf16x8 *>(&sB[2][o]);
-                    f32x4 a = acc[jt];
-                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b1, ta[1], a, 0, 0, 0);
-                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b2, ta[0], a, 0, 0, 0);
-                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0, ta[2], a, 0, 0, 0);
-                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b1, ta[0], a, 0, 0, 0);
-                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0, ta[1], a, 0, 0, 0);
-                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0, ta[0], a, 0, 0, 0);
-                    acc[jt] = a;
-                }
-            };
-#pragma unroll
-            for (int g = 0; g < kGroups32; ++g)
-                if (g + 1 < (int)ng) group(g, ra[g][0], ra[g][1]);
-            group(ng - 1, piece_last(rl0, d0, lim0), piece_last(rl1, d1, lim0 - 4));
-            if (col0 + lc < P.ncols) {
-#pragma unroll
-                for (int jt = 0; jt < JT; ++jt)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        unsigned const j = 16 * jt + 4 * lq + r;
-                        if (j < P.m) stg<float>(P.c + (uint64_t)j * P.ncols + col0 + lc, acc[jt][r]);
-                    }
+        for (int g = 0; g < kGroups32; ++g)
+            if (g + 1 < (int)ng) {
+                ra[g][0] = ldg<f32x4u>(arow + 32 * g);
+                ra[g][1] = ldg<f32x4u>(arow + 32 * g + 4);
             }
+        uint64_t const e0 = n * nq + 32 * (ng - 1) + 8 * lq, e1 = e0 + 4;
+        uint64_t const ec0 = e0 + 4 <= total ? e0 : total - 4, ec1 = e1 + 4 <= total ? e1 : total - 4;
+        int const d0 = (int)(e0 - ec0), d1 = (int)(e1 - ec1);
+        f32x4 const rl0 = ldg<f32x4u>(P.a + ec0), rl1 = ldg<f32x4u>(P.a + ec1);
+
+        f32x4 acc[JT];
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) acc[jt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        auto group = [&](unsigned g, f32x4 lo, f32x4 hi) {
+            bf16x8 ta[3];
+            split3x8(lo, hi, ta);
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) {
+                unsigned const o = boff + 16 * jt * S + 32 * g;
+                bf16x8 const b0 = *reinterpret_cast<const bf16x8 *>(&sB[0][o]);
+                bf16x8 const b1 = *reinterpret_cast<const bf16x8 *>(&sB[1][o]);
+                bf16x8 const b2 = *reinterpret_cast<const bf16x8 *>(&sB[2][o]);
+                f32x4 a = acc[jt];
+                a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b1, ta[1], a, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b2, ta[0], a, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0, ta[2], a, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b1, ta[0], a, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0, ta[1], a, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0, ta[0], a, 0, 0, 0);
+                acc[jt] = a;
+            }
+        };
+#pragma unroll
+        for (int g = 0; g < kGroups32; ++g)
+            if (g + 1 < (int)ng) group(g, ra[g][0], ra[g][1]);
+        group(ng - 1, piece_last(rl0, d0, lim0), piece_last(rl1, d1, lim0 - 4));
+        if (col0 + lc < P.ncols) {
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    unsigned const j = 16 * jt + 4 * lq + r;
+                    if (j < P.m) stg<float>(P.c + (uint64_t)j * P.ncols + col0 + lc, acc[jt][r]);
+                }
         }
-        if (seg_end >= u1) break;
-        ++p;
-    }
+    };
+    rows_run<W>(probs, sc, stage, unit);
 }
 
 // (row height, waves per workgroup, waves per SIMD): one workgroup per
@@ -363,17 +349,19 @@ bool rows_eligible(uint64_t nq, uint64_t m, uint64_t ncols)
 
 unsigned rows_wg_per_cu(int) { return 1; }
 
+unsigned rows_waves(int height, bool bf16x3) { return bf16x3 ? 8u : height == 16 ? (unsigned)kW16 : height == 32 ? (unsigned)kW32 : (unsigned)kW64; }
+
+
 std::string rows_symbol(int height, bool bf16x3)
 {
     char buf[64];
-    int const w = bf16x3 ? 8 : height == 16 ? kW16 : height == 32 ? kW32 : kW64;
+    int const w = (int)rows_waves(height, bf16x3);
     int const wpe = !bf16x3 && height == 16 ? 4 : 2;
     std::snprintf(buf, sizeof buf, "%s<%d, %d, %d>", bf16x3 ? "k_rows_bf16x3" : "k_rows_grouped", height, w, wpe);
     return buf;
 }
 
-void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, std::vector<unsigned> &wg_units,
-                 std::vector<unsigned> &wg_prob, bool bf16x3)
+void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, RowsLayout &out, bool bf16x3)
 {
     // cost of a unit in bytes: HBM traffic or the padded matrix work at the
     // f32 MFMA ridge (157.3 TFLOP/s / 8 TB/s = 19.7 flop per byte), whichever
@@ -383,8 +371,10 @@ void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, std
     // the matrix time.
     double const mfma_scale = bf16x3 ? 0.375 : 1.0;
     std::vector<double> unit_cost;
+    std::vector<unsigned> unit_prob;
     unsigned begin = 0;
-    for (RowsProblem &P : probs) {
+    for (size_t pi = 0; pi < probs.size(); ++pi) {
+        RowsProblem &P = probs[pi];
         P.unit_begin = begin;
         P.nunits = (unsigned)((P.ncols + 15) / 16);
         begin += P.nunits;
@@ -394,38 +384,36 @@ void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, std
             double const bytes = cols * 4.0 * (P.nq + P.m);
             double const flops = 2.0 * 16.0 * padded * height;
             unit_cost.push_back(std::max(bytes, mfma_scale * flops / 19.66) + 2048.0);
+            unit_prob.push_back((unsigned)pi);
         }
     }
+    out.units = begin;
     double total = 0;
     for (double c : unit_cost) total += c;
-    wg_units.assign(grid + 1, begin);
-    wg_prob.assign(grid, 0);
+    out.wg_units.assign(grid + 1, begin);
+    out.wg_prob.assign(grid, 0);
     double acc = 0;
     unsigned u = 0;
     for (unsigned b = 0; b < grid; ++b) {
-        wg_units[b] = u;
+        out.wg_units[b] = u;
+        out.wg_prob[b] = u < begin ? unit_prob[u] : 0;
         double const target = total * (b + 1) / grid;
         while (u < begin && acc + 0.5 * unit_cost[u] <= target) acc += unit_cost[u++];
     }
-    wg_units[grid] = begin;
-    unsigned pi = 0;
-    for (unsigned b = 0; b < grid; ++b) {
-        while (pi + 1 < probs.size() && wg_units[b] >= probs[pi].unit_begin + probs[pi].nunits) ++pi;
-        wg_prob[b] = pi;
-    }
+    out.wg_units[grid] = begin;
 }
 
-ttm_status_t launch_rows_grouped(int height, const RowsProblem *d_probs, const unsigned *d_wg_units,
-                                 const unsigned *d_wg_prob, unsigned grid, hipStream_t s, bool bf16x3)
+ttm_status_t launch_rows_grouped(int height, const RowsProblem *d_probs, const RowsSched &sc, unsigned grid,
+                                 hipStream_t s, bool bf16x3)
 {
     if (grid == 0) return TTM_SUCCESS;
     switch (height * (bf16x3 ? -1 : 1)) {
-    case 16: hipLaunchKernelGGL((k_rows_grouped<16, kW16, 4>), dim3(grid), dim3(64 * kW16), 0, s, d_probs, d_wg_units, d_wg_prob); break;
-    case 32: hipLaunchKernelGGL((k_rows_grouped<32, kW32, 2>), dim3(grid), dim3(64 * kW32), 0, s, d_probs, d_wg_units, d_wg_prob); break;
-    case 64: hipLaunchKernelGGL((k_rows_grouped<64, kW64, 2>), dim3(grid), dim3(64 * kW64), 0, s, d_probs, d_wg_units, d_wg_prob); break;
-    case -16: hipLaunchKernelGGL((k_rows_bf16x3<16, 8, 2>), dim3(grid), dim3(64 * 8), 0, s, d_probs, d_wg_units, d_wg_prob); break;
-    case -32: hipLaunchKernelGGL((k_rows_bf16x3<32, kW32, 2>), dim3(grid), dim3(64 * kW32), 0, s, d_probs, d_wg_units, d_wg_prob); break;
-    case -64: hipLaunchKernelGGL((k_rows_bf16x3<64, kW64, 2>), dim3(grid), dim3(64 * kW64), 0, s, d_probs, d_wg_units, d_wg_prob); break;
+    case 16: hipLaunchKernelGGL((k_rows_grouped<16, kW16, 4>), dim3(grid), dim3(64 * kW16), 0, s, d_probs, sc); break;
+    case 32: hipLaunchKernelGGL((k_rows_grouped<32, kW32, 2>), dim3(grid), dim3(64 * kW32), 0, s, d_probs, sc); break;
+    case 64: hipLaunchKernelGGL((k_rows_grouped<64, kW64, 2>), dim3(grid), dim3(64 * kW64), 0, s, d_probs, sc); break;
+    case -16: hipLaunchKernelGGL((k_rows_bf16x3<16, 8, 2>), dim3(grid), dim3(64 * 8), 0, s, d_probs, sc); break;
+    case -32: hipLaunchKernelGGL((k_rows_bf16x3<32, 8, 2>), dim3(grid), dim3(64 * 8), 0, s, d_probs, sc); break;
+    case -64: hipLaunchKernelGGL((k_rows_bf16x3<64, 8, 2>), dim3(grid), dim3(64 * 8), 0, s, d_probs, sc); break;
     default: set_error(TTM_ERR_ARGUMENT, "short-row kernel: row height must be 16, 32 or 64"); return TTM_ERR_ARGUMENT;
     }
     hipError_t const e = hipGetLastError();

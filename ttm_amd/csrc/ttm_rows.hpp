@@ -45,15 +45,31 @@ unsigned rows_wg_per_cu(int height);
 // bfloat16 terms, six products on the 16x16x32 bf16 MFMA.
 std::string rows_symbol(int height, bool bf16x3 = false);
 
-// Units of the problems (unit_begin / nunits filled, in order) and the
-// launch's partition: workgroup b runs units [wg_units[b], wg_units[b + 1]),
-// which start inside problem wg_prob[b].  The ranges are contiguous and of
-// nearly equal estimated cost (HBM bytes or padded matrix work, whichever is
-// larger, plus a fixed cost per unit).
-void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, std::vector<unsigned> &wg_units,
-                 std::vector<unsigned> &wg_prob, bool bf16x3 = false);
+// Waves per workgroup of the kernel.
+unsigned rows_waves(int height, bool bf16x3 = false);
 
-ttm_status_t launch_rows_grouped(int height, const RowsProblem *d_probs, const unsigned *d_wg_units,
-                                 const unsigned *d_wg_prob, unsigned grid, hipStream_t s, bool bf16x3 = false);
+// A launch's schedule: workgroup b runs the units [wg_units[b],
+// wg_units[b + 1]), which start inside problem wg_prob[b].
+struct RowsLayout
+{
+    std::vector<unsigned> wg_units, wg_prob;
+    unsigned units = 0;
+};
+
+// The same in device memory, passed to the kernel by value.
+struct RowsSched
+{
+    const unsigned *wg_units;
+    const unsigned *wg_prob;
+};
+
+// Units of the problems (unit_begin / nunits filled, in order) and the
+// launch's schedule: contiguous ranges of nearly equal estimated cost (HBM
+// bytes or padded matrix work, whichever is larger, plus a fixed cost per
+// unit).
+void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, RowsLayout &out, bool bf16x3 = false);
+
+ttm_status_t launch_rows_grouped(int height, const RowsProblem *d_probs, const RowsSched &sc, unsigned grid,
+                                 hipStream_t s, bool bf16x3 = false);
 
 } // namespace ttm_amd
