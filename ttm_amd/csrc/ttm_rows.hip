@@ -1,5 +1,5 @@
-// Grouped TT products with short contractions (float32), for the stages of
-// batched multi-TTM chains:
+// Grouped TT products with short contractions (float32, plain and BF16X3),
+// for the stages of batched multi-TTM chains:
 //   C[j * ncols + n] = sum_{l < nq} B[j * nq + l] * A[n * nq + l],
 // m <= 64 rows of B, 4 <= nq <= 256.
 //
@@ -24,8 +24,9 @@
 // Workgroups are persistent over contiguous ranges of the launch's unit list
 // (ranges of nearly equal cost, computed on the host), so a workgroup stages
 // B only when its range crosses into the next problem; its waves take the
-// units of a range round-robin.  The staging is bracketed by barriers that
-// every wave of the workgroup reaches once per problem of the range.
+// units of each problem's part of the range round-robin.  The staging is
+// bracketed by barriers that every wave of the workgroup reaches once per
+// problem of the range.
 #include "ttm_rows.hpp"
 #include "ttm_plan.hpp"
 
@@ -81,9 +82,7 @@ __device__ __forceinline__ f32x4 piece_last(f32x4 rl, int d, int lim)
 // The launch's schedule: workgroup b runs the units [wg_units[b],
 // wg_units[b + 1]) (contiguous, of nearly equal estimated cost, starting in
 // problem wg_prob[b]) as blocks cut at problem ends; B is staged at the start
-// of each block, and the waves take the block's units round-robin.  (A pool
-// of chunks handed out through an atomic counter after about 85 % of the
-// static work measured the same within run-to-run noise: not kept.)
+// of each block, and the waves take the block's units round-robin.
 template <int W, class Stage, class Unit>
 __device__ __forceinline__ void rows_run(const RowsProblem *__restrict__ probs, const RowsSched &sc, Stage &&stage,
                                          Unit &&unit)
