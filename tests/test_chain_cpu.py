@@ -43,3 +43,28 @@ def test_precision_rules(ttmlib):
     assert L.ttm_chain_plan_create(1, descs, 0, 2, ctypes.byref(h)) == 22   # only default / BF16X3
     assert L.ttm_chain_plan_create(1, None, 0, 0, ctypes.byref(h)) == 25
     assert L.ttm_chain_plan_create(1, descs, 0, 0, None) == 25
+
+
+def test_short_row_schedule_covers_every_unit_once(ttmlib):
+    """rows_layout (the short-row kernels' host schedule) on 400 random
+    problem lists: contiguous in-order ranges that cover every unit exactly
+    once and name the problem of their first unit (tests/cpp/rows_schedule.cpp,
+    built against the library; host code only)."""
+    import pathlib
+    import shutil
+    import subprocess
+    root = pathlib.Path(__file__).resolve().parents[1]
+    gxx = shutil.which("g++")
+    if not gxx:
+        pytest.skip("no g++")
+    build = root / "tests" / "_build"
+    build.mkdir(exist_ok=True)
+    exe = build / "rows_schedule"
+    cmd = [gxx, "-O1", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           f"-I{root / 'include'}", f"-I{root / 'ttm_amd' / 'csrc'}", str(root / "tests" / "cpp" / "rows_schedule.cpp"),
+           "-o", str(exe), f"-L{root / 'ttm_amd' / 'lib'}", "-lttm_amd", f"-Wl,-rpath,{root / 'ttm_amd' / 'lib'}"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0, run.stdout + run.stderr
+    assert "0 violations" in run.stdout
