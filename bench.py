@@ -370,7 +370,8 @@ def tucker_items(torch, dev, count, seed=0x77140005):
 
 def tucker_batch(args):
     """Tucker batch (tucker_items) of 256 tensors as one plan, replayed as one
-    hipGraph, in FP32, BF16X3 and FP16X2."""
+    hipGraph, in FP32, BF16X3 and FP16X2, and the same tensors and factors in
+    float64 (FP64: twice the compulsory bytes)."""
     import torch
     from ttm_amd.chain import ChainPlan
     dev = torch.device("cuda:0")
@@ -379,9 +380,10 @@ def tucker_batch(args):
                        "A x1 U1 x2 U2 x3 U3, orthonormal rows, one hipGraph replay per step",
            "gflop": round(flops / 1e9, 2), "compulsory_gb": round(compulsory / 1e9, 3)}
     stream = torch.cuda.current_stream()
-    for prec in ("fp32", "bf16x3", "fp16x2"):
-        plan = ChainPlan(items, precision=None if prec == "fp32" else prec)
-        if prec == "fp32":
+    items64 = [(A.double(), [U.double() for U in Us], C.double()) for A, Us, C in items]
+    for prec in ("fp32", "bf16x3", "fp16x2", "fp64"):
+        plan = ChainPlan(items64 if prec == "fp64" else items, precision=prec if prec in ("bf16x3", "fp16x2") else None)
+        if prec in ("fp32", "fp64"):
             plan.tune()
         plan.run_graph()
         plan.run_graph()
@@ -393,12 +395,13 @@ def tucker_batch(args):
         e1.record(stream)
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / args.steps
+        cb = compulsory * (2 if prec == "fp64" else 1)
         out[prec] = {"ms": round(ms, 3), "tflops": round(flops / ms / 1e9, 3),
-                     "compulsory_gbps": round(compulsory / ms / 1e6, 1),
-                     "hbm_frac_of_compulsory": round(compulsory / ms / 1e6 / (PEAK_HBM / 1e9), 4),
+                     "compulsory_gbps": round(cb / ms / 1e6, 1),
+                     "hbm_frac_of_compulsory": round(cb / ms / 1e6 / (PEAK_HBM / 1e9), 4),
                      "stage_kernels": plan.stage_kernels}
         plan.close()
-    del items
+    del items, items64
     torch.cuda.empty_cache()
     return out
 

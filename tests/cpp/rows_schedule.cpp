@@ -26,10 +26,10 @@ int main()
             p.m = 1 + rg() % 64;
         }
         int const height = (int[]){16, 32, 64}[rg() % 3];
-        bool const bx = rg() % 2;
+        RowsKind const kind = (RowsKind[]){ROWS_F32, ROWS_BF16X3, ROWS_F64}[rg() % 3];
         unsigned const grid = (unsigned[]){1, 7, 256, 512}[rg() % 4];
         RowsLayout L;
-        rows_layout(P, height, grid, L, bx);
+        rows_layout(P, height, grid, L, kind);
         unsigned total = 0;
         for (int i = 0; i < np; ++i) {
             if (P[i].unit_begin != total || P[i].nunits != (P[i].ncols + 15) / 16) ++bad;

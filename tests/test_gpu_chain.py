@@ -157,30 +157,32 @@ def _short_row_items(torch, dev, rng, dtype):
     return items, refs
 
 
-@pytest.mark.parametrize("prec", [None, "bf16x3"])
-def test_short_row_kernel_matches_the_definition(gpu, monkeypatch, prec):
+@pytest.mark.parametrize("kind", ["f32", "bf16x3", "f64"])
+def test_short_row_kernel_matches_the_definition(gpu, monkeypatch, kind):
     """Stages with contractions of 4..256 and ranks up to 64 run the
-    short-row kernels (k_rows_grouped, k_rows_bf16x3) by default in float32
-    and BF16X3; with TTM_AMD_CHAIN_ROWS=0 the same plan runs the grouped
-    tiles.  Both are checked against the float64 definition (every output
-    written: C starts as NaN), eagerly and as a graph replay, and against
-    each other."""
+    short-row kernels (k_rows_grouped, k_rows_bf16x3, k_rows_f64) by default
+    in float32, BF16X3 and float64; with TTM_AMD_CHAIN_ROWS=0 the same plan
+    runs the grouped tiles.  Both are checked against the float64 definition
+    (every output written: C starts as NaN), eagerly and as a graph replay,
+    and against each other."""
     import torch
     from ttm_amd.chain import ChainPlan
     rng = np.random.default_rng(31)
-    items, refs = _short_row_items(torch, gpu, rng, torch.float32)
-    name = "k_rows_bf16x3<" if prec else "k_rows_grouped<"
+    dtype = torch.float64 if kind == "f64" else torch.float32
+    prec = "bf16x3" if kind == "bf16x3" else None
+    items, refs = _short_row_items(torch, gpu, rng, dtype)
+    name = {"f32": "k_rows_grouped<", "bf16x3": "k_rows_bf16x3<", "f64": "k_rows_f64<"}[kind]
     plan = ChainPlan(items, precision=prec)
     kernels = plan.stage_kernels
     _say("short-row stage kernels", kernels)
     assert sum(k.startswith(name) for k in kernels) >= 3
     plan.run()
-    check(items, refs, torch.float32, "short rows", prec=prec)
+    check(items, refs, dtype, "short rows", prec=prec)
     got_rows = [it[2].clone() for it in items]
     for it in items:
         it[2].fill_(float("nan"))
     plan.run_graph()
-    check(items, refs, torch.float32, "short rows graph", prec=prec)
+    check(items, refs, dtype, "short rows graph", prec=prec)
     plan.close()
 
     monkeypatch.setenv("TTM_AMD_CHAIN_ROWS", "0")
@@ -189,8 +191,8 @@ def test_short_row_kernel_matches_the_definition(gpu, monkeypatch, prec):
     plan = ChainPlan(items, precision=prec)
     assert not any(k.startswith("k_rows_") for k in plan.stage_kernels)
     plan.run()
-    check(items, refs, torch.float32, "grouped tiles", prec=prec)
-    eps = np.finfo(np.float32).eps
+    check(items, refs, dtype, "grouped tiles", prec=prec)
+    eps = np.finfo(np.float64 if kind == "f64" else np.float32).eps
     for a, it, (A, Us, _) in zip(got_rows, items, refs):
         scale = np.linalg.norm(A) * np.prod([np.linalg.norm(U) for U in Us])
         diff = float(torch.linalg.vector_norm((a - it[2]).double()))

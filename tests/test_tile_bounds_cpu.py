@@ -727,3 +727,81 @@ def test_k_rows_bf16x3_unit_model(MB):
             D = rows_unit_bf16x3(chk, A, B, nq, m, ncols, u, MB)
             cols = min(16, ncols - 16 * u)
             np.testing.assert_allclose(D[:m, :cols], want[:, 16 * u:16 * u + cols], rtol=1e-12, atol=1e-12)
+
+
+def rows_unit_f64(chk, A, B, nq, m, ncols, u, MB):
+    """k_rows_f64 (ttm_rows.hip), one unit: groups of 8 contraction indices,
+    lane (c, q) reads the 2-element piece A[n][8 g + 2 q ..] (the last
+    group's clamped to A's last two elements and shifted), B staged with row
+    stride 8 k + 2 and read at (16 jt + c) S + 2 q + 8 g, accumulator rows
+    16 jt + 4 r + q."""
+    S = (nq + 7) // 8 * 8 + 2
+    ng = (nq + 7) // 8
+    width = 8 * ng
+    total = ncols * nq
+    sB = np.full(MB * S, np.nan)
+    for j in range(MB):
+        for l in range(width):
+            assert j * S + l < MB * (256 + 2)
+            if j < m and l < nq:
+                chk("b", j * nq + l)
+                sB[j * S + l] = B[j * nq + l]
+            else:
+                sB[j * S + l] = 0.0
+    col0 = u * 16
+    D = np.zeros((MB, 16))
+    for lane in range(64):
+        lc, lq = lane % 16, lane // 16
+        n = col0 + lc if col0 + lc < ncols else ncols - 1
+        groups = []
+        for g in range(ng - 1):
+            s = n * nq + 2 * lq + 8 * g
+            chk("a", s, 2)
+            assert 8 * g + 2 * lq + 1 < nq
+            groups.append(A[s:s + 2])
+        lim = nq - (8 * (ng - 1) + 2 * lq)
+        e = n * nq + 8 * (ng - 1) + 2 * lq
+        ec = e if e + 2 <= total else total - 2
+        d = e - ec
+        chk("a", ec, 2)
+        rl = A[ec:ec + 2]
+        af = np.zeros(2)
+        for x in range(2):
+            if x < lim:
+                assert x + d < 2 and ec + x + d == e + x
+                af[x] = rl[x + d]
+        groups.append(af)
+        for g, fr in enumerate(groups):
+            for x in range(2):
+                for i in range(MB):
+                    addr = i * S + 2 * lq + 8 * g + x
+                    assert 0 <= addr < MB * S and 8 * g + 2 * lq + x < width
+                    D[i, lc] += sB[addr] * fr[x]
+    for lc in range(16):
+        if col0 + lc < ncols:
+            for r in range(4):
+                for jt in range(MB // 16):
+                    for q in range(4):  # lane quarter q holds rows 16 jt + 4 r + q
+                        jj = 16 * jt + 4 * r + q
+                        if jj < m:
+                            chk("c", jj * ncols + col0 + lc)
+    return D
+
+
+@pytest.mark.parametrize("MB", [16, 32, 64])
+def test_k_rows_f64_unit_model(MB):
+    """The float64 short-row kernel's lane arithmetic on the CPU (8-index
+    groups, a clamped 2-element last piece, B stride 8 k + 2): in-bounds
+    loads and the unit's block of C equal to B A^T."""
+    rng = np.random.default_rng(200 + MB)
+    cases = [(4, 1), (5, 3), (7, 17), (8, 16), (9, 33), (15, 40), (100, 5), (255, 18), (256, 16), (129, 1)]
+    for nq, ncols in cases:
+        m = int(rng.integers(MB // 2 + 1, MB + 1)) if MB > 16 else int(rng.integers(1, 17))
+        A = rng.standard_normal(ncols * nq)
+        B = rng.standard_normal(m * nq)
+        chk = Bounds({"a": ncols * nq, "b": m * nq, "c": m * ncols})
+        want = B.reshape(m, nq) @ A.reshape(ncols, nq).T
+        for u in range((ncols + 15) // 16):
+            D = rows_unit_f64(chk, A, B, nq, m, ncols, u, MB)
+            cols = min(16, ncols - 16 * u)
+            np.testing.assert_allclose(D[:m, :cols], want[:, 16 * u:16 * u + cols], rtol=1e-12, atol=1e-12)

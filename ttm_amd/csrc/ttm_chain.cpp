@@ -120,6 +120,11 @@ namespace
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+RowsKind rows_kind(ttm_chain_plan_t plan)
+{
+    return plan->is_double ? ROWS_F64 : plan->prec == TTM_PREC_BF16X3 ? ROWS_BF16X3 : ROWS_F32;
+}
+
 ttm_status_t fail(ttm_status_t st, const char *detail)
 {
     set_error(st, detail);
@@ -160,13 +165,12 @@ ttm_status_t layout_tiles(ttm_chain_plan_t plan)
         if (st.rows) {
             st.rprobs.clear();
             for (GroupProblem const &P : st.probs)
-                st.rprobs.push_back(RowsProblem{static_cast<const float *>(P.a), static_cast<const float *>(P.b),
-                                                static_cast<float *>(P.c), P.ncols, (unsigned)P.nq, (unsigned)P.m, 0, 0});
+                st.rprobs.push_back(RowsProblem{P.a, P.b, P.c, P.ncols, (unsigned)P.nq, (unsigned)P.m, 0, 0});
             st.grid = (unsigned)std::max(plan->num_cu, 1) * rows_wg_per_cu(st.rows_height);
             uint64_t units = 0;
             for (RowsProblem const &P : st.rprobs) units += (P.ncols + 15) / 16;
             if (units >= (1ull << 31)) return fail(TTM_ERR_ARGUMENT, "too many columns in one stage");
-            rows_layout(st.rprobs, st.rows_height, st.grid, st.rlayout, plan->prec == TTM_PREC_BF16X3);
+            rows_layout(st.rprobs, st.rows_height, st.grid, st.rlayout, rows_kind(plan));
             st.total = st.rlayout.units;
             bytes = align_up(bytes, 256) + st.rprobs.size() * sizeof(RowsProblem);
             bytes = align_up(bytes, 256) + (st.rlayout.wg_units.size() + st.rlayout.wg_prob.size()) * sizeof(unsigned);
@@ -262,7 +266,7 @@ ttm_status_t launch_stage(ttm_chain_plan_t plan, const Stage &st, hipStream_t s)
 {
     if (st.total == 0) return TTM_SUCCESS;
     if (st.rows)
-        return launch_rows_grouped(st.rows_height, st.d_rprobs, st.rsched, st.grid, s, plan->prec == TTM_PREC_BF16X3);
+        return launch_rows_grouped(st.rows_height, st.d_rprobs, st.rsched, st.grid, s, rows_kind(plan));
     if (st.transpose)
         return plan->is_double ? launch_transpose_grouped<double>(st.d_probs, st.d_tile_prob, st.total, st.grid, s)
                                : launch_transpose_grouped<float>(st.d_probs, st.d_tile_prob, st.total, st.grid, s);
@@ -418,7 +422,7 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
                 << log2_class(st.probs.size());
             st.key = key.str();
             st.cfg = default_config(stage_max_m[g], lowp_group_precision(precision));
-            if (!is_double && (precision == TTM_PREC_DEFAULT || precision == TTM_PREC_BF16X3) && rows_enabled()) {
+            if ((precision == TTM_PREC_DEFAULT || precision == TTM_PREC_BF16X3) && rows_enabled()) {
                 bool ok = true;
                 for (GroupProblem const &P : st.probs) ok = ok && rows_eligible(P.nq, P.m, P.ncols);
                 if (ok) st.rows_height = rows_height(stage_max_m[g]);
@@ -555,7 +559,7 @@ unsigned ttm_chain_stages(ttm_chain_plan_t plan) { return plan ? (unsigned)plan-
 const char *ttm_chain_stage_kernel(ttm_chain_plan_t plan, unsigned s)
 {
     if (!plan || s >= plan->stages.size()) return "";
-    plan->symbol = plan->stages[s].rows               ? rows_symbol(plan->stages[s].rows_height, plan->prec == TTM_PREC_BF16X3)
+    plan->symbol = plan->stages[s].rows               ? rows_symbol(plan->stages[s].rows_height, rows_kind(plan))
                    : plan->stages[s].transpose        ? (plan->is_double ? "k_transpose_grouped<double>"
                                                                          : "k_transpose_grouped<float>")
                    : lowp_group_precision(plan->prec) ? lowp_group_symbol(plan->prec, plan->stages[s].cfg.bm())

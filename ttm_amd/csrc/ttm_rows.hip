@@ -31,6 +31,7 @@
 #include "ttm_plan.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 
 namespace ttm_amd
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(64 * W, WPE) void k_rows_grouped(const RowsProblem 
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
                 unsigned const j = wave + W * r, l = lane + 64 * c;
-                t[r][c] = (j < P.m && l < nq) ? ldg<float>(P.b + (uint64_t)j * nq + l) : 0.f;
+                t[r][c] = (j < P.m && l < nq) ? ldg<float>(static_cast<const float *>(P.b) + (uint64_t)j * nq + l) : 0.f;
             }
 #pragma unroll
         for (int r = 0; r < RPW; ++r)
@@ -145,7 +146,8 @@ __global__ __launch_bounds__(64 * W, WPE) void k_rows_grouped(const RowsProblem 
         uint64_t const col0 = (uint64_t)(v - P.unit_begin) * 16u;
         // columns past the last one read the last one (not stored)
         uint64_t const n = col0 + lc < P.ncols ? col0 + lc : P.ncols - 1;
-        const float *arow = P.a + n * nq + 4 * lq;
+        const float *a = static_cast<const float *>(P.a);
+        const float *arow = a + n * nq + 4 * lq;
         f32x4 ra[kGroups];
 #pragma unroll
         for (int g = 0; g < kGroups; ++g)
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(64 * W, WPE) void k_rows_grouped(const RowsProblem 
         uint64_t const e = n * nq + 16 * (ng - 1) + 4 * lq;
         uint64_t const ec = e + 4 <= total ? e : total - 4;
         int const d = (int)(e - ec);
-        f32x4 const rl = ldg<f32x4u>(P.a + ec);
+        f32x4 const rl = ldg<f32x4u>(a + ec);
 
         f32x4 acc[JT];
 #pragma unroll
@@ -182,7 +184,7 @@ __global__ __launch_bounds__(64 * W, WPE) void k_rows_grouped(const RowsProblem 
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     unsigned const j = 16 * jt + 4 * lq + r;
-                    if (j < P.m) stg<float>(P.c + (uint64_t)j * P.ncols + col0 + lc, acc[jt][r]);
+                    if (j < P.m) stg<float>(static_cast<float *>(P.c) + (uint64_t)j * P.ncols + col0 + lc, acc[jt][r]);
                 }
         }
     };
@@ -255,7 +257,7 @@ __global__ __launch_bounds__(64 * W, WPE) void k_rows_bf16x3(const RowsProblem *
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
                 unsigned const j = wave + W * r, l = 2 * (lane + 64 * c);
-                const float *bj = P.b + (uint64_t)j * nq;
+                const float *bj = static_cast<const float *>(P.b) + (uint64_t)j * nq;
                 t[r][c][0] = (j < P.m && l < nq) ? ldg<float>(bj + l) : 0.f;
                 t[r][c][1] = (j < P.m && l + 1 < nq) ? ldg<float>(bj + l + 1) : 0.f;
             }
@@ -280,7 +282,8 @@ __global__ __launch_bounds__(64 * W, WPE) void k_rows_bf16x3(const RowsProblem *
         int const lim0 = (int)nq - (int)(32 * (ng - 1) + 8 * lq); // valid elements of the last group's first piece
         uint64_t const col0 = (uint64_t)(v - P.unit_begin) * 16u;
         uint64_t const n = col0 + lc < P.ncols ? col0 + lc : P.ncols - 1;
-        const float *arow = P.a + n * nq + 8 * lq;
+        const float *a = static_cast<const float *>(P.a);
+        const float *arow = a + n * nq + 8 * lq;
         f32x4 ra[kGroups32][2];
 #pragma unroll
         for (int g = 0; g < kGroups32; ++g)
@@ -291,7 +294,7 @@ __global__ __launch_bounds__(64 * W, WPE) void k_rows_bf16x3(const RowsProblem *
         uint64_t const e0 = n * nq + 32 * (ng - 1) + 8 * lq, e1 = e0 + 4;
         uint64_t const ec0 = e0 + 4 <= total ? e0 : total - 4, ec1 = e1 + 4 <= total ? e1 : total - 4;
         int const d0 = (int)(e0 - ec0), d1 = (int)(e1 - ec1);
-        f32x4 const rl0 = ldg<f32x4u>(P.a + ec0), rl1 = ldg<f32x4u>(P.a + ec1);
+        f32x4 const rl0 = ldg<f32x4u>(a + ec0), rl1 = ldg<f32x4u>(a + ec1);
 
         f32x4 acc[JT];
 #pragma unroll
@@ -325,17 +328,137 @@ __global__ __launch_bounds__(64 * W, WPE) void k_rows_bf16x3(const RowsProblem *
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     unsigned const j = 16 * jt + 4 * lq + r;
-                    if (j < P.m) stg<float>(P.c + (uint64_t)j * P.ncols + col0 + lc, acc[jt][r]);
+                    if (j < P.m) stg<float>(static_cast<float *>(P.c) + (uint64_t)j * P.ncols + col0 + lc, acc[jt][r]);
                 }
         }
     };
     rows_run<W>(probs, sc, stage, unit);
 }
 
-// (row height, waves per workgroup, waves per SIMD): one workgroup per
-// compute unit; 16 waves for m <= 16, eight for the taller rows (150 and 206
-// registers).
-constexpr int kW16 = 16, kW32 = 8, kW64 = 8;
+// ---------------------------------------------------------------------------
+// Float64: the same units on the 16x16x4 f64 MFMA.  A group is 8 contraction
+// indices: lane (c, q) reads the 16-byte piece A[n0 + c][8 g + 2 q .. + 1]
+// (element-aligned when nq is odd) and MFMA x (of two) takes element x, so
+// the contraction index of MFMA x at lane quarter q is 8 g + 2 q + x.  A unit
+// of 256 contraction indices is 32 groups (128 registers), so the groups are
+// loaded and multiplied in chunks of 16.  B is staged with a row stride of
+// 8 k + 2 doubles (16-byte fragment reads of 16 rows on distinct bank groups).
+// The accumulator of a lane holds C rows 16 jt + 4 r + q of column n0 + c.
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kGroups8 = (int)(kRowsMaxNq / 8);
+constexpr int kChunk8 = 16;
+constexpr int kStrideMaxD = (int)kRowsMaxNq + 2;
+__host__ __device__ constexpr unsigned rows_stride_d(unsigned nq) { return (nq + 7) / 8 * 8 + 2; }
+
+// The last group's piece: A's last two elements when it would run past A's
+// end (shift d); elements past the row end (x >= lim) are zero.
+__device__ __forceinline__ f64x2 piece_last_d(f64x2 rl, int d, int lim)
+{
+    f64x2 af;
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        double t = rl[x];
+        if (d == 1) t = x == 0 ? rl[1] : 0.0;
+        af[x] = x < lim ? t : 0.0;
+    }
+    return af;
+}
+
+template <int MB, int W, int WPE>
+__global__ __launch_bounds__(64 * W, WPE) void k_rows_f64(const RowsProblem *__restrict__ probs, RowsSched sc)
+{
+    __shared__ __attribute__((aligned(16))) double sB[MB * kStrideMaxD];
+    constexpr int JT = MB / 16;
+    int const tid = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int const lc = lane & 15, lq = lane >> 4;
+    auto stage = [&](const RowsProblem &P) {
+        static_assert(MB % W == 0, "whole rows per wave");
+        constexpr int RPW = MB / W;
+        constexpr int CPL = kStrideMaxD / 64;
+        unsigned const nq = P.nq, S = rows_stride_d(nq), width = (nq + 7) / 8 * 8;
+        double t[RPW][CPL];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                unsigned const j = wave + W * r, l = lane + 64 * c;
+                t[r][c] = (j < P.m && l < nq) ? ldg<double>(static_cast<const double *>(P.b) + (uint64_t)j * nq + l) : 0.0;
+            }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                unsigned const j = wave + W * r, l = lane + 64 * c;
+                if (l < width) sB[j * S + l] = t[r][c];
+            }
+    };
+    auto unit = [&](const RowsProblem &P, unsigned v) {
+        unsigned const nq = P.nq, S = rows_stride_d(nq), ng = (nq + 7) / 8;
+        uint64_t const total = P.ncols * nq;
+        const double *brow = sB + lc * S + 2 * lq;
+        int const lim = (int)nq - (int)(8 * (ng - 1) + 2 * lq); // valid elements of the last group's piece
+        uint64_t const col0 = (uint64_t)(v - P.unit_begin) * 16u;
+        uint64_t const n = col0 + lc < P.ncols ? col0 + lc : P.ncols - 1;
+        const double *a = static_cast<const double *>(P.a);
+        const double *arow = a + n * nq + 2 * lq;
+        uint64_t const e = n * nq + 8 * (ng - 1) + 2 * lq;
+        uint64_t const ec = e + 2 <= total ? e : total - 2;
+        int const d = (int)(e - ec);
+        f64x2 const rl = ldg<f64x2u>(a + ec);
+
+        f64x4 acc[JT];
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) acc[jt] = f64x4{0.0, 0.0, 0.0, 0.0};
+        auto group = [&](unsigned g, f64x2 af) {
+            f64x2 bf[JT];
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) bf[jt] = *reinterpret_cast<const f64x2 *>(brow + 16 * jt * S + 8 * g);
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int jt = 0; jt < JT; ++jt)
+                    acc[jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[jt][x], af[x], acc[jt], 0, 0, 0);
+        };
+        for (int c0 = 0; c0 + 1 < (int)ng; c0 += kChunk8) {
+            f64x2 ra[kChunk8];
+#pragma unroll
+            for (int g = 0; g < kChunk8; ++g)
+                if (c0 + g + 1 < (int)ng) ra[g] = ldg<f64x2u>(arow + 8 * (c0 + g));
+#pragma unroll
+            for (int g = 0; g < kChunk8; ++g)
+                if (c0 + g + 1 < (int)ng) group(c0 + g, ra[g]);
+        }
+        group(ng - 1, piece_last_d(rl, d, lim));
+        if (col0 + lc < P.ncols) {
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    unsigned const j = 16 * jt + 4 * r + lq;
+                    if (j < P.m) stg<double>(static_cast<double *>(P.c) + (uint64_t)j * P.ncols + col0 + lc, acc[jt][r]);
+                }
+        }
+    };
+    rows_run<W>(probs, sc, stage, unit);
+}
+
+// Waves per workgroup (one workgroup per compute unit) and waves per SIMD
+// the registers are budgeted for: 16 and 4 for float32 and float64 with
+// m <= 16, 8 and 2 for the taller rows and every BF16X3 kernel (whose
+// accumulators, split terms and fragments need up to 256 registers).
+struct RowsCfg
+{
+    int w, wpe;
+};
+constexpr RowsCfg rows_cfg(int height, RowsKind kind)
+{
+    return kind != ROWS_BF16X3 && height == 16 ? RowsCfg{16, 4} : RowsCfg{8, 2};
+}
 
 } // namespace
 
@@ -348,27 +471,28 @@ bool rows_eligible(uint64_t nq, uint64_t m, uint64_t ncols)
 
 unsigned rows_wg_per_cu(int) { return 1; }
 
-unsigned rows_waves(int height, bool bf16x3) { return bf16x3 ? 8u : height == 16 ? (unsigned)kW16 : height == 32 ? (unsigned)kW32 : (unsigned)kW64; }
+unsigned rows_waves(int height, RowsKind kind) { return (unsigned)rows_cfg(height, kind).w; }
 
-
-std::string rows_symbol(int height, bool bf16x3)
+std::string rows_symbol(int height, RowsKind kind)
 {
     char buf[64];
-    int const w = (int)rows_waves(height, bf16x3);
-    int const wpe = !bf16x3 && height == 16 ? 4 : 2;
-    std::snprintf(buf, sizeof buf, "%s<%d, %d, %d>", bf16x3 ? "k_rows_bf16x3" : "k_rows_grouped", height, w, wpe);
+    RowsCfg const c = rows_cfg(height, kind);
+    char const *name = kind == ROWS_BF16X3 ? "k_rows_bf16x3" : kind == ROWS_F64 ? "k_rows_f64" : "k_rows_grouped";
+    std::snprintf(buf, sizeof buf, "%s<%d, %d, %d>", name, height, c.w, c.wpe);
     return buf;
 }
 
-void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, RowsLayout &out, bool bf16x3)
+void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, RowsLayout &out, RowsKind kind)
 {
     // cost of a unit in bytes: HBM traffic or the padded matrix work at the
-    // f32 MFMA ridge (157.3 TFLOP/s / 8 TB/s = 19.7 flop per byte), whichever
-    // is larger, plus a fixed 2 KB for its instruction stream.  BF16X3 takes
-    // six 16x16x32 bf16 products (16 cycles each) per 32 contraction indices
-    // where float32 takes eight 16x16x4 products (32 cycles each): 0.375 of
-    // the matrix time.
-    double const mfma_scale = bf16x3 ? 0.375 : 1.0;
+    // MFMA ridge (float32: 157.3 TFLOP/s / 8 TB/s = 19.7 flop per byte;
+    // float64: 78.6 / 8 = 9.8), whichever is larger, plus a fixed 2 KB for its
+    // instruction stream.  BF16X3 takes six 16x16x32 bf16 products (16 cycles
+    // each) per 32 contraction indices where float32 takes eight 16x16x4
+    // products (32 cycles each): 0.375 of the matrix time.
+    double const ridge = kind == ROWS_F64 ? 9.83 : kind == ROWS_BF16X3 ? 19.66 / 0.375 : 19.66;
+    double const esz = kind == ROWS_F64 ? 8.0 : 4.0;
+    double const grp = kind == ROWS_F64 ? 8.0 : kind == ROWS_BF16X3 ? 32.0 : 16.0; // contraction group
     std::vector<double> unit_cost;
     std::vector<unsigned> unit_prob;
     unsigned begin = 0;
@@ -377,12 +501,12 @@ void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, Row
         P.unit_begin = begin;
         P.nunits = (unsigned)((P.ncols + 15) / 16);
         begin += P.nunits;
-        double const padded = bf16x3 ? 32.0 * ((P.nq + 31) / 32) : 16.0 * ((P.nq + 15) / 16);
+        double const padded = grp * std::ceil(P.nq / grp);
         for (unsigned u = 0; u < P.nunits; ++u) {
             double const cols = (double)std::min<uint64_t>(16, P.ncols - (uint64_t)u * 16);
-            double const bytes = cols * 4.0 * (P.nq + P.m);
+            double const bytes = cols * esz * (P.nq + P.m);
             double const flops = 2.0 * 16.0 * padded * height;
-            unit_cost.push_back(std::max(bytes, mfma_scale * flops / 19.66) + 2048.0);
+            unit_cost.push_back(std::max(bytes, flops / ridge) + 2048.0);
             unit_prob.push_back((unsigned)pi);
         }
     }
@@ -402,18 +526,41 @@ void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, Row
     out.wg_units[grid] = begin;
 }
 
+namespace
+{
+template <RowsKind K, int H>
+void launch_one(const RowsProblem *d_probs, const RowsSched &sc, unsigned grid, hipStream_t s)
+{
+    constexpr RowsCfg c = rows_cfg(H, K);
+    if constexpr (K == ROWS_F32)
+        hipLaunchKernelGGL((k_rows_grouped<H, c.w, c.wpe>), dim3(grid), dim3(64 * c.w), 0, s, d_probs, sc);
+    else if constexpr (K == ROWS_BF16X3)
+        hipLaunchKernelGGL((k_rows_bf16x3<H, c.w, c.wpe>), dim3(grid), dim3(64 * c.w), 0, s, d_probs, sc);
+    else
+        hipLaunchKernelGGL((k_rows_f64<H, c.w, c.wpe>), dim3(grid), dim3(64 * c.w), 0, s, d_probs, sc);
+}
+template <RowsKind K>
+bool launch_kind(int height, const RowsProblem *d_probs, const RowsSched &sc, unsigned grid, hipStream_t s)
+{
+    switch (height) {
+    case 16: launch_one<K, 16>(d_probs, sc, grid, s); return true;
+    case 32: launch_one<K, 32>(d_probs, sc, grid, s); return true;
+    case 64: launch_one<K, 64>(d_probs, sc, grid, s); return true;
+    default: return false;
+    }
+}
+} // namespace
+
 ttm_status_t launch_rows_grouped(int height, const RowsProblem *d_probs, const RowsSched &sc, unsigned grid,
-                                 hipStream_t s, bool bf16x3)
+                                 hipStream_t s, RowsKind kind)
 {
     if (grid == 0) return TTM_SUCCESS;
-    switch (height * (bf16x3 ? -1 : 1)) {
-    case 16: hipLaunchKernelGGL((k_rows_grouped<16, kW16, 4>), dim3(grid), dim3(64 * kW16), 0, s, d_probs, sc); break;
-    case 32: hipLaunchKernelGGL((k_rows_grouped<32, kW32, 2>), dim3(grid), dim3(64 * kW32), 0, s, d_probs, sc); break;
-    case 64: hipLaunchKernelGGL((k_rows_grouped<64, kW64, 2>), dim3(grid), dim3(64 * kW64), 0, s, d_probs, sc); break;
-    case -16: hipLaunchKernelGGL((k_rows_bf16x3<16, 8, 2>), dim3(grid), dim3(64 * 8), 0, s, d_probs, sc); break;
-    case -32: hipLaunchKernelGGL((k_rows_bf16x3<32, 8, 2>), dim3(grid), dim3(64 * 8), 0, s, d_probs, sc); break;
-    case -64: hipLaunchKernelGGL((k_rows_bf16x3<64, 8, 2>), dim3(grid), dim3(64 * 8), 0, s, d_probs, sc); break;
-    default: set_error(TTM_ERR_ARGUMENT, "short-row kernel: row height must be 16, 32 or 64"); return TTM_ERR_ARGUMENT;
+    bool const ok = kind == ROWS_F32      ? launch_kind<ROWS_F32>(height, d_probs, sc, grid, s)
+                    : kind == ROWS_BF16X3 ? launch_kind<ROWS_BF16X3>(height, d_probs, sc, grid, s)
+                                          : launch_kind<ROWS_F64>(height, d_probs, sc, grid, s);
+    if (!ok) {
+        set_error(TTM_ERR_ARGUMENT, "short-row kernel: row height must be 16, 32 or 64");
+        return TTM_ERR_ARGUMENT;
     }
     hipError_t const e = hipGetLastError();
     if (e != hipSuccess) {

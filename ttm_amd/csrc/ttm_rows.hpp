@@ -1,4 +1,4 @@
-// Grouped TT products with short contractions (float32, plain or BF16X3):
+// Grouped TT products with short contractions (float32, BF16X3, float64):
 // the kernels for multi-TTM chain stages whose contracted extent is at most 256.
 #pragma once
 
@@ -15,13 +15,15 @@ namespace ttm_amd
 
 // One product of a short-row launch, in the TT view:
 //   C[j * ncols + n] = sum_{l < nq} B[j * nq + l] * A[n * nq + l]
-// Its columns are cut into units of 16 (unit u covers columns 16 u ..
-// 16 u + 15); unit_begin is the problem's first unit in the launch's list.
+// (elements of the launch's kind: float for ROWS_F32 and ROWS_BF16X3, double
+// for ROWS_F64).  Its columns are cut into units of 16 (unit u covers
+// columns 16 u .. 16 u + 15); unit_begin is the problem's first unit in the
+// launch's list.
 struct RowsProblem
 {
-    const float *a;
-    const float *b;
-    float *c;
+    const void *a;
+    const void *b;
+    void *c;
     uint64_t ncols;
     unsigned nq;
     unsigned m;
@@ -29,24 +31,33 @@ struct RowsProblem
     unsigned nunits;
 };
 
+// The arithmetic of a launch: plain float32 (k_rows_grouped), float32 with A
+// and B split into three bfloat16 terms and six products on the 16x16x32 bf16
+// MFMA (k_rows_bf16x3), plain float64 (k_rows_f64).
+enum RowsKind : int
+{
+    ROWS_F32 = 0,
+    ROWS_BF16X3 = 1,
+    ROWS_F64 = 2
+};
+
 constexpr unsigned kRowsMinNq = 4;
 constexpr unsigned kRowsMaxNq = 256;
 
-// Row heights the kernel is compiled for (16, 32, 64); 0 when m > 64.
+// Row heights the kernels are compiled for (16, 32, 64); 0 when m > 64.
 int rows_height(uint64_t m);
 
-// A problem the kernel can run: 4 <= nq <= 256, 1 <= m <= 64.
+// A problem the kernels can run: 4 <= nq <= 256, 1 <= m <= 64.
 bool rows_eligible(uint64_t nq, uint64_t m, uint64_t ncols);
 
 // Workgroups per compute unit of the kernel for a row height.
 unsigned rows_wg_per_cu(int height);
 
-// bf16x3: the BF16X3 form (k_rows_bf16x3), A and B split into three
-// bfloat16 terms, six products on the 16x16x32 bf16 MFMA.
-std::string rows_symbol(int height, bool bf16x3 = false);
+// Kernel instance name, as profilers print it.
+std::string rows_symbol(int height, RowsKind kind = ROWS_F32);
 
 // Waves per workgroup of the kernel.
-unsigned rows_waves(int height, bool bf16x3 = false);
+unsigned rows_waves(int height, RowsKind kind = ROWS_F32);
 
 // A launch's schedule: workgroup b runs the units [wg_units[b],
 // wg_units[b + 1]), which start inside problem wg_prob[b].
@@ -67,9 +78,10 @@ struct RowsSched
 // launch's schedule: contiguous ranges of nearly equal estimated cost (HBM
 // bytes or padded matrix work, whichever is larger, plus a fixed cost per
 // unit).
-void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, RowsLayout &out, bool bf16x3 = false);
+void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, RowsLayout &out,
+                 RowsKind kind = ROWS_F32);
 
 ttm_status_t launch_rows_grouped(int height, const RowsProblem *d_probs, const RowsSched &sc, unsigned grid,
-                                 hipStream_t s, bool bf16x3 = false);
+                                 hipStream_t s, RowsKind kind = ROWS_F32);
 
 } // namespace ttm_amd
