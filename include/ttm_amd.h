@@ -260,7 +260,7 @@ ttm_status_t ttm_chain_run_graph(ttm_chain_plan_t plan, ttm_stream_t stream);
  * fastest (also for later plans of the same shape class in this process, and
  * in the file named by TTM_AMD_TUNE_CACHE when set).  Drops a captured graph.
  * Stages whose contractions are 4..256 long and whose ranks are at most 64
- * run the short-row kernels by default in float32, BF16X3 and float64
+ * run the short-row kernels by default (float32, BF16X3, FP16X2, float64)
  * (TTM_AMD_CHAIN_ROWS=0 at plan creation keeps them on the tiles); the
  * plain-precision tuner times the short-row kernel next to the tile shapes. */
 ttm_status_t ttm_chain_tune(ttm_chain_plan_t plan, ttm_stream_t stream);

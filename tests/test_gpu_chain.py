@@ -157,32 +157,36 @@ def _short_row_items(torch, dev, rng, dtype):
     return items, refs
 
 
-@pytest.mark.parametrize("kind", ["f32", "bf16x3", "f64"])
+@pytest.mark.parametrize("kind", ["f32", "bf16x3", "fp16x2", "f64"])
 def test_short_row_kernel_matches_the_definition(gpu, monkeypatch, kind):
     """Stages with contractions of 4..256 and ranks up to 64 run the
-    short-row kernels (k_rows_grouped, k_rows_bf16x3, k_rows_f64) by default
-    in float32, BF16X3 and float64; with TTM_AMD_CHAIN_ROWS=0 the same plan
-    runs the grouped tiles.  Both are checked against the float64 definition
-    (every output written: C starts as NaN), eagerly and as a graph replay,
-    and against each other."""
+    short-row kernels (k_rows_grouped, k_rows_bf16x3, k_rows_fp16x2,
+    k_rows_f64) by default in float32, BF16X3, FP16X2 and float64; with
+    TTM_AMD_CHAIN_ROWS=0 the same plan runs the grouped tiles.  Both are
+    checked against the float64 definition (every output written: C starts
+    as NaN), eagerly and as a graph replay, and against each other.  The
+    split precisions are held to the float32 bar here (normwise error
+    relative to ||A|| prod ||U_k||): the batch has cores of one element that
+    cancel hundreds of terms, where an error relative to the result itself
+    measures the cancellation, not the arithmetic."""
     import torch
     from ttm_amd.chain import ChainPlan
     rng = np.random.default_rng(31)
     dtype = torch.float64 if kind == "f64" else torch.float32
-    prec = "bf16x3" if kind == "bf16x3" else None
+    prec = kind if kind in ("bf16x3", "fp16x2") else None
     items, refs = _short_row_items(torch, gpu, rng, dtype)
-    name = {"f32": "k_rows_grouped<", "bf16x3": "k_rows_bf16x3<", "f64": "k_rows_f64<"}[kind]
+    name = {"f32": "k_rows_grouped<", "bf16x3": "k_rows_bf16x3<", "fp16x2": "k_rows_fp16x2<", "f64": "k_rows_f64<"}[kind]
     plan = ChainPlan(items, precision=prec)
     kernels = plan.stage_kernels
     _say("short-row stage kernels", kernels)
     assert sum(k.startswith(name) for k in kernels) >= 3
     plan.run()
-    check(items, refs, dtype, "short rows", prec=prec)
+    check(items, refs, dtype, "short rows")
     got_rows = [it[2].clone() for it in items]
     for it in items:
         it[2].fill_(float("nan"))
     plan.run_graph()
-    check(items, refs, dtype, "short rows graph", prec=prec)
+    check(items, refs, dtype, "short rows graph")
     plan.close()
 
     monkeypatch.setenv("TTM_AMD_CHAIN_ROWS", "0")
@@ -191,12 +195,45 @@ def test_short_row_kernel_matches_the_definition(gpu, monkeypatch, kind):
     plan = ChainPlan(items, precision=prec)
     assert not any(k.startswith("k_rows_") for k in plan.stage_kernels)
     plan.run()
-    check(items, refs, dtype, "grouped tiles", prec=prec)
+    check(items, refs, dtype, "grouped tiles")
     eps = np.finfo(np.float64 if kind == "f64" else np.float32).eps
     for a, it, (A, Us, _) in zip(got_rows, items, refs):
         scale = np.linalg.norm(A) * np.prod([np.linalg.norm(U) for U in Us])
         diff = float(torch.linalg.vector_norm((a - it[2]).double()))
         assert diff <= 8 * sum(A.shape) * eps * scale, (A.shape, diff / scale)
+    plan.close()
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x2"])
+def test_short_row_split_precisions_far_from_one(gpu, prec):
+    """The split short-row kernels on tensors and factors scaled by powers of
+    two far outside the binary16 range (A by 2^-40 .. 2^40, the factors by
+    2^-20 .. 2^20; every result stays a normal float32): FP16X2 scales A per
+    unit and B per problem before the split, BF16X3 needs no scale; both stay
+    at the 1e-6 normwise bar against the float64 definition."""
+    import torch
+    from ttm_amd.chain import ChainPlan
+    rng = np.random.default_rng(47)
+    items, refs = [], []
+    for t in range(24):
+        na = [int(x) for x in rng.integers(4, 70, size=3)]
+        pi = [int(x) + 1 for x in rng.permutation(3)]
+        sa, su = 2.0 ** int(rng.integers(-40, 41)), 2.0 ** int(rng.integers(-20, 21))
+        A = rng.uniform(-1, 1, size=na) * sa
+        Us = [rng.uniform(-1, 1, size=(int(rng.integers(1, 65)), n)) * su for n in na]
+        At = to_layout(torch, gpu, A, pi, torch.float32)
+        Ut = [torch.from_numpy(U).to(gpu, torch.float32).contiguous() for U in Us]
+        Ct = to_layout(torch, gpu, np.full([U.shape[0] for U in Us], np.nan), pi, torch.float32)
+        # the reference takes the float32-rounded inputs (natural mode order)
+        A32 = At.double().cpu().numpy()
+        items.append((At, Ut, Ct, pi))
+        refs.append((A32, [U.double().cpu().numpy() for U in Ut], einsum_chain(A32, [U.double().cpu().numpy() for U in Ut])))
+    plan = ChainPlan(items, precision=prec)
+    kernels = plan.stage_kernels
+    _say(prec, "far-from-one stage kernels", kernels)
+    assert all(k.startswith("k_rows_") for k in kernels)
+    plan.run()
+    check(items, refs, torch.float32, f"{prec} far from one", prec=prec)
     plan.close()
 
 

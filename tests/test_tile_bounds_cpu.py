@@ -714,7 +714,8 @@ def rows_unit_bf16x3(chk, A, B, nq, m, ncols, u, MB):
 def test_k_rows_bf16x3_unit_model(MB):
     """The BF16X3 short-row kernel's lane arithmetic (32-index groups, two
     clamped pieces in the last group, three B planes) on the CPU: in-bounds
-    loads and the unit's block of C equal to B A^T."""
+    loads and the unit's block of C equal to B A^T.  k_rows_fp16x2 takes the
+    same groups, pieces and plane layout (two planes)."""
     rng = np.random.default_rng(100 + MB)
     cases = [(4, 1), (5, 3), (31, 17), (32, 16), (33, 33), (63, 40), (100, 5), (255, 18), (256, 16), (129, 1)]
     for nq, ncols in cases:

@@ -122,7 +122,10 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 RowsKind rows_kind(ttm_chain_plan_t plan)
 {
-    return plan->is_double ? ROWS_F64 : plan->prec == TTM_PREC_BF16X3 ? ROWS_BF16X3 : ROWS_F32;
+    return plan->is_double                    ? ROWS_F64
+           : plan->prec == TTM_PREC_BF16X3 ? ROWS_BF16X3
+           : plan->prec == TTM_PREC_FP16X2 ? ROWS_FP16X2
+                                           : ROWS_F32;
 }
 
 ttm_status_t fail(ttm_status_t st, const char *detail)
@@ -360,9 +363,9 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
     // larger) so that each launch gets a tile height that fits its ranks,
     // plus one copy launch for the modes without a factor; launches of one
     // stage are independent, stages run in order.
-    // (FP16X2 keeps the two tile heights of its grouped kernel; BF16X3 and
-    // the plain precisions take the short-row kernels' classes)
-    int const classes = precision == TTM_PREC_FP16X2 || (precision == TTM_PREC_BF16X3 && !rows_enabled()) ? 2 : 4;
+    // (with the short-row kernels switched off, the reduced precisions keep
+    // the two tile heights of their grouped kernel)
+    int const classes = lowp_group_precision(precision) && !rows_enabled() ? 2 : 4;
     int const groups_per_stage = classes + 1;
     auto m_class = [&](uint64_t m) {
         if (classes == 2) return m <= 64 ? 0 : 1; // the two reduced-precision tile heights
@@ -422,7 +425,7 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
                 << log2_class(st.probs.size());
             st.key = key.str();
             st.cfg = default_config(stage_max_m[g], lowp_group_precision(precision));
-            if ((precision == TTM_PREC_DEFAULT || precision == TTM_PREC_BF16X3) && rows_enabled()) {
+            if (rows_enabled()) {
                 bool ok = true;
                 for (GroupProblem const &P : st.probs) ok = ok && rows_eligible(P.nq, P.m, P.ncols);
                 if (ok) st.rows_height = rows_height(stage_max_m[g]);

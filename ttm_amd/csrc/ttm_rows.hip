@@ -1,5 +1,5 @@
-// Grouped TT products with short contractions (float32, plain and BF16X3),
-// for the stages of batched multi-TTM chains:
+// Grouped TT products with short contractions (float32 plain, BF16X3 and
+// FP16X2, and float64), for the stages of batched multi-TTM chains:
 //   C[j * ncols + n] = sum_{l < nq} B[j * nq + l] * A[n * nq + l],
 // m <= 64 rows of B, 4 <= nq <= 256.
 //
@@ -336,6 +336,174 @@ __global__ __launch_bounds__(64 * W, WPE) void k_rows_bf16x3(const RowsProblem *
 }
 
 // ---------------------------------------------------------------------------
+// FP16X2: the BF16X3 units with A and B split into two binary16 terms after
+// scaling by powers of two (x 2^e = x0 + x1 + O(2^-22 |x 2^e|), the
+// difference exact in float32) and three products on the 16x16x32 f16 MFMA,
+// smallest first: (1,0), (0,1), (0,0).  The exponent e = 15 - E of an
+// operand's largest finite magnitude f 2^E (f in [0.5, 1)) keeps its scaled
+// values below 2^15; A's exponent is taken per unit (the largest finite
+// magnitude among the unit's 16 rows, a wave reduction), B's per problem (a
+// workgroup reduction when it is staged), and the accumulators are scaled
+// back by 2^-(ea + eb).  Infinities and NaNs set no scale and pass through
+// the split.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ int fp16x2_exp(float amax)
+{
+    if (!(amax > 0.f) || !__builtin_isfinite(amax)) return 0;
+    int e = 0;
+    (void)frexpf(amax, &e);
+    return 15 - e;
+}
+
+__device__ __forceinline__ float finite_abs(float x) { return __builtin_isfinite(x) ? fabsf(x) : 0.f; }
+
+__device__ __forceinline__ void split2h(f32x2 x, f16x2 &p0, f16x2 &p1)
+{
+    p0 = __builtin_convertvector(x, f16x2);
+    p1 = __builtin_convertvector(x - __builtin_convertvector(p0, f32x2), f16x2);
+}
+
+// The two terms of eight floats scaled by 2^e, as f16x8 operands.
+__device__ __forceinline__ void split2x8(f32x4 lo, f32x4 hi, int e, f16x8 (&t)[2])
+{
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        f32x2 x = h < 2 ? f32x2{lo[2 * h], lo[2 * h + 1]} : f32x2{hi[2 * h - 4], hi[2 * h - 3]};
+        x = f32x2{ldexpf(x[0], e), ldexpf(x[1], e)};
+        f16x2 p0, p1;
+        split2h(x, p0, p1);
+        t[0][2 * h] = p0[0];
+        t[0][2 * h + 1] = p0[1];
+        t[1][2 * h] = p1[0];
+        t[1][2 * h + 1] = p1[1];
+    }
+}
+
+template <int MB, int W, int WPE>
+__global__ __launch_bounds__(64 * W, WPE) void k_rows_fp16x2(const RowsProblem *__restrict__ probs, RowsSched sc)
+{
+    __shared__ __attribute__((aligned(16))) _Float16 sB[2][MB * kStrideMaxH];
+    __shared__ float s_red[W];
+    constexpr int JT = MB / 16;
+    int const tid = threadIdx.x;
+    int const lane = tid & 63;
+    int const wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int const lc = lane & 15, lq = lane >> 4;
+    int eb = 0; // B's exponent of the problem in LDS
+    auto wave_max = [](float v) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+        return v;
+    };
+    auto stage = [&](const RowsProblem &P) {
+        static_assert(MB % W == 0, "whole rows per wave");
+        constexpr int RPW = MB / W;
+        constexpr int CPL = (kRowsMaxNq / 2 + 63) / 64; // column pairs per lane, at most
+        unsigned const nq = P.nq, S = rows_stride_h(nq), width = (nq + 31) / 32 * 32;
+        f32x2 t[RPW][CPL];
+        float amax = 0.f;
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                unsigned const j = wave + W * r, l = 2 * (lane + 64 * c);
+                const float *bj = static_cast<const float *>(P.b) + (uint64_t)j * nq;
+                t[r][c][0] = (j < P.m && l < nq) ? ldg<float>(bj + l) : 0.f;
+                t[r][c][1] = (j < P.m && l + 1 < nq) ? ldg<float>(bj + l + 1) : 0.f;
+                amax = fmaxf(amax, fmaxf(finite_abs(t[r][c][0]), finite_abs(t[r][c][1])));
+            }
+        amax = wave_max(amax);
+        if (lane == 0) s_red[wave] = amax;
+        __syncthreads();
+        amax = 0.f;
+#pragma unroll
+        for (int w = 0; w < W; ++w) amax = fmaxf(amax, s_red[w]);
+        eb = fp16x2_exp(amax);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                unsigned const j = wave + W * r, l = 2 * (lane + 64 * c);
+                if (l < width) {
+                    f16x2 p0, p1;
+                    split2h(f32x2{ldexpf(t[r][c][0], eb), ldexpf(t[r][c][1], eb)}, p0, p1);
+                    *reinterpret_cast<f16x2 *>(&sB[0][j * S + l]) = p0;
+                    *reinterpret_cast<f16x2 *>(&sB[1][j * S + l]) = p1;
+                }
+            }
+    };
+    auto unit = [&](const RowsProblem &P, unsigned v) {
+        unsigned const nq = P.nq, S = rows_stride_h(nq), ng = (nq + 31) / 32;
+        uint64_t const total = P.ncols * nq;
+        unsigned const boff = lc * S + 8 * lq;
+        int const lim0 = (int)nq - (int)(32 * (ng - 1) + 8 * lq);
+        uint64_t const col0 = (uint64_t)(v - P.unit_begin) * 16u;
+        uint64_t const n = col0 + lc < P.ncols ? col0 + lc : P.ncols - 1;
+        const float *a = static_cast<const float *>(P.a);
+        const float *arow = a + n * nq + 8 * lq;
+        f32x4 ra[kGroups32][2];
+#pragma unroll
+        for (int g = 0; g < kGroups32; ++g)
+            if (g + 1 < (int)ng) {
+                ra[g][0] = ldg<f32x4u>(arow + 32 * g);
+                ra[g][1] = ldg<f32x4u>(arow + 32 * g + 4);
+            }
+        uint64_t const e0 = n * nq + 32 * (ng - 1) + 8 * lq, e1 = e0 + 4;
+        uint64_t const ec0 = e0 + 4 <= total ? e0 : total - 4, ec1 = e1 + 4 <= total ? e1 : total - 4;
+        int const d0 = (int)(e0 - ec0), d1 = (int)(e1 - ec1);
+        f32x4 const last0 = piece_last(ldg<f32x4u>(a + ec0), d0, lim0);
+        f32x4 const last1 = piece_last(ldg<f32x4u>(a + ec1), d1, lim0 - 4);
+        // A's exponent: the largest finite magnitude of the unit
+        float amax = 0.f;
+#pragma unroll
+        for (int g = 0; g < kGroups32; ++g)
+            if (g + 1 < (int)ng)
+#pragma unroll
+                for (int x = 0; x < 4; ++x) amax = fmaxf(amax, fmaxf(finite_abs(ra[g][0][x]), finite_abs(ra[g][1][x])));
+#pragma unroll
+        for (int x = 0; x < 4; ++x) amax = fmaxf(amax, fmaxf(finite_abs(last0[x]), finite_abs(last1[x])));
+        int const ea = fp16x2_exp(wave_max(amax));
+
+        f32x4 acc[JT];
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) acc[jt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        auto group = [&](unsigned g, f32x4 lo, f32x4 hi) {
+            f16x8 ta[2];
+            split2x8(lo, hi, ea, ta);
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) {
+                unsigned const o = boff + 16 * jt * S + 32 * g;
+                f16x8 const b0 = *reinterpret_cast<const f16x8 *>(&sB[0][o]);
+                f16x8 const b1 = *reinterpret_cast<const f16x8 *>(&sB[1][o]);
+                f32x4 t = acc[jt];
+                t = __builtin_amdgcn_mfma_f32_16x16x32_f16(b0, ta[1], t, 0, 0, 0);
+                t = __builtin_amdgcn_mfma_f32_16x16x32_f16(b1, ta[0], t, 0, 0, 0);
+                t = __builtin_amdgcn_mfma_f32_16x16x32_f16(b0, ta[0], t, 0, 0, 0);
+                acc[jt] = t;
+            }
+        };
+#pragma unroll
+        for (int g = 0; g < kGroups32; ++g)
+            if (g + 1 < (int)ng) group(g, ra[g][0], ra[g][1]);
+        group(ng - 1, last0, last1);
+        int const back = -(ea + eb);
+        if (col0 + lc < P.ncols) {
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    unsigned const j = 16 * jt + 4 * lq + r;
+                    if (j < P.m)
+                        stg<float>(static_cast<float *>(P.c) + (uint64_t)j * P.ncols + col0 + lc, ldexpf(acc[jt][r], back));
+                }
+        }
+    };
+    rows_run<W>(probs, sc, stage, unit);
+}
+
+// ---------------------------------------------------------------------------
 // Float64: the same units on the 16x16x4 f64 MFMA.  A group is 8 contraction
 // indices: lane (c, q) reads the 16-byte piece A[n0 + c][8 g + 2 q .. + 1]
 // (element-aligned when nq is odd) and MFMA x (of two) takes element x, so
@@ -449,15 +617,15 @@ __global__ __launch_bounds__(64 * W, WPE) void k_rows_f64(const RowsProblem *__r
 
 // Waves per workgroup (one workgroup per compute unit) and waves per SIMD
 // the registers are budgeted for: 16 and 4 for float32 and float64 with
-// m <= 16, 8 and 2 for the taller rows and every BF16X3 kernel (whose
-// accumulators, split terms and fragments need up to 256 registers).
+// m <= 16, 8 and 2 for the taller rows and every split-precision kernel
+// (whose accumulators, split terms and fragments need up to 256 registers).
 struct RowsCfg
 {
     int w, wpe;
 };
 constexpr RowsCfg rows_cfg(int height, RowsKind kind)
 {
-    return kind != ROWS_BF16X3 && height == 16 ? RowsCfg{16, 4} : RowsCfg{8, 2};
+    return (kind == ROWS_F32 || kind == ROWS_F64) && height == 16 ? RowsCfg{16, 4} : RowsCfg{8, 2};
 }
 
 } // namespace
@@ -477,7 +645,10 @@ std::string rows_symbol(int height, RowsKind kind)
 {
     char buf[64];
     RowsCfg const c = rows_cfg(height, kind);
-    char const *name = kind == ROWS_BF16X3 ? "k_rows_bf16x3" : kind == ROWS_F64 ? "k_rows_f64" : "k_rows_grouped";
+    char const *name = kind == ROWS_BF16X3   ? "k_rows_bf16x3"
+                       : kind == ROWS_FP16X2 ? "k_rows_fp16x2"
+                       : kind == ROWS_F64    ? "k_rows_f64"
+                                             : "k_rows_grouped";
     std::snprintf(buf, sizeof buf, "%s<%d, %d, %d>", name, height, c.w, c.wpe);
     return buf;
 }
@@ -489,10 +660,14 @@ void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, Row
     // float64: 78.6 / 8 = 9.8), whichever is larger, plus a fixed 2 KB for its
     // instruction stream.  BF16X3 takes six 16x16x32 bf16 products (16 cycles
     // each) per 32 contraction indices where float32 takes eight 16x16x4
-    // products (32 cycles each): 0.375 of the matrix time.
-    double const ridge = kind == ROWS_F64 ? 9.83 : kind == ROWS_BF16X3 ? 19.66 / 0.375 : 19.66;
+    // products (32 cycles each): 0.375 of the matrix time; FP16X2 three
+    // products: 0.1875.
+    double const ridge = kind == ROWS_F64      ? 9.83
+                         : kind == ROWS_BF16X3 ? 19.66 / 0.375
+                         : kind == ROWS_FP16X2 ? 19.66 / 0.1875
+                                               : 19.66;
     double const esz = kind == ROWS_F64 ? 8.0 : 4.0;
-    double const grp = kind == ROWS_F64 ? 8.0 : kind == ROWS_BF16X3 ? 32.0 : 16.0; // contraction group
+    double const grp = kind == ROWS_F64 ? 8.0 : kind == ROWS_F32 ? 16.0 : 32.0; // contraction group
     std::vector<double> unit_cost;
     std::vector<unsigned> unit_prob;
     unsigned begin = 0;
@@ -536,6 +711,8 @@ void launch_one(const RowsProblem *d_probs, const RowsSched &sc, unsigned grid, 
         hipLaunchKernelGGL((k_rows_grouped<H, c.w, c.wpe>), dim3(grid), dim3(64 * c.w), 0, s, d_probs, sc);
     else if constexpr (K == ROWS_BF16X3)
         hipLaunchKernelGGL((k_rows_bf16x3<H, c.w, c.wpe>), dim3(grid), dim3(64 * c.w), 0, s, d_probs, sc);
+    else if constexpr (K == ROWS_FP16X2)
+        hipLaunchKernelGGL((k_rows_fp16x2<H, c.w, c.wpe>), dim3(grid), dim3(64 * c.w), 0, s, d_probs, sc);
     else
         hipLaunchKernelGGL((k_rows_f64<H, c.w, c.wpe>), dim3(grid), dim3(64 * c.w), 0, s, d_probs, sc);
 }
@@ -557,6 +734,7 @@ ttm_status_t launch_rows_grouped(int height, const RowsProblem *d_probs, const R
     if (grid == 0) return TTM_SUCCESS;
     bool const ok = kind == ROWS_F32      ? launch_kind<ROWS_F32>(height, d_probs, sc, grid, s)
                     : kind == ROWS_BF16X3 ? launch_kind<ROWS_BF16X3>(height, d_probs, sc, grid, s)
+                    : kind == ROWS_FP16X2 ? launch_kind<ROWS_FP16X2>(height, d_probs, sc, grid, s)
                                           : launch_kind<ROWS_F64>(height, d_probs, sc, grid, s);
     if (!ok) {
         set_error(TTM_ERR_ARGUMENT, "short-row kernel: row height must be 16, 32 or 64");

@@ -1,4 +1,4 @@
-// Grouped TT products with short contractions (float32, BF16X3, float64):
+// Grouped TT products with short contractions (float32, BF16X3, FP16X2, float64):
 // the kernels for multi-TTM chain stages whose contracted extent is at most 256.
 #pragma once
 
@@ -33,12 +33,15 @@ struct RowsProblem
 
 // The arithmetic of a launch: plain float32 (k_rows_grouped), float32 with A
 // and B split into three bfloat16 terms and six products on the 16x16x32 bf16
-// MFMA (k_rows_bf16x3), plain float64 (k_rows_f64).
+// MFMA (k_rows_bf16x3), float32 with A and B scaled by powers of two and split
+// into two binary16 terms, three products on the 16x16x32 f16 MFMA
+// (k_rows_fp16x2), plain float64 (k_rows_f64).
 enum RowsKind : int
 {
     ROWS_F32 = 0,
     ROWS_BF16X3 = 1,
-    ROWS_F64 = 2
+    ROWS_F64 = 2,
+    ROWS_FP16X2 = 3
 };
 
 constexpr unsigned kRowsMinNq = 4;
