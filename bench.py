@@ -368,6 +368,30 @@ def tucker_items(torch, dev, count, seed=0x77140005):
     return items, ranks, flops, compulsory
 
 
+def tucker_launch_roofline(ranks, extents, esz, peak):
+    """Sum over the launches of a chain plan (stage s of every tensor, split by
+    rank class m <= 16 / 32 / 64 / larger) of each launch's own bound:
+    max(HBM bytes / 8 TB/s, matrix flops / peak), the ranks padded to the
+    class height and the contraction to whole groups of 16 (the short-row
+    kernels' and the tiles' padding).  The plan cannot beat this sum (its
+    launches run one after the other within a stage class and stage by stage)."""
+    total = 0.0
+    for s in range(3):
+        cls = {}
+        for rk, na in zip(ranks, extents):
+            cur = list(na)
+            for k in range(s):
+                cur[k] = rk[k]
+            nq, m = na[s], rk[s]
+            ncols = int(np.prod(cur)) // nq
+            h = 16 if m <= 16 else 32 if m <= 32 else 64 if m <= 64 else 128 * ((m + 127) // 128)
+            b, f = cls.get(h, (0.0, 0.0))
+            cls[h] = (b + esz * (ncols * nq + ncols * m + m * nq),
+                      f + 2.0 * h * (16 * ((nq + 15) // 16)) * ncols)
+        total += sum(max(b / PEAK_HBM, f / peak) for b, f in cls.values())
+    return total * 1e3  # ms
+
+
 def tucker_batch(args):
     """Tucker batch (tucker_items) of 256 tensors as one plan, replayed as one
     hipGraph, in FP32, BF16X3 and FP16X2, and the same tensors and factors in
@@ -375,7 +399,8 @@ def tucker_batch(args):
     import torch
     from ttm_amd.chain import ChainPlan
     dev = torch.device("cuda:0")
-    items, _, flops, compulsory = tucker_items(torch, dev, args.tucker_count)
+    items, ranks, flops, compulsory = tucker_items(torch, dev, args.tucker_count)
+    extents = [list(A.shape) for A, _, _ in items]
     out = {"workload": f"{args.tucker_count} tensors, extents U{{16..256}}^3, ranks U{{4..64}}, "
                        "A x1 U1 x2 U2 x3 U3, orthonormal rows, one hipGraph replay per step",
            "gflop": round(flops / 1e9, 2), "compulsory_gb": round(compulsory / 1e9, 3)}
@@ -400,6 +425,11 @@ def tucker_batch(args):
                      "compulsory_gbps": round(cb / ms / 1e6, 1),
                      "hbm_frac_of_compulsory": round(cb / ms / 1e6 / (PEAK_HBM / 1e9), 4),
                      "stage_kernels": plan.stage_kernels}
+        if prec in ("fp32", "fp64"):
+            roof = tucker_launch_roofline(ranks, extents, 8 if prec == "fp64" else 4,
+                                          PEAK_FLOPS["f64" if prec == "fp64" else "f32"])
+            out[prec]["launch_roofline_ms"] = round(roof, 3)
+            out[prec]["frac_of_launch_rooflines"] = round(roof / ms, 4)
         plan.close()
     del items, items64
     torch.cuda.empty_cache()
