@@ -133,7 +133,7 @@ def test_modes_without_factor_are_copies(gpu):
 
 def _short_row_items(torch, dev, rng, dtype):
     """Chains whose stages the short-row kernel can take (contractions
-    4..256, ranks 1..64): orders 2 and 3, every layout, contraction lengths
+    4..256, ranks 1..64): orders 2 to 5, every layout, contraction lengths
     at 4, 5, 15..17, 255, 256 and random, rank classes 16 / 32 / 64 at their
     edges, single columns, and 200 small tensors (more problems than
     workgroups, so workgroup ranges cross many problem boundaries)."""
@@ -142,6 +142,9 @@ def _short_row_items(torch, dev, rng, dtype):
     for _ in range(200):
         p = int(rng.integers(2, 4))
         specs.append(([int(x) for x in rng.integers(4, 41 if p == 3 else 257, size=p)], int(rng.integers(1, 65))))
+    for _ in range(16):  # orders 4 and 5: longer chains, every stage on the short-row kernels
+        p = int(rng.integers(4, 6))
+        specs.append(([int(x) for x in rng.integers(4, 13, size=p)], int(rng.integers(1, 65))))
     items, refs = [], []
     for na, rmax in specs:
         p = len(na)
