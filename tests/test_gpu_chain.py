@@ -144,7 +144,7 @@ def _short_row_items(torch, dev, rng, dtype):
         specs.append(([int(x) for x in rng.integers(4, 41 if p == 3 else 257, size=p)], int(rng.integers(1, 65))))
     for _ in range(16):  # orders 4 and 5: longer chains, every stage on the short-row kernels
         p = int(rng.integers(4, 6))
-        specs.append(([int(x) for x in rng.integers(4, 13, size=p)], int(rng.integers(1, 65))))
+        specs.append(([int(x) for x in rng.integers(4, 13, size=p)], int(rng.integers(1, 17))))
     items, refs = [], []
     for na, rmax in specs:
         p = len(na)
