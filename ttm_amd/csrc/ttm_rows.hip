@@ -668,35 +668,56 @@ void rows_layout(std::vector<RowsProblem> &probs, int height, unsigned grid, Row
                                                : 19.66;
     double const esz = kind == ROWS_F64 ? 8.0 : 4.0;
     double const grp = kind == ROWS_F64 ? 8.0 : kind == ROWS_F32 ? 16.0 : 32.0; // contraction group
-    std::vector<double> unit_cost;
-    std::vector<unsigned> unit_prob;
+    // every unit of a problem but its last (fewer columns) costs the same, so
+    // the walk below takes whole runs of units at once: O(problems + grid)
+    auto cost = [&](const RowsProblem &P, double cols) {
+        double const padded = grp * std::ceil(P.nq / grp);
+        double const bytes = cols * esz * (P.nq + P.m);
+        double const flops = 2.0 * 16.0 * padded * height;
+        return std::max(bytes, flops / ridge) + 2048.0;
+    };
+    std::vector<double> full(probs.size()), last(probs.size());
     unsigned begin = 0;
+    double total = 0;
     for (size_t pi = 0; pi < probs.size(); ++pi) {
         RowsProblem &P = probs[pi];
         P.unit_begin = begin;
         P.nunits = (unsigned)((P.ncols + 15) / 16);
         begin += P.nunits;
-        double const padded = grp * std::ceil(P.nq / grp);
-        for (unsigned u = 0; u < P.nunits; ++u) {
-            double const cols = (double)std::min<uint64_t>(16, P.ncols - (uint64_t)u * 16);
-            double const bytes = cols * esz * (P.nq + P.m);
-            double const flops = 2.0 * 16.0 * padded * height;
-            unit_cost.push_back(std::max(bytes, flops / ridge) + 2048.0);
-            unit_prob.push_back((unsigned)pi);
-        }
+        full[pi] = cost(P, 16.0);
+        last[pi] = cost(P, (double)(P.ncols - (uint64_t)(P.nunits - 1) * 16));
+        total += full[pi] * (P.nunits - 1) + last[pi];
     }
     out.units = begin;
-    double total = 0;
-    for (double c : unit_cost) total += c;
     out.wg_units.assign(grid + 1, begin);
     out.wg_prob.assign(grid, 0);
+    // workgroup b takes units while the running cost up to the middle of the
+    // next unit stays within b + 1 grid-ths of the total
     double acc = 0;
     unsigned u = 0;
+    size_t pi = 0;
     for (unsigned b = 0; b < grid; ++b) {
+        while (pi < probs.size() && u >= probs[pi].unit_begin + probs[pi].nunits) ++pi;
         out.wg_units[b] = u;
-        out.wg_prob[b] = u < begin ? unit_prob[u] : 0;
+        out.wg_prob[b] = pi < probs.size() ? (unsigned)pi : 0u;
         double const target = total * (b + 1) / grid;
-        while (u < begin && acc + 0.5 * unit_cost[u] <= target) acc += unit_cost[u++];
+        while (pi < probs.size()) {
+            RowsProblem const &P = probs[pi];
+            unsigned const lastu = P.unit_begin + P.nunits - 1;
+            if (u < lastu) {
+                // whole units at the full cost: n with acc + (n - 1/2) full <= target
+                double const room = (target - acc) / full[pi] + 0.5;
+                if (room < 1.0) break;
+                unsigned const n = (unsigned)std::min<double>(std::floor(room), (double)(lastu - u));
+                acc += n * full[pi];
+                u += n;
+                if (u < lastu) break; // the budget ends inside this problem
+            }
+            if (acc + 0.5 * last[pi] > target) break;
+            acc += last[pi];
+            u = lastu + 1;
+            ++pi;
+        }
     }
     out.wg_units[grid] = begin;
 }
