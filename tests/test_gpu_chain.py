@@ -240,6 +240,47 @@ def test_short_row_split_precisions_far_from_one(gpu, prec):
     plan.close()
 
 
+@pytest.mark.parametrize("kind", ["f32", "bf16x3", "fp16x2", "f64"])
+def test_mixed_kernel_batch(gpu, kind):
+    """One plan whose stages mix the short-row kernels and the tiles: rank
+    classes above 64 and contractions above 256 or below 4 (tiles) next to
+    eligible ones, orders 1 to 4, modes without a factor (copy stages), every
+    layout; against the float64 definition at the float32 (float64) bar."""
+    import torch
+    from ttm_amd.chain import ChainPlan
+    rng = np.random.default_rng(53)
+    dtype = torch.float64 if kind == "f64" else torch.float32
+    prec = kind if kind in ("bf16x3", "fp16x2") else None
+    items, refs = [], []
+    for t in range(48):
+        p = int(rng.integers(1, 5))
+        hi = {1: 600, 2: 300, 3: 40, 4: 12}[p]
+        na = [int(x) for x in rng.integers(1, hi + 1, size=p)]
+        pi = [int(x) + 1 for x in rng.permutation(p)]
+        A = rng.uniform(-1, 1, size=na)
+        Us = []
+        for k in range(p):
+            if rng.random() < 0.15:
+                Us.append(None)
+            else:
+                r = int(rng.integers(1, 101)) if p <= 2 else int(rng.integers(1, 17))
+                Us.append(rng.uniform(-1, 1, size=(r, na[k])))
+        nc = [U.shape[0] if U is not None else n for U, n in zip(Us, na)]
+        At = to_layout(torch, gpu, A, pi, dtype)
+        Ut = [torch.from_numpy(U).to(gpu, dtype).contiguous() if U is not None else None for U in Us]
+        Ct = to_layout(torch, gpu, np.full(nc, np.nan), pi, dtype)
+        items.append((At, Ut, Ct, pi))
+        refs.append((A, Us, einsum_chain(A, Us)))
+    plan = ChainPlan(items, precision=prec)
+    kernels = plan.stage_kernels
+    _say(kind, "mixed stage kernels", kernels)
+    assert any(k.startswith("k_rows_") for k in kernels)
+    assert any(not k.startswith("k_rows_") and not k.startswith("k_transpose") for k in kernels)
+    plan.run_graph()
+    check(items, refs, dtype, f"mixed {kind}")
+    plan.close()
+
+
 def test_graph_replay_matches_eager(gpu):
     """Replays of the captured chain, with eager work in between, equal the eager run."""
     import torch

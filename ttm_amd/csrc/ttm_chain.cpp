@@ -360,13 +360,16 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
 
     // ---- stages ---------------------------------------------------------------
     // Stage s of every chain, split by row class of m (<= 16, <= 32, <= 64,
-    // larger) so that each launch gets a tile height that fits its ranks,
-    // plus one copy launch for the modes without a factor; launches of one
-    // stage are independent, stages run in order.
+    // larger) so that each launch gets a tile height that fits its ranks, and
+    // within a class into the problems the short-row kernels can take (4 <=
+    // n_q <= 256) and the rest, plus one copy launch for the modes without a
+    // factor; launches of one stage are independent, stages run in order.
     // (with the short-row kernels switched off, the reduced precisions keep
     // the two tile heights of their grouped kernel)
     int const classes = lowp_group_precision(precision) && !rows_enabled() ? 2 : 4;
-    int const groups_per_stage = classes + 1;
+    // per stage: the rank classes of the problems the short-row kernels can
+    // take, the same classes of the rest, the copy launch
+    int const groups_per_stage = 2 * classes + 1;
     auto m_class = [&](uint64_t m) {
         if (classes == 2) return m <= 64 ? 0 : 1; // the two reduced-precision tile heights
         return m <= 16 ? 0 : m <= 32 ? 1 : m <= 64 ? 2 : 3;
@@ -399,7 +402,9 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
             bool const b_al = (reinterpret_cast<uintptr_t>(P.b) & 15u) == 0;
             P.flags = ((a_al && P.nq % vec == 0) ? TILE_A_VEC : 0u) | ((c_al && ncols % vec == 0) ? TILE_C_VEC : 0u) |
                       (b_al ? TILE_B_VEC : 0u);
-            size_t const g = (size_t)s * groups_per_stage + (d.u[mode] ? m_class(P.m) : classes);
+            bool const short_rows = rows_enabled() && rows_eligible(P.nq, P.m, P.ncols);
+            size_t const g = (size_t)s * groups_per_stage +
+                             (d.u[mode] ? (size_t)m_class(P.m) + (short_rows ? 0 : classes) : 2 * (size_t)classes);
             groups[g].transpose = !d.u[mode];
             groups[g].probs.push_back(P);
             stage_max_m[g] = std::max(stage_max_m[g], P.m);
@@ -422,7 +427,7 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
             std::ostringstream key;
             key << (is_double ? "f64" : "f32") << "/p" << (int)precision << "/m" << log2_class(stage_max_m[g])
                 << "/nq" << log2_class(stage_nq[g]) << "/cols" << log2_class(stage_cols[g]) << "/n"
-                << log2_class(st.probs.size());
+                << log2_class(st.probs.size()) << "/r" << (g % groups_per_stage < (size_t)classes ? 1 : 0);
             st.key = key.str();
             st.cfg = default_config(stage_max_m[g], lowp_group_precision(precision));
             if (rows_enabled()) {
