@@ -368,23 +368,29 @@ def tucker_items(torch, dev, count, seed=0x77140005):
     return items, ranks, flops, compulsory
 
 
-def tucker_launch_roofline(ranks, extents, esz, peak):
+def tucker_launch_roofline(ranks, extents, esz, peak, cus):
     """Sum over the launches of a chain plan (stage s of every tensor, split by
-    rank class m <= 16 / 32 / 64 / larger) of each launch's own bound:
-    max(HBM bytes / 8 TB/s, matrix flops / peak), the ranks padded to the
-    class height and the contraction to whole groups of 16 (the short-row
-    kernels' and the tiles' padding).  The plan cannot beat this sum (its
-    launches run one after the other within a stage class and stage by stage)."""
+    rank class m <= 16 / 32 / 48 / 64 / larger; ranks 33..48 share the 64-row
+    launch in stages where the ranks 33..64 have fewer than 64 units of 16
+    columns per compute unit) of each launch's own bound: max(HBM bytes /
+    8 TB/s, matrix flops / peak), the ranks padded to the class height and the
+    contraction to whole groups of 16 (the short-row kernels' and the tiles'
+    padding).  The plan cannot beat this sum (its launches run one after the
+    other within a stage class and stage by stage)."""
     total = 0.0
     for s in range(3):
-        cls = {}
+        probs = []
         for rk, na in zip(ranks, extents):
             cur = list(na)
             for k in range(s):
                 cur[k] = rk[k]
             nq, m = na[s], rk[s]
-            ncols = int(np.prod(cur)) // nq
-            h = 16 if m <= 16 else 32 if m <= 32 else 64 if m <= 64 else 128 * ((m + 127) // 128)
+            probs.append((nq, m, int(np.prod(cur)) // nq))
+        split48 = sum((nc + 15) // 16 for _, m, nc in probs if 32 < m <= 64) >= 64 * cus
+        cls = {}
+        for nq, m, ncols in probs:
+            h = (16 if m <= 16 else 32 if m <= 32 else 48 if m <= 48 and split48 else 64 if m <= 64
+                 else 128 * ((m + 127) // 128))
             b, f = cls.get(h, (0.0, 0.0))
             cls[h] = (b + esz * (ncols * nq + ncols * m + m * nq),
                       f + 2.0 * h * (16 * ((nq + 15) // 16)) * ncols)
@@ -427,7 +433,8 @@ def tucker_batch(args):
                      "stage_kernels": plan.stage_kernels}
         if prec in ("fp32", "fp64"):
             roof = tucker_launch_roofline(ranks, extents, 8 if prec == "fp64" else 4,
-                                          PEAK_FLOPS["f64" if prec == "fp64" else "f32"])
+                                          PEAK_FLOPS["f64" if prec == "fp64" else "f32"],
+                                          torch.cuda.get_device_properties(dev).multi_processor_count)
             out[prec]["launch_roofline_ms"] = round(roof, 3)
             out[prec]["frac_of_launch_rooflines"] = round(roof / ms, 4)
         plan.close()

@@ -25,7 +25,7 @@ int main()
             p.nq = 4 + rg() % 253;
             p.m = 1 + rg() % 64;
         }
-        int const height = (int[]){16, 32, 64}[rg() % 3];
+        int const height = (int[]){16, 32, 48, 64}[rg() % 4];
         RowsKind const kind = (RowsKind[]){ROWS_F32, ROWS_BF16X3, ROWS_F64}[rg() % 3];
         unsigned const grid = (unsigned[]){1, 7, 256, 512}[rg() % 4];
         RowsLayout L;

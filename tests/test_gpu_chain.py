@@ -207,6 +207,48 @@ def test_short_row_kernel_matches_the_definition(gpu, monkeypatch, kind):
     plan.close()
 
 
+@pytest.mark.parametrize("kind", ["f32", "bf16x3", "fp16x2", "f64"])
+def test_short_row_48_and_64_row_launches(gpu, kind):
+    """A first stage big enough (24 tensors 40 x 128 x 128, 64 units of 16
+    columns per compute unit and more) that ranks 33..48 and 49..64 take two
+    launches, the 48-row and the 64-row short-row kernel; the later, smaller
+    stages keep one launch per class.  Against the float64 definition at the
+    float32 (float64) bar, eagerly and as a graph replay."""
+    import torch
+    from ttm_amd.chain import ChainPlan
+    rng = np.random.default_rng(53)
+    dtype = torch.float64 if kind == "f64" else torch.float32
+    prec = kind if kind in ("bf16x3", "fp16x2") else None
+    items, refs = [], []
+    for t in range(24):
+        na = [40, 128, 128]
+        r1 = int(rng.integers(33, 49)) if t % 2 == 0 else int(rng.integers(49, 65))
+        ranks = [r1, int(rng.integers(4, 17)), int(rng.integers(4, 17))]
+        A = rng.uniform(-1, 1, size=na)
+        Us = [rng.uniform(-1, 1, size=(r, n)) for r, n in zip(ranks, na)]
+        pi = [1, 2, 3]
+        At = to_layout(torch, gpu, A, pi, dtype)
+        Ut = [torch.from_numpy(U).to(gpu, dtype).contiguous() for U in Us]
+        Ct = to_layout(torch, gpu, np.full(ranks, np.nan), pi, dtype)
+        items.append((At, Ut, Ct, pi))
+        refs.append((A, Us, einsum_chain(A, Us)))
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    assert 24 * (128 * 128 // 16) >= 64 * cus
+    name = {"f32": "k_rows_grouped<", "bf16x3": "k_rows_bf16x3<", "fp16x2": "k_rows_fp16x2<", "f64": "k_rows_f64<"}[kind]
+    plan = ChainPlan(items, precision=prec)
+    kernels = plan.stage_kernels
+    _say(kind, "48/64-row stage kernels", kernels)
+    assert any(k.startswith(name + "48,") for k in kernels)
+    assert any(k.startswith(name + "64,") for k in kernels)
+    plan.run()
+    check(items, refs, dtype, f"48/64-row launches {kind}")
+    for it in items:
+        it[2].fill_(float("nan"))
+    plan.run_graph()
+    check(items, refs, dtype, f"48/64-row launches {kind} graph")
+    plan.close()
+
+
 @pytest.mark.parametrize("prec", ["bf16x3", "fp16x2"])
 def test_short_row_split_precisions_far_from_one(gpu, prec):
     """The split short-row kernels on tensors and factors scaled by powers of

@@ -628,7 +628,7 @@ def rows_unit(chk, A, B, nq, m, ncols, u, MB):
     return D
 
 
-@pytest.mark.parametrize("MB", [16, 32, 64])
+@pytest.mark.parametrize("MB", [16, 32, 48, 64])
 def test_k_rows_grouped_unit_model(MB):
     """The short-row kernel's lane arithmetic on the CPU: loads stay inside
     A (the last group of the last row clamps to A's end and shifts), the
@@ -710,7 +710,7 @@ def rows_unit_bf16x3(chk, A, B, nq, m, ncols, u, MB):
     return D
 
 
-@pytest.mark.parametrize("MB", [16, 32, 64])
+@pytest.mark.parametrize("MB", [16, 32, 48, 64])
 def test_k_rows_bf16x3_unit_model(MB):
     """The BF16X3 short-row kernel's lane arithmetic (32-index groups, two
     clamped pieces in the last group, three B planes) on the CPU: in-bounds
@@ -789,7 +789,7 @@ def rows_unit_f64(chk, A, B, nq, m, ncols, u, MB):
     return D
 
 
-@pytest.mark.parametrize("MB", [16, 32, 64])
+@pytest.mark.parametrize("MB", [16, 32, 48, 64])
 def test_k_rows_f64_unit_model(MB):
     """The float64 short-row kernel's lane arithmetic on the CPU (8-index
     groups, a clamped 2-element last piece, B stride 8 k + 2): in-bounds

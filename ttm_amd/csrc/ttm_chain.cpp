@@ -359,20 +359,20 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
     char *sbase = static_cast<char *>(plan->d_static);
 
     // ---- stages ---------------------------------------------------------------
-    // Stage s of every chain, split by row class of m (<= 16, <= 32, <= 64,
-    // larger) so that each launch gets a tile height that fits its ranks, and
-    // within a class into the problems the short-row kernels can take (4 <=
-    // n_q <= 256) and the rest, plus one copy launch for the modes without a
-    // factor; launches of one stage are independent, stages run in order.
-    // (with the short-row kernels switched off, the reduced precisions keep
-    // the two tile heights of their grouped kernel)
-    int const classes = lowp_group_precision(precision) && !rows_enabled() ? 2 : 4;
+    // Stage s of every chain, split by row class of m (<= 16, <= 32, <= 48,
+    // <= 64, larger) so that each launch gets a tile height that fits its
+    // ranks, and within a class into the problems the short-row kernels can
+    // take (4 <= n_q <= 256) and the rest, plus one copy launch for the modes
+    // without a factor; launches of one stage are independent, stages run in
+    // order.  (with the short-row kernels switched off, the reduced precisions
+    // keep the two tile heights of their grouped kernel)
+    int const classes = lowp_group_precision(precision) && !rows_enabled() ? 2 : 5;
     // per stage: the rank classes of the problems the short-row kernels can
     // take, the same classes of the rest, the copy launch
     int const groups_per_stage = 2 * classes + 1;
     auto m_class = [&](uint64_t m) {
         if (classes == 2) return m <= 64 ? 0 : 1; // the two reduced-precision tile heights
-        return m <= 16 ? 0 : m <= 32 ? 1 : m <= 64 ? 2 : 3;
+        return m <= 16 ? 0 : m <= 32 ? 1 : m <= 48 ? 2 : m <= 64 ? 3 : 4;
     };
     std::vector<Stage> groups(max_p * groups_per_stage);
     std::vector<uint64_t> stage_max_m(groups.size(), 0), stage_cols(groups.size(), 0), stage_nq(groups.size(), 0);
@@ -414,6 +414,30 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
             in = out;
         }
     }
+    // Ranks 33..48 keep a launch of their own (the 48-row short-row kernels:
+    // a quarter less matrix work than on 64 rows) only where the stage gives
+    // the two launches enough work, at least 64 units of 16 columns per
+    // compute unit together; in smaller stages the second launch's start and
+    // tail cost more than the rows save.  The tiles have one 64-row height
+    // for both classes, so there the two are always one launch.
+    if (classes == 5) {
+        uint64_t const min_units = 64ull * (uint64_t)std::max(plan->num_cu, 1);
+        for (size_t s = 0; s < max_p; ++s)
+            for (int rest = 0; rest < 2; ++rest) {
+                size_t const g48 = s * groups_per_stage + (size_t)(rest * classes) + 2, g64 = g48 + 1;
+                if (groups[g48].probs.empty()) continue;
+                uint64_t units = 0;
+                for (size_t g : {g48, g64})
+                    for (GroupProblem const &P : groups[g].probs) units += (P.ncols + 15) / 16;
+                if (rest == 0 && units >= min_units) continue;
+                Stage &lo = groups[g48], &hi = groups[g64];
+                hi.probs.insert(hi.probs.end(), lo.probs.begin(), lo.probs.end());
+                lo.probs.clear();
+                stage_max_m[g64] = std::max(stage_max_m[g64], stage_max_m[g48]);
+                stage_cols[g64] += stage_cols[g48];
+                stage_nq[g64] = std::max(stage_nq[g64], stage_nq[g48]);
+            }
+    }
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         tune_load_locked();
@@ -427,7 +451,8 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
             std::ostringstream key;
             key << (is_double ? "f64" : "f32") << "/p" << (int)precision << "/m" << log2_class(stage_max_m[g])
                 << "/nq" << log2_class(stage_nq[g]) << "/cols" << log2_class(stage_cols[g]) << "/n"
-                << log2_class(st.probs.size()) << "/r" << (g % groups_per_stage < (size_t)classes ? 1 : 0);
+                << log2_class(st.probs.size()) << "/r"
+                << (g % groups_per_stage < (size_t)classes ? rows_height(stage_max_m[g]) : 0);
             st.key = key.str();
             st.cfg = default_config(stage_max_m[g], lowp_group_precision(precision));
             if (rows_enabled()) {

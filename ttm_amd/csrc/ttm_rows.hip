@@ -630,7 +630,7 @@ constexpr RowsCfg rows_cfg(int height, RowsKind kind)
 
 } // namespace
 
-int rows_height(uint64_t m) { return m <= 16 ? 16 : m <= 32 ? 32 : m <= 64 ? 64 : 0; }
+int rows_height(uint64_t m) { return m <= 16 ? 16 : m <= 32 ? 32 : m <= 48 ? 48 : m <= 64 ? 64 : 0; }
 
 bool rows_eligible(uint64_t nq, uint64_t m, uint64_t ncols)
 {
@@ -743,6 +743,7 @@ bool launch_kind(int height, const RowsProblem *d_probs, const RowsSched &sc, un
     switch (height) {
     case 16: launch_one<K, 16>(d_probs, sc, grid, s); return true;
     case 32: launch_one<K, 32>(d_probs, sc, grid, s); return true;
+    case 48: launch_one<K, 48>(d_probs, sc, grid, s); return true;
     case 64: launch_one<K, 64>(d_probs, sc, grid, s); return true;
     default: return false;
     }

@@ -47,7 +47,7 @@ enum RowsKind : int
 constexpr unsigned kRowsMinNq = 4;
 constexpr unsigned kRowsMaxNq = 256;
 
-// Row heights the kernels are compiled for (16, 32, 64); 0 when m > 64.
+// Row heights the kernels are compiled for (16, 32, 48, 64); 0 when m > 64.
 int rows_height(uint64_t m);
 
 // A problem the kernels can run: 4 <= nq <= 256, 1 <= m <= 64.
