@@ -134,7 +134,7 @@ def test_modes_without_factor_are_copies(gpu):
 def _short_row_items(torch, dev, rng, dtype):
     """Chains whose stages the short-row kernel can take (contractions
     4..256, ranks 1..64): orders 2 to 5, every layout, contraction lengths
-    at 4, 5, 15..17, 255, 256 and random, rank classes 16 / 32 / 64 at their
+    at 4, 5, 15..17, 255, 256 and random, rank classes 16 / 32 / 48 / 64 at their
     edges, single columns, and 200 small tensors (more problems than
     workgroups, so workgroup ranges cross many problem boundaries)."""
     specs = [([4, 256], 1), ([256, 5], 64), ([17, 15, 16], 33), ([255, 7, 9], 16), ([64, 65, 4], 17),
@@ -399,27 +399,30 @@ def _tensordot_chain(A, Us):
     return X
 
 
-@pytest.mark.parametrize("prec", [None, "bf16x3", "fp16x2"])
+@pytest.mark.parametrize("prec", [None, "bf16x3", "fp16x2", "f64"])
 def test_tucker_batch_config5_full_size(gpu, prec):
     """The Tucker batch of the benchmark at its stated size: 256 first-order
     float32 tensors, extents U{16..256}^3, ranks U{4..64}, orthonormal-row
     factors (bench.tucker_items, the same seed), as one hipGraph replay of a
-    plan (tuned tiles for FP32), every core against a float64 tensordot
-    chain on the host."""
+    plan (tuned for FP32), every core against a float64 tensordot chain on
+    the host; "f64" runs the same tensors and factors converted to float64
+    (the bench's FP64 leg), tuned, at the float64 bar."""
     import torch
     import bench
     from ttm_amd.chain import ChainPlan
     items, ranks, flops, _ = bench.tucker_items(torch, gpu, 256)
+    if prec == "f64":
+        items = [(A.double(), [U.double() for U in Us], C.double()) for A, Us, C in items]
     _say(f"config 5 {prec or 'fp32'}: {len(items)} tensors, {flops / 1e9:.1f} GFLOP")
-    plan = ChainPlan(items, precision=prec)
-    if prec is None:
+    plan = ChainPlan(items, precision=prec if prec != "f64" else None)
+    if prec in (None, "f64"):
         plan.tune()
     for it in items:
         it[2].fill_(float("nan"))
     plan.run_graph()
     torch.cuda.synchronize()
     _say("stage kernels", plan.stage_kernels)
-    eps = np.finfo(np.float32).eps
+    eps = np.finfo(np.float64 if prec == "f64" else np.float32).eps
     worst = 0.0
     for t, (A, Us, C) in enumerate(items):
         a = A.double().cpu().numpy()
@@ -427,7 +430,7 @@ def test_tucker_batch_config5_full_size(gpu, prec):
         ref = _tensordot_chain(a, us)
         got = C.double().cpu().numpy()
         err = float(np.linalg.norm(got - ref))
-        if prec is None:
+        if prec in (None, "f64"):
             nqs = sum(a.shape) + sum(a.shape)
             scale = np.linalg.norm(a) * np.prod([np.linalg.norm(U) for U in us])
             bound = 4 * nqs * eps * scale
