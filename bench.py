@@ -372,11 +372,13 @@ def tucker_launch_roofline(ranks, extents, esz, peak, cus):
     """Sum over the launches of a chain plan (stage s of every tensor, split by
     rank class m <= 16 / 32 / 48 / 64 / larger; ranks 33..48 share the 64-row
     launch in stages where the ranks 33..64 have fewer than 64 units of 16
-    columns per compute unit) of each launch's own bound: max(HBM bytes /
-    8 TB/s, matrix flops / peak), the ranks padded to the class height and the
-    contraction to whole groups of 16 (the short-row kernels' and the tiles'
-    padding).  The plan cannot beat this sum (its launches run one after the
-    other within a stage class and stage by stage)."""
+    columns per compute unit, and all ranks up to 64 share one launch in
+    stages where they have fewer than 128 units per compute unit) of each
+    launch's own bound: max(HBM bytes / 8 TB/s, matrix flops / peak), the
+    ranks padded to the launch's row height and the contraction to whole
+    groups of 16 (the short-row kernels' and the tiles' padding).  The plan
+    cannot beat this sum (its launches run one after the other within a
+    stage and stage by stage)."""
     total = 0.0
     for s in range(3):
         probs = []
@@ -387,10 +389,16 @@ def tucker_launch_roofline(ranks, extents, esz, peak, cus):
             nq, m = na[s], rk[s]
             probs.append((nq, m, int(np.prod(cur)) // nq))
         split48 = sum((nc + 15) // 16 for _, m, nc in probs if 32 < m <= 64) >= 64 * cus
+        small = sum((nc + 15) // 16 for _, m, nc in probs if m <= 64) < 128 * cus
+        top = max((m for _, m, _ in probs if m <= 64), default=0)
+        top_h = 16 if top <= 16 else 32 if top <= 32 else 48 if top <= 48 else 64
         cls = {}
         for nq, m, ncols in probs:
-            h = (16 if m <= 16 else 32 if m <= 32 else 48 if m <= 48 and split48 else 64 if m <= 64
-                 else 128 * ((m + 127) // 128))
+            if m <= 64 and small:
+                h = top_h
+            else:
+                h = (16 if m <= 16 else 32 if m <= 32 else 48 if m <= 48 and split48 else 64 if m <= 64
+                     else 128 * ((m + 127) // 128))
             b, f = cls.get(h, (0.0, 0.0))
             cls[h] = (b + esz * (ncols * nq + ncols * m + m * nq),
                       f + 2.0 * h * (16 * ((nq + 15) // 16)) * ncols)

@@ -262,7 +262,9 @@ ttm_status_t ttm_chain_run_graph(ttm_chain_plan_t plan, ttm_stream_t stream);
  * Stages whose contractions are 4..256 long and whose ranks are at most 64
  * run the short-row kernels by default (float32, BF16X3, FP16X2, float64)
  * (TTM_AMD_CHAIN_ROWS=0 at plan creation keeps them on the tiles); the
- * plain-precision tuner times the short-row kernel next to the tile shapes. */
+ * plain-precision tuner times the short-row kernel next to the tile shapes.
+ * Small stages run their rank classes up to 64 as one launch
+ * (TTM_AMD_CHAIN_MERGE=0 at plan creation keeps one launch per class). */
 ttm_status_t ttm_chain_tune(ttm_chain_plan_t plan, ttm_stream_t stream);
 /* Launches of the plan (each stage of the chains split by row class of the
  * ranks, in execution order) and the tile shape (kernel instance) of launch `s`. */

@@ -164,8 +164,10 @@ def _short_row_items(torch, dev, rng, dtype):
 def test_short_row_kernel_matches_the_definition(gpu, monkeypatch, kind):
     """Stages with contractions of 4..256 and ranks up to 64 run the
     short-row kernels (k_rows_grouped, k_rows_bf16x3, k_rows_fp16x2,
-    k_rows_f64) by default in float32, BF16X3, FP16X2 and float64; with
-    TTM_AMD_CHAIN_ROWS=0 the same plan runs the grouped tiles.  Both are
+    k_rows_f64) by default in float32, BF16X3, FP16X2 and float64, one
+    launch per stage for a batch this small; with TTM_AMD_CHAIN_MERGE=0 one
+    launch per rank class (every row height); with TTM_AMD_CHAIN_ROWS=0 the
+    same plan runs the grouped tiles.  Both are
     checked against the float64 definition (every output written: C starts
     as NaN), eagerly and as a graph replay, and against each other.  The
     split precisions are held to the float32 bar here (normwise error
@@ -179,10 +181,24 @@ def test_short_row_kernel_matches_the_definition(gpu, monkeypatch, kind):
     prec = kind if kind in ("bf16x3", "fp16x2") else None
     items, refs = _short_row_items(torch, gpu, rng, dtype)
     name = {"f32": "k_rows_grouped<", "bf16x3": "k_rows_bf16x3<", "fp16x2": "k_rows_fp16x2<", "f64": "k_rows_f64<"}[kind]
+    # by default every stage of this small batch is one launch on the
+    # tallest row height it needs
+    plan = ChainPlan(items, precision=prec)
+    _say("merged short-row stage kernels", plan.stage_kernels)
+    stages = max(len(it[0].shape) for it in items)
+    assert len(plan.stage_kernels) == stages and all(k.startswith(name) for k in plan.stage_kernels)
+    plan.run()
+    check(items, refs, dtype, "short rows, merged classes")
+    plan.close()
+    # every rank class in a launch of its own: each row height on the edge cases
+    monkeypatch.setenv("TTM_AMD_CHAIN_MERGE", "0")
+    for it in items:
+        it[2].fill_(float("nan"))
     plan = ChainPlan(items, precision=prec)
     kernels = plan.stage_kernels
     _say("short-row stage kernels", kernels)
-    assert sum(k.startswith(name) for k in kernels) >= 3
+    for h in (16, 32, 48, 64):
+        assert any(k.startswith(f"{name}{h},") for k in kernels), h
     plan.run()
     check(items, refs, dtype, "short rows")
     got_rows = [it[2].clone() for it in items]
@@ -209,10 +225,10 @@ def test_short_row_kernel_matches_the_definition(gpu, monkeypatch, kind):
 
 @pytest.mark.parametrize("kind", ["f32", "bf16x3", "fp16x2", "f64"])
 def test_short_row_48_and_64_row_launches(gpu, kind):
-    """A first stage big enough (24 tensors 40 x 128 x 128, 64 units of 16
+    """A first stage big enough (36 tensors 40 x 128 x 128: 128 units of 16
     columns per compute unit and more) that ranks 33..48 and 49..64 take two
     launches, the 48-row and the 64-row short-row kernel; the later, smaller
-    stages keep one launch per class.  Against the float64 definition at the
+    stages are one launch each.  Against the float64 definition at the
     float32 (float64) bar, eagerly and as a graph replay."""
     import torch
     from ttm_amd.chain import ChainPlan
@@ -220,7 +236,7 @@ def test_short_row_48_and_64_row_launches(gpu, kind):
     dtype = torch.float64 if kind == "f64" else torch.float32
     prec = kind if kind in ("bf16x3", "fp16x2") else None
     items, refs = [], []
-    for t in range(24):
+    for t in range(36):
         na = [40, 128, 128]
         r1 = int(rng.integers(33, 49)) if t % 2 == 0 else int(rng.integers(49, 65))
         ranks = [r1, int(rng.integers(4, 17)), int(rng.integers(4, 17))]
@@ -233,7 +249,7 @@ def test_short_row_48_and_64_row_launches(gpu, kind):
         items.append((At, Ut, Ct, pi))
         refs.append((A, Us, einsum_chain(A, Us)))
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    assert 24 * (128 * 128 // 16) >= 64 * cus
+    assert 36 * (128 * 128 // 16) >= 128 * cus
     name = {"f32": "k_rows_grouped<", "bf16x3": "k_rows_bf16x3<", "fp16x2": "k_rows_fp16x2<", "f64": "k_rows_f64<"}[kind]
     plan = ChainPlan(items, precision=prec)
     kernels = plan.stage_kernels

@@ -93,6 +93,14 @@ bool rows_enabled()
     return !(v && v[0] == '0');
 }
 
+// TTM_AMD_CHAIN_MERGE=0 keeps every rank class of a stage in a launch of its
+// own (no class is merged into a taller one).
+bool merge_enabled()
+{
+    const char *v = std::getenv("TTM_AMD_CHAIN_MERGE");
+    return !(v && v[0] == '0');
+}
+
 int log2_class(uint64_t x)
 {
     int c = 0;
@@ -419,24 +427,44 @@ ttm_status_t ttm_chain_plan_create(size_t count, const ttm_chain_desc_t *descs, 
     // the two launches enough work, at least 64 units of 16 columns per
     // compute unit together; in smaller stages the second launch's start and
     // tail cost more than the rows save.  The tiles have one 64-row height
-    // for both classes, so there the two are always one launch.
-    if (classes == 5) {
-        uint64_t const min_units = 64ull * (uint64_t)std::max(plan->num_cu, 1);
-        for (size_t s = 0; s < max_p; ++s)
-            for (int rest = 0; rest < 2; ++rest) {
-                size_t const g48 = s * groups_per_stage + (size_t)(rest * classes) + 2, g64 = g48 + 1;
-                if (groups[g48].probs.empty()) continue;
-                uint64_t units = 0;
-                for (size_t g : {g48, g64})
-                    for (GroupProblem const &P : groups[g].probs) units += (P.ncols + 15) / 16;
-                if (rest == 0 && units >= min_units) continue;
-                Stage &lo = groups[g48], &hi = groups[g64];
-                hi.probs.insert(hi.probs.end(), lo.probs.begin(), lo.probs.end());
-                lo.probs.clear();
-                stage_max_m[g64] = std::max(stage_max_m[g64], stage_max_m[g48]);
-                stage_cols[g64] += stage_cols[g48];
-                stage_nq[g64] = std::max(stage_nq[g64], stage_nq[g48]);
+    // for both classes, so there the two are always one launch.  A stage
+    // whose short-row problems of ranks up to 64 have fewer than 128 units
+    // per compute unit in all runs them as one launch on the tallest height
+    // it needs: each launch of so little work is mostly its start and its
+    // tail.
+    if (classes == 5 && merge_enabled()) {
+        uint64_t const cu = (uint64_t)std::max(plan->num_cu, 1);
+        auto units_of = [&](size_t g) {
+            uint64_t u = 0;
+            for (GroupProblem const &P : groups[g].probs) u += (P.ncols + 15) / 16;
+            return u;
+        };
+        auto merge = [&](size_t from, size_t into) {
+            Stage &lo = groups[from], &hi = groups[into];
+            hi.probs.insert(hi.probs.end(), lo.probs.begin(), lo.probs.end());
+            lo.probs.clear();
+            stage_max_m[into] = std::max(stage_max_m[into], stage_max_m[from]);
+            stage_cols[into] += stage_cols[from];
+            stage_nq[into] = std::max(stage_nq[into], stage_nq[from]);
+            stage_max_m[from] = stage_cols[from] = stage_nq[from] = 0;
+        };
+        for (size_t s = 0; s < max_p; ++s) {
+            size_t const base = s * groups_per_stage;
+            uint64_t all = 0;
+            size_t top = base; // the tallest non-empty short-row class up to 64 rows
+            for (size_t c = 0; c < 4; ++c) {
+                all += units_of(base + c);
+                if (!groups[base + c].probs.empty()) top = base + c;
             }
+            if (all < 128 * cu) {
+                for (size_t c = 0; c < 4; ++c)
+                    if (base + c != top && !groups[base + c].probs.empty()) merge(base + c, top);
+            } else if (!groups[base + 2].probs.empty() && units_of(base + 2) + units_of(base + 3) < 64 * cu) {
+                merge(base + 2, base + 3);
+            }
+            size_t const rest = base + (size_t)classes;
+            if (!groups[rest + 2].probs.empty()) merge(rest + 2, rest + 3);
+        }
     }
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
