@@ -315,3 +315,39 @@ def test_host_entry_reuses_pooled_device_buffers(gpu):
         ttm_amd.ttm_raw(q, 3, a, na, wa, pia, b, [m, na[q - 1]], [2, 1], c, nc, wc)
     _say(f"pool after the first call {first} B, after ten more {ttm_amd.workspace_bytes()} B")
     assert first > 0 and ttm_amd.workspace_bytes() == first
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_tn_tile_b_orientations_give_the_same_bits(gpu, oracle, dtype):
+    """TN products (q fastest) on the 128-row tile: a row-major and a
+    column-major B fill the tile's LDS with the same values (16-byte vectors
+    along l or along j), so the two orientations give the same bits; each
+    also against the FP64 brute force, with a split contraction (few column
+    blocks, long n_q), ragged edges (n_q, m and the column count off the tile
+    sizes) and an m that is not a multiple of the vector."""
+    import torch
+    import ttm_amd
+    rng = np.random.default_rng(71)
+    vec = 4 if dtype == np.float32 else 2
+    cases = [([1000, 300], [1, 2], 1, 132),        # ragged n_q, columns and m
+             ([8192, 40], [1, 2], 1, 256),         # 40 columns x 2 row blocks: contraction split
+             ([96, 7, 130], [1, 3, 2], 1, 1024),   # three modes, q fastest
+             ([200, 500], [1, 2], 1, 130 if vec == 4 else 131)]  # m % vec != 0
+    for na, pia, q, m in cases:
+        nq = na[q - 1]
+        a = rng.uniform(-1, 1, int(np.prod(na))).astype(dtype)
+        Bm = rng.uniform(-1, 1, (m, nq)).astype(dtype)
+        A = torch.from_numpy(a).to(gpu)
+        Bt = torch.from_numpy(Bm).to(gpu)
+        C_row, wa, nc, wc = _run(torch, ttm_amd, q, A, na, pia, Bt, False, dtype)
+        sym_row, splits = ttm_amd.last_kernel()
+        C_col, _, _, _ = _run(torch, ttm_amd, q, A, na, pia, Bt, True, dtype)
+        sym_col, _ = ttm_amd.last_kernel()
+        label = f"{np.dtype(dtype).name} na={na} q={q} m={m}"
+        _say(f"{label}: row-major B {sym_row} x{splits}, column-major B {sym_col}")
+        assert sym_row.startswith("k_gemm<") and ", 4, 4, 2, 16, true," in sym_row, (label, sym_row)
+        assert torch.equal(C_row, C_col), label
+        c64 = oracle.ttm_brute_f64(q, a, na, wa, Bm.reshape(-1).copy(), m, False, nc, wc)
+        err = rc.normwise_error(C_row.cpu().numpy(), c64, a, Bm.reshape(-1))
+        _say(f"{label}: normwise {err:.3e}")
+        assert err <= 4 * nq * rc.eps(dtype), (label, err)
