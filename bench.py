@@ -665,15 +665,20 @@ def cpu_baseline(args, budget_s):
         prepared.append((c, b, nc, orc_strides(ext, c["pia"]), orc_strides(nc, c["pia"])))
     flops_sweep = sum(2.0 * c["m"] * int(np.prod(ext)) for c, *_ in prepared)
 
+    case_s = [0.0] * len(prepared)
+
     def sweep():
         branches = []
-        for c, b, nc, wa, wc in prepared:
+        for i, (c, b, nc, wa, wc) in enumerate(prepared):
+            t0 = time.perf_counter()
             _, br = orc.ttm_reference(c["q"], data[c["dtype"]], ext, wa, c["pia"], b, [c["m"], ext[c["q"] - 1]],
                                       [2, 1], nc, wc, use_blas=True, cores=threads)
+            case_s[i] += time.perf_counter() - t0
             branches.append(br)
         return branches
 
     branches = sweep()  # warm-up
+    case_s[:] = [0.0] * len(prepared)
     # each timed sweep follows an untimed 32 MiB scrub of the caches, as the
     # reference's measurement loop does (example/measure.cpp:85-100)
     scrub = np.empty(8 << 20, dtype=np.float32)
@@ -687,6 +692,33 @@ def cpu_baseline(args, budget_s):
         if time.perf_counter() - t_start >= budget_s:
             break
     dt = float(np.mean(times))
+    per_case = []
+    for (c, *_), s in zip(prepared, case_s):
+        ms = s / len(times) * 1e3
+        per_case.append({"label": c["label"], "ms": round(ms, 4),
+                         "tflops": round(2.0 * c["m"] * int(np.prod(ext)) / (ms * 1e-3) / 1e12, 4)})
+    # the 64x48x32 products at their own size (q = 2, m = 40, both B layouts):
+    # microseconds per call through the oracle's C entry (ctypes call included)
+    config1 = []
+    n1, m1, q1 = [64, 48, 32], 40, 2
+    for dtn, npdt in (("f32", np.float32), ("f64", np.float64)):
+        a1 = rng.uniform(-1, 1, size=int(np.prod(n1))).astype(npdt)
+        bm = rng.uniform(-1, 1, size=(m1, n1[q1 - 1])).astype(npdt)
+        nc1 = [64, m1, 32]
+        for lay, pib, bflat in (("row", [2, 1], bm.reshape(-1).copy()), ("col", [1, 2], bm.T.reshape(-1).copy())):
+            row = {"dtype": dtn, "b": lay}
+            for key, nth in (("us_per_call", threads), ("us_per_call_1_thread", 1)):
+                def call():
+                    orc.ttm_reference(q1, a1, n1, [1, 64, 64 * 48], [1, 2, 3], bflat, [m1, n1[q1 - 1]], pib,
+                                      nc1, [1, 64, 64 * m1], use_blas=True, cores=nth)
+                for _ in range(20):
+                    call()
+                reps = 400
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    call()
+                row[key] = round((time.perf_counter() - t0) / reps * 1e6, 2)
+            config1.append(row)
     return {
         "value": flops_sweep / dt / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
         "sample": (f"the same 24-product sweep (f32+f64, first/last-order, q=1..3, m in {{16,1024}}) on A "
@@ -696,7 +728,7 @@ def cpu_baseline(args, budget_s):
         "median_value": flops_sweep / float(np.median(times)) / 1e12,
         "blas": os.path.basename(blas), "branches": sorted(set(branches)),
         "host_cores_reference_count": orc.cores(), "hardware_concurrency": os.cpu_count(),
-        "cpus_available": avail,
+        "cpus_available": avail, "cases": per_case, "config1_latency": config1,
     }
 
 
