@@ -163,14 +163,18 @@ def test_random_shapes_all_families_vs_fp64(gpu, oracle, dtype):
                     assert e <= bound, (na, pia, q, m, colmajor, fam, offset, e)
 
 
-def test_order5_random_layout_all_modes(gpu, oracle):
-    """A 32x24x40x16x48 float32, seeded random layout, q = 1..5, B 64 x n_q: sampled entries
-    against the float64 definition (the full oracle is too slow at this size)."""
+@pytest.mark.parametrize("layout", ["seeded", "bench", "first"])
+def test_order5_random_layout_all_modes(gpu, oracle, layout):
+    """A 32x24x40x16x48 float32, q = 1..5, B 64 x n_q, in a seeded random
+    layout, in the layout the benchmark times (4, 2, 3, 1, 5) and in the
+    first-order layout: sampled entries against the float64 definition (the
+    full oracle is too slow at this size)."""
     import torch
     import ttm_amd
     na = [32, 24, 40, 16, 48]
     rng = np.random.default_rng(0x77140003)
-    pia = [int(x) + 1 for x in rng.permutation(5)]
+    pia = {"seeded": [int(x) + 1 for x in rng.permutation(5)], "bench": [4, 2, 3, 1, 5],
+           "first": [1, 2, 3, 4, 5]}[layout]
     A = torch.empty(int(np.prod(na)), dtype=torch.float32, device=gpu).uniform_(-1, 1)
     wa = rc.dense_strides(na, pia)
     for q in range(1, 6):
