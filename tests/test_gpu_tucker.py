@@ -66,3 +66,29 @@ def test_hosvd_batch_uneven_layouts(gpu):
     for R, A in zip(recs, refs):
         err = np.linalg.norm(R.cpu().numpy() - A) / np.linalg.norm(A)
         assert err <= 1e-10, err
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_gram_matrices_are_this_librarys_ttm(gpu, dtype):
+    """The HOSVD's Gram matrices A_(k) A_(k)^T come from this library's TTM
+    (G = X x_2 X with X = A_(k)): equal to the float64 product of the
+    unfolding with its transpose on the host, for every mode of tensors in
+    several layouts, one with a long unfolding (the contraction split)."""
+    import torch
+    import ttm_amd
+    from ttm_amd import tucker
+    rng = np.random.default_rng(81)
+    tdt = torch.float32 if dtype == "f32" else torch.float64
+    for n, pi in (([40, 30, 20], [1, 2, 3]), ([17, 64, 9], [3, 1, 2]), ([256, 256, 64], [1, 2, 3])):
+        X = rng.uniform(-1, 1, size=n)
+        A = to_layout(torch, gpu, X, pi, tdt)
+        Xq = A.double().cpu().numpy()   # the values the GPU sees (float32-rounded for f32)
+        for k in range(3):
+            G = tucker._gram(A, k)
+            torch.cuda.synchronize()
+            sym, splits = ttm_amd.last_kernel()
+            U = np.moveaxis(Xq, k, 0).reshape(n[k], -1)
+            ref = U @ U.T
+            err = float(np.linalg.norm(G.cpu().numpy() - ref) / np.linalg.norm(ref))
+            print(f"{n} pi={pi} mode {k + 1}: {sym} x{splits}, relative error {err:.2e}", flush=True)
+            assert err <= 1e-13, (n, pi, k, err)

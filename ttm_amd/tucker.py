@@ -5,14 +5,17 @@ The caller one step above the chain engine: for each tensor A and ranks
 
 * factor k = the r_k leading left singular vectors of the mode-k unfolding
   A_(k), taken as the top eigenvectors of the Gram matrix A_(k) A_(k)^T
-  (n_k x n_k, float64, on the device);
+  (n_k x n_k, float64, on the device).  The Gram matrix is itself a TTM of
+  this library: with X = A_(k) as a two-way tensor (a float64 copy, mode k
+  slowest), G = X x_2 X contracts X's long mode with the rows of X (the
+  split-contraction tile, n_k * N_A multiply-adds per mode);
 * core = A x_1 U_1^T x_2 U_2^T ... x_p U_p^T, the multi-TTM chain with factors
   of orthonormal rows -- every core of a batch comes from one chain plan
   (one persistent launch per stage, hipGraph replay);
 * ``reconstruct`` multiplies a core back by the factors (again one plan).
 
-The Gram products and the eigendecompositions are small (n_k <= a few
-hundred) library calls; the tensor-sized work is the chains.
+The eigendecompositions (n_k x n_k, n_k at most a few hundred) are torch
+calls; every product over the tensor is this library's.
 """
 from __future__ import annotations
 
@@ -20,22 +23,30 @@ from .chain import ChainPlan
 
 
 def _gram(A, k):
-    """A_(k) A_(k)^T in float64 (the mode-k unfolding times its transpose)."""
+    """A_(k) A_(k)^T in float64: X = A_(k) (n_k x M, M contiguous) as a
+    two-way tensor of extents (n_k, M) with mode 2 fastest, and
+    G[i][j] = sum_l X[j][l] X[i][l] = (X x_2 X)[i][j], with X itself as the
+    row-major n_k x M matrix B."""
     import torch
+    from .api import ttm_raw
     p = A.dim()
     order = [k] + [r for r in range(p) if r != k]
-    X = A.permute(*order).reshape(A.shape[k], -1).to(torch.float64)
-    return X @ X.transpose(0, 1).contiguous()
+    X = A.permute(*order).reshape(A.shape[k], -1).to(torch.float64).contiguous()
+    n, M = X.shape
+    G = torch.empty((n, n), dtype=torch.float64, device=X.device)
+    ttm_raw(2, 2, X, [n, M], [M, 1], [2, 1], X, [n, M], [2, 1], G, [n, n], [n, 1])
+    return G
 
 
 def factors_of(A, ranks):
     """Factors with orthonormal rows, U_k^T of shape r_k x n_k, in A's dtype."""
     import torch
-    out = []
     for k, r in enumerate(ranks):
         n = A.shape[k]
         if not 1 <= r <= n:
             raise ValueError(f"rank of mode {k + 1} must be in 1..{n}")
+    out = []
+    for k, r in enumerate(ranks):
         w, V = torch.linalg.eigh(_gram(A, k))           # ascending eigenvalues
         Ut = V[:, torch.argsort(w, descending=True)[:r]].transpose(0, 1)
         out.append(Ut.to(A.dtype).contiguous())
