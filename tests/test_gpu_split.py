@@ -351,3 +351,29 @@ def test_tn_tile_b_orientations_give_the_same_bits(gpu, oracle, dtype):
         err = rc.normwise_error(C_row.cpu().numpy(), c64, a, Bm.reshape(-1))
         _say(f"{label}: normwise {err:.3e}")
         assert err <= 4 * nq * rc.eps(dtype), (label, err)
+
+
+def test_float64_tiles_of_17_to_64_rows_vs_oracle(gpu, oracle):
+    """float64 products with 17..64 rows take the 128-column tiles (2 and 4
+    row tiles of 16, three blocks per CU) in both views: against the FP64
+    brute force, with ragged columns, slabs and contraction lengths."""
+    import torch
+    import ttm_amd
+    rng = np.random.default_rng(91)
+    cases = [([300, 70, 9], [1, 2, 3], 1, 17), ([300, 70, 9], [1, 2, 3], 2, 32), ([45, 33, 130], [3, 1, 2], 3, 33),
+             ([45, 33, 130], [2, 3, 1], 1, 48), ([130, 257, 5], [1, 2, 3], 2, 64), ([64, 12, 200], [3, 2, 1], 3, 64),
+             ([200, 9, 130], [1, 3, 2], 3, 40), ([45, 33, 13], [1, 2, 3], 3, 50), ([45, 33, 13], [1, 2, 3], 3, 20)]
+    for na, pia, q, m in cases:   # the last two: odd slab width 1485 (the padded-slab tile)
+        nq = na[q - 1]
+        for colmajor in (False, True):
+            a = rng.uniform(-1, 1, int(np.prod(na)))
+            Bm = rng.uniform(-1, 1, (m, nq))
+            A, Bt = torch.from_numpy(a).to(gpu), torch.from_numpy(Bm).to(gpu)
+            C, wa, nc, wc = _run(torch, ttm_amd, q, A, na, pia, Bt, colmajor, np.float64)
+            sym, splits = ttm_amd.last_kernel()
+            label = f"na={na} pi={pia} q={q} m={m} col={colmajor}"
+            _say(f"{label}: {sym} x{splits}")
+            assert sym.startswith("k_gemm<double, ") and ", 2, 1, 16, " in sym, (label, sym)
+            c64 = oracle.ttm_brute_f64(q, a, na, wa, Bm.reshape(-1).copy(), m, False, nc, wc)
+            err = rc.normwise_error(C.cpu().numpy(), c64, a, Bm.reshape(-1))
+            assert err <= 4 * nq * rc.eps(np.float64), (label, err)

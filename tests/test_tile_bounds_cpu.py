@@ -10,9 +10,9 @@ shorter than a vector.
 
 A GPU memory fault once came from interior B loads of 16- and 32-row tiles
 that re-read an index past the tile (the first version's fallback for
-vectors past B_VECS): the model runs exactly those loads.  The compiled
-configuration lists are read from the sources, so a new tile shape is
-covered as soon as it is added.  No GPU is needed.
+vectors past B_VECS): the model runs exactly those loads.  The grouped
+configuration list is read from the source and the k_gemm list mirrors
+launch_layout's instances.  No GPU is needed.
 """
 import itertools
 import pathlib
@@ -46,10 +46,12 @@ def gemm_configs():
     out = []
     for t, vec in (("float", 4), ("double", 2)):
         base = [(1, 4, 1, 16), (2, 4, 1, 16), (4, 4, 1, 16), (4, 4, 2, 16)]
+        if t == "double":
+            base += [(2, 2, 1, 16), (4, 2, 1, 16)]
         for tn in (True, False):
             cfgs = list(base)
             if tn:
-                cfgs.append((1, 4, 1, 32) if t == "float" else (1, 2, 1, 16))
+                cfgs += [(1, 4, 1, 32)] if t == "float" else [(1, 2, 1, 16)]
             for cfg, bcol in itertools.product(cfgs, (True, False)):
                 for pad in ((False,) if tn else (False, True)):
                     out.append((t, vec) + cfg + (tn, bcol, pad))

@@ -818,6 +818,12 @@ Config pick_config(const Fold &f, bool is_double)
     // double rows in the TN view: 128-column blocks (half the LDS, twice the
     // workgroups per CU): 0.64-0.68 -> 0.67-0.69 of 8 TB/s at m = 16
     if (f.m <= 16 && f.w == 1) return {1, 2, 1, 16};
+    // float64 blocks of 32 and 64 rows (both views): 128 columns.  With 256
+    // columns the 64-row block needs 83 KB of LDS and 388 registers (one
+    // wave per SIMD), the 32-row one 75 KB (two); with 128 columns 50 / 42 KB
+    // and 167 / 146 registers, three blocks per CU
+    if (is_double && f.m > 16 && f.m <= 32) return {2, 2, 1, 16};
+    if (is_double && f.m > 16 && f.m <= 64) return {4, 2, 1, 16};
     if (f.m <= 16) return {1, 4, 1, 16};
     if (f.m <= 32) return {2, 4, 1, 16};
     if (f.m <= 64) return {4, 4, 1, 16};
@@ -913,9 +919,16 @@ void launch_layout(const Fold &f, const Geom &g, const Split &sp, unsigned flags
             return;
         }
     }
-    if constexpr (sizeof(T) == 8 && TN) {
+    if constexpr (sizeof(T) == 8) {
         if (cfg.nt == 2) {
-            launch_cfg<T, 1, 2, 1, 16, TN, BCOL>(f, g, sp, flags, a, b, c, s);
+            if constexpr (TN) {
+                if (cfg.jt == 1) {
+                    launch_cfg<T, 1, 2, 1, 16, TN, BCOL>(f, g, sp, flags, a, b, c, s);
+                    return;
+                }
+            }
+            if (cfg.jt == 2) launch_cfg<T, 2, 2, 1, 16, TN, BCOL>(f, g, sp, flags, a, b, c, s);
+            else launch_cfg<T, 4, 2, 1, 16, TN, BCOL>(f, g, sp, flags, a, b, c, s);
             return;
         }
     }
