@@ -121,9 +121,10 @@ def gemm_tile_accesses(chk, vec, JT, NT, WM, BK, view, bcol, pad, *, ncols, nq, 
             th["b"].append((b_in, brow, bc))
         threads.append(th)
 
-    full = n0 + BN <= ncols and j0 + BM <= m and ((a_vec and b_vec_ok) if view == "TT" else (TN or a_vec or not pad))
+    rows_ok = j0 + BM <= m or not bcol or m % vec == 0
+    full = n0 + BN <= ncols and rows_ok and ((a_vec and b_vec_ok) if view == "TT" else (TN or a_vec or not pad))
     if pad and not a_vec:
-        full = n0 + BN <= ncols and j0 + BM <= m and all(nv == vec for th in threads for _, _, _, _, nv in th["a"])
+        full = n0 + BN <= ncols and rows_ok and all(nv == vec for th in threads for _, _, _, _, nv in th["a"])
 
     def load_full(l0):
         for th in threads:
@@ -134,7 +135,11 @@ def gemm_tile_accesses(chk, vec, JT, NT, WM, BK, view, bcol, pad, *, ncols, nq, 
             for b_in, brow, bc in th["b"]:
                 jj = bc if b_in else 0
                 ll = brow if b_in else 0
-                p = (j0 + jj + (l0 + ll) * m) if bcol else ((j0 + jj) * ld + l0 + ll)
+                j = j0 + jj
+                if bcol:
+                    p = (j if j + vec <= m else m - vec) + (l0 + ll) * m
+                else:
+                    p = (j if j < m else m - 1) * ld + l0 + ll
                 chk("b", b_off + p, vec)
 
     def load_guarded(l0):
@@ -257,7 +262,8 @@ def gemm_tile_accesses(chk, vec, JT, NT, WM, BK, view, bcol, pad, *, ncols, nq, 
 
 def gemm_problems(vec, tn, bm, bn, bk, pad):
     """(outer, nq, w, m) boundary problems for one tile shape."""
-    ms = sorted({1, 3, bm - 1, bm, bm + 1, 2 * bm + 5} - {0})
+    # ragged m with m % vec == 0 (vec, bm + vec, 2 bm - vec) runs the unguarded loop
+    ms = sorted({1, 3, vec, bm - 1, bm, bm + 1, bm + vec, 2 * bm - vec, 2 * bm + 5} - {0})
     nqs = sorted({1, vec - 1, bk, bk + 1, 2 * bk + 3, 64 + 5} - {0})
     if tn:
         shapes = [(outer, 1) for outer in (1, 5, bn, bn + 3)]

@@ -307,11 +307,17 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
     // TN rows and B at element alignment take element-aligned 16-byte loads
     // (every vector of a whole slice lies inside its row or column); the TT
     // tiles of the grouped kernels keep the aligned-only rule.
-    bool full = n0 + BN <= ncols && j0 + BM <= m &&
+    // Rows past m (the last row block of a ragged m) read row m - 1 of a
+    // row-major B, or the column's last whole vector of a column-major B
+    // (which needs m % VEC == 0, so that no vector is part in and part out
+    // of range), and their products are never stored: such blocks keep the
+    // unguarded loop.
+    bool const rows_ok = j0 + BM <= m || !BCOL || m % VEC == 0;
+    bool full = n0 + BN <= ncols && rows_ok &&
                 (VIEW == VIEW_TT ? (a_vec && b_vec_ok) : (TN || a_vec || !PAD));
     if constexpr (PAD) {
         if (!a_vec) {
-            bool whole = n0 + BN <= ncols && j0 + BM <= m;
+            bool whole = n0 + BN <= ncols && rows_ok;
 #pragma unroll
             for (int e = 0; e < A_PER; ++e) whole = whole && a_nv[e] == VEC;
             full = __syncthreads_and(whole ? 1 : 0) != 0;
@@ -463,7 +469,9 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ a, const T *__re
         for (int e = 0; e < B_PER; ++e) {
             uint64_t const jj = b_in[e] ? (uint64_t)b_col[e] : 0;
             uint64_t const ll = b_in[e] ? (uint64_t)b_row[e] : 0;
-            T const *bp = BCOL ? b + j0 + jj + (l0 + ll) * m : b + (j0 + jj) * ld + l0 + ll;
+            uint64_t const j = j0 + jj;
+            T const *bp = BCOL ? b + (j + VEC <= m ? j : m - VEC) + (l0 + ll) * m
+                               : b + (j < m ? j : m - 1) * ld + l0 + ll;
             if (VIEW == VIEW_TT || b_vec_ok) rb[S][e] = ldg<V>(bp);
             else rb[S][e] = ldg<typename Gt::vec_u>(bp);
         }
