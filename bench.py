@@ -571,6 +571,24 @@ def config4_order5(args):
             "bf16x3_min_frac": min(r["frac"] for r in rows if r["precision"] == "bf16x3")}
 
 
+def mid_m_products(args):
+    """A 1024x1024x512 first-order with 32, 48 and 64 rows of B (between the
+    sweep's 16 and 1024): the TN view (q = 1) and the NN view (q = 2), float32
+    and float64, each after a cache flush."""
+    import torch
+    dev = torch.device("cuda:0")
+    flush = Flusher(torch, dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0x77140008)
+    rows = []
+    for dtype in ("f32", "f64"):
+        for q in (1, 2):
+            for m in (32, 48, 64):
+                rows.append(_timed_product(torch, args, list(EXTENTS), [1, 2, 3], q, m, False, dtype, gen, flush))
+                torch.cuda.empty_cache()
+    return rows
+
+
 def split_and_gemv(args):
     """Products whose blocks alone would not fill the chip: the p = 1 GEMV
     (n_q = 2^24, m in {1, 16}), A [2^20, 4] with q = 1 at m = 1024, and
@@ -797,6 +815,7 @@ def main():
     extras = None
     if args.extras:
         extras = {"config1_latency": config1_latency(args), "config4_order5": config4_order5(args),
+                  "mid_m": mid_m_products(args),
                   "colmajor_b_m16": colb, "split_and_gemv": split_and_gemv(args), "odd_extents": odd_extents(args)}
 
     cpu = None
